@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X gate-application paths; prints ONE JSON line.
+
+Headline (`value`): 28-qubit complex128 statevector sweep, gates/s, on the
+workload "20 layers of U3 on every qubit + CZ ladder, then QFT-28" (1520
+logical gates) with the state resident in HBM.  A step applies the whole
+circuit once.  The line also carries:
+  roofline     : HBM bytes of the dominant kernel (the fused pass) per launch
+                 over its average duration, against 8 TB/s,
+  cpu_baseline : the NumPy oracle's pair updates on the host, on a bounded
+                 sample of the same 28-qubit workload,
+  mps          : two-site MPS updates/s at chi=64 (1000 qubits, brick-wall of
+                 exp(-0.1iH) gates) for the batched FP32-MFMA path and the
+                 plain FP64 path (one update per launch), when built.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mps]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, 'fpga-mps-simulator_amd')
+sys.path.insert(0, PKG)
+
+METRIC = "MPS 2q updates/s @chi=64 (FP32-MFMA vs FP64); 28q SV gates/s + HBM GB/s"
+HBM_PEAK_GBPS = 8000.0
+SV_QUBITS = 28
+SV_DEPTH = 20
+SV_SEED = 1234
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--no-mps', action='store_true', help='skip the MPS leg')
+    ap.add_argument('--cpu-gates', type=int, default=6,
+                    help='logical gates of the workload timed on the CPU')
+    return ap.parse_args()
+
+
+def sv_leg(steps: int, warmup: int):
+    import torch
+    from qsim_amd import workloads
+    from qsim_amd.sv import StateVector, gates_to_records, plan_passes
+
+    gates = workloads.sv_benchmark_circuit(SV_QUBITS, SV_DEPTH, SV_SEED)
+    recs, _ = gates_to_records(gates, SV_QUBITS)
+    npass = plan_passes(SV_QUBITS, recs)
+    sv = StateVector(SV_QUBITS, 0)
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        sv.apply_records(recs)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        sv.apply_records(recs)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    norm = sv.norm_sq()
+    bytes_per_pass = 2 * 16 * (1 << SV_QUBITS)
+    per_launch_s = gpu_ms / 1e3 / (steps * npass)
+    return {
+        'wall_s': wall, 'gpu_ms': gpu_ms, 'npass': npass, 'ngates': len(gates),
+        'per_launch_s': per_launch_s, 'bytes_per_pass': bytes_per_pass,
+        'norm_sq': norm,
+    }
+
+
+def cpu_leg(ngates: int):
+    """NumPy pair updates of the oracle on the first `ngates` logical gates at 28 qubits."""
+    import numpy as np
+    from oracle import sv_ref
+    from qsim_amd import workloads
+    gates = workloads.sv_benchmark_circuit(SV_QUBITS, SV_DEPTH, SV_SEED)
+    # a bounded sample: U3s of the first layer and two CZs, plus one cp from the QFT
+    sample = gates[:ngates - 3] + [g for g in gates if g[0] == 'cz'][:2] + \
+        [g for g in gates if g[0] == 'cp'][:1]
+    state = np.zeros(1 << SV_QUBITS, dtype=np.complex128)
+    state[0] = 1.0
+    ops = sv_ref.lower(sample)
+    t0 = time.perf_counter()
+    sv_ref.apply_kernel_ops(state, SV_QUBITS, ops)
+    dt = time.perf_counter() - t0
+    return {'value': len(sample) / dt, 'unit': 'gates/s', 'cores': 1, 'kind': 'port',
+            'sample': f'{len(sample)} logical gates ({len(ops)} kernel ops after the '
+                      f'reference lowering) of the 28q workload, NumPy pair updates, '
+                      f'{dt:.1f} s'}
+
+
+def traffic_from_profiles():
+    """HBM bytes per pass launch from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, 'profiles', 'sv_pass_28q_pmc.json')
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get('hbm_bytes_per_launch')
+    except Exception:
+        return None
+
+
+def mps_leg():
+    try:
+        from qsim_amd import mps_bench
+    except ImportError:
+        return None
+    return mps_bench.run()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1',
+                          'n_gpus': args.gpus}))
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({'error': 'no GPU visible'}))
+        sys.exit(2)
+
+    sv = sv_leg(args.steps, args.warmup)
+    value = sv['ngates'] * args.steps / (sv['gpu_ms'] / 1e3)
+    achieved = sv['bytes_per_pass'] / sv['per_launch_s'] / 1e9
+    out = {
+        'metric': METRIC,
+        'value': value,
+        'unit': 'gates/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': sv['gpu_ms'] / args.steps,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'c128',
+        'data': 'synthetic',
+        'config': {'workload': '28q statevector: 20x[U3 all qubits + CZ ladder] + QFT-28 '
+                               '(1520 gates), complex128, state resident in HBM',
+                   'qubits': SV_QUBITS, 'gates': sv['ngates'], 'passes_per_step': sv['npass'],
+                   'hbm_gbps_whole_step': sv['npass'] * sv['bytes_per_pass'] * args.steps /
+                   (sv['gpu_ms'] / 1e3) / 1e9},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBPS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBPS,
+                     'traffic': traffic_from_profiles(),
+                     'kernel': 'sv_pass_kernel<4>',
+                     'per_launch_us': sv['per_launch_s'] * 1e6},
+        'check': {'norm_sq_after': sv['norm_sq']},
+    }
+    if not args.no_mps:
+        m = mps_leg()
+        if m is not None:
+            out['mps'] = m
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_leg(args.cpu_gates)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

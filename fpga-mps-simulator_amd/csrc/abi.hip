@@ -1,0 +1,317 @@
+// extern "C" entry points of libqsim_hip.so (declared in include/qsim.h).
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <sstream>
+#include <string>
+
+#include "qsim_internal.h"
+
+namespace qs {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+namespace {
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int check_state(const void* state, int nq) {
+    QS_REQUIRE(state != nullptr, QS_E_ARG, "state pointer is null");
+    QS_REQUIRE(nq >= 1 && nq <= SV_MAX_QUBITS, QS_E_RANGE, "nq=%d outside [1, %d]", nq,
+               SV_MAX_QUBITS);
+    return QS_OK;
+}
+
+// 2x2 of a one-qubit opcode of the statevector kernel (opt_kernel.cpp:257-347), with
+// the parameters already decoded from their float32 words.
+bool words_matrix(int op, double p0, double p1, double p2, double* m, bool* diag) {
+    const double h = 0.7071067811865476;
+    std::memset(m, 0, sizeof(double) * 8);
+    auto set = [&](double a, double b, double c, double d, double e, double f, double g,
+                   double k) {
+        m[0] = a; m[1] = b; m[2] = c; m[3] = d; m[4] = e; m[5] = f; m[6] = g; m[7] = k;
+    };
+    *diag = false;
+    switch (op) {
+        case 1: set(h, 0, h, 0, h, 0, -h, 0); break;
+        case 2: set(0, 0, 1, 0, 1, 0, 0, 0); break;
+        case 3: set(0, 0, 0, -1, 0, 1, 0, 0); break;
+        case 4: set(1, 0, 0, 0, 0, 0, -1, 0); *diag = true; break;
+        case 5: set(1, 0, 0, 0, 0, 0, 0, 1); *diag = true; break;
+        case 6: set(1, 0, 0, 0, 0, 0, h, h); *diag = true; break;
+        case 7: set(1, 0, 0, 0, 0, 0, 0, -1); *diag = true; break;
+        case 8: set(1, 0, 0, 0, 0, 0, h, -h); *diag = true; break;
+        case 9: set(0.5, 0.5, 0.5, -0.5, 0.5, -0.5, 0.5, 0.5); break;
+        case 10: set(0.5, -0.5, 0.5, 0.5, 0.5, 0.5, 0.5, -0.5); break;
+        case 11: {
+            const double c = std::cos(p0 * 0.5), s = std::sin(p0 * 0.5);
+            set(c, 0, 0, -s, 0, -s, c, 0);
+            break;
+        }
+        case 12: {
+            const double c = std::cos(p0 * 0.5), s = std::sin(p0 * 0.5);
+            set(c, 0, -s, 0, s, 0, c, 0);
+            break;
+        }
+        case 13: {
+            const double c = std::cos(p0 * 0.5), s = std::sin(p0 * 0.5);
+            set(c, -s, 0, 0, 0, 0, c, s);
+            *diag = true;
+            break;
+        }
+        case 14: set(1, 0, 0, 0, 0, 0, std::cos(p0), std::sin(p0)); *diag = true; break;
+        case 15:
+            set(h, 0, -h * std::cos(p1), -h * std::sin(p1), h * std::cos(p0), h * std::sin(p0),
+                h * std::cos(p0 + p1), h * std::sin(p0 + p1));
+            break;
+        case 16: {
+            const double ct = std::cos(p0 * 0.5), st = std::sin(p0 * 0.5);
+            set(ct, 0, -st * std::cos(p2), -st * std::sin(p2), st * std::cos(p1),
+                st * std::sin(p1), ct * std::cos(p1 + p2), ct * std::sin(p1 + p2));
+            break;
+        }
+        default: return false;
+    }
+    return true;
+}
+
+double f32_word(int32_t w) {
+    float f;
+    std::memcpy(&f, &w, sizeof(f));
+    return (double)f;
+}
+
+// Uploads the plan's tables with a stream-ordered allocation and launches it.
+int run_plan(double2* state, int nq, const SvPlan& plan, hipStream_t stream) {
+    if (plan.passes.empty()) return QS_OK;
+    const size_t pbytes = sv_blob_offset(plan.passes.size() * sizeof(DevPass));
+    const size_t sbytes = sv_blob_offset(plan.stages.size() * sizeof(DevStage));
+    const size_t gbytes = std::max<size_t>(plan.gates.size(), 1) * sizeof(DevGate);
+    std::vector<char> blob(pbytes + sbytes + gbytes, 0);
+    std::memcpy(blob.data(), plan.passes.data(), plan.passes.size() * sizeof(DevPass));
+    std::memcpy(blob.data() + pbytes, plan.stages.data(), plan.stages.size() * sizeof(DevStage));
+    std::memcpy(blob.data() + pbytes + sbytes, plan.gates.data(),
+                plan.gates.size() * sizeof(DevGate));
+    void* dev = nullptr;
+    QS_HIP(hipMallocAsync(&dev, blob.size(), stream));
+    // Pageable source: the runtime stages the bytes before returning.
+    QS_HIP(hipMemcpyAsync(dev, blob.data(), blob.size(), hipMemcpyHostToDevice, stream));
+    int st = sv_launch_plan(state, nq, plan, dev, stream);
+    QS_HIP(hipFreeAsync(dev, stream));
+    return st;
+}
+
+}  // namespace
+}  // namespace qs
+
+using namespace qs;
+
+extern "C" {
+
+int qs_abi_version(void) { return QS_ABI_VERSION; }
+
+const char* qs_last_error(void) { return g_last_error.c_str(); }
+
+int qs_sv_max_qubits(void) { return SV_MAX_QUBITS; }
+
+int qs_init(int device) {
+    clear_error();
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    QS_REQUIRE(e == hipSuccess && count > 0, QS_E_NODEV, "no HIP device visible (%s)",
+               hipGetErrorString(e));
+    QS_REQUIRE(device >= 0 && device < count, QS_E_NODEV, "device %d not in [0, %d)", device,
+               count);
+    QS_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    QS_HIP(hipGetDeviceProperties(&prop, device));
+    QS_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, QS_E_NODEV,
+               "device %d is %s; this library is built for gfx950 only", device,
+               prop.gcnArchName);
+    return QS_OK;
+}
+
+int qs_sv_set_zero(void* state, int nq, void* stream) {
+    clear_error();
+    int st = check_state(state, nq);
+    if (st) return st;
+    return sv_fill_zero(static_cast<double2*>(state), nq, as_stream(stream));
+}
+
+int qs_sv_apply(void* state, int nq, const qs_gate* gates, int ng, void* stream) {
+    clear_error();
+    int st = check_state(state, nq);
+    if (st) return st;
+    QS_REQUIRE(ng >= 0, QS_E_ARG, "negative gate count");
+    if (ng == 0) return QS_OK;
+    QS_REQUIRE(gates != nullptr, QS_E_ARG, "gate array is null");
+    SvPlan plan;
+    st = sv_build_plan(nq, gates, ng, &plan);
+    if (st) return st;
+    return run_plan(static_cast<double2*>(state), nq, plan, as_stream(stream));
+}
+
+int qs_sv_apply_words(void* state, int nq, const int32_t* words, int num_gates, void* stream) {
+    clear_error();
+    int st = check_state(state, nq);
+    if (st) return st;
+    QS_REQUIRE(num_gates >= 0, QS_E_ARG, "negative gate count");
+    if (num_gates == 0) return QS_OK;
+    QS_REQUIRE(words != nullptr, QS_E_ARG, "gate_sequence is null");
+    std::vector<qs_gate> gates;
+    gates.reserve(num_gates);
+    for (int i = 0; i < num_gates; ++i) {
+        const int32_t* w = words + 8 * (size_t)i;
+        const int op = w[0];
+        qs_gate g;
+        std::memset(&g, 0, sizeof(g));
+        if (op == 0) continue;
+        if (op == 17 || op == 18) {
+            QS_REQUIRE(w[1] >= 0 && w[1] < nq && w[2] >= 0 && w[2] < nq && w[1] != w[2],
+                       QS_E_ARG, "gate %d: bad qubits (%d, %d)", i, w[1], w[2]);
+            if (op == 17) {          // CX: X on q1 controlled by q0
+                g.kind = QS_GATE_U1;
+                g.q[0] = w[2];
+                g.ctrl_mask = 1ull << w[1];
+                g.m[2] = 1.0;
+                g.m[4] = 1.0;
+            } else {                 // CZ
+                g.kind = QS_GATE_DIAG2;
+                g.q[0] = w[1];
+                g.q[1] = w[2];
+                g.m[0] = g.m[2] = g.m[4] = 1.0;
+                g.m[6] = -1.0;
+            }
+            gates.push_back(g);
+            continue;
+        }
+        double m[8];
+        bool diag = false;
+        QS_REQUIRE(words_matrix(op, f32_word(w[4]), f32_word(w[5]), f32_word(w[6]), m, &diag),
+                   QS_E_ARG, "gate %d: unknown opcode %d", i, op);
+        QS_REQUIRE(w[1] >= 0 && w[1] < nq, QS_E_ARG, "gate %d: qubit %d out of range", i, w[1]);
+        g.q[0] = w[1];
+        if (diag) {
+            g.kind = QS_GATE_DIAG1;
+            g.m[0] = m[0]; g.m[1] = m[1]; g.m[2] = m[6]; g.m[3] = m[7];
+        } else {
+            g.kind = QS_GATE_U1;
+            std::memcpy(g.m, m, sizeof(m));
+        }
+        gates.push_back(g);
+    }
+    if (gates.empty()) return QS_OK;
+    SvPlan plan;
+    st = sv_build_plan(nq, gates.data(), (int)gates.size(), &plan);
+    if (st) return st;
+    return run_plan(static_cast<double2*>(state), nq, plan, as_stream(stream));
+}
+
+int qs_sv_plan_passes(int nq, const qs_gate* gates, int ng, int* npasses) {
+    clear_error();
+    QS_REQUIRE(npasses != nullptr, QS_E_ARG, "npasses is null");
+    QS_REQUIRE(nq >= 1 && nq <= SV_MAX_QUBITS, QS_E_RANGE, "nq=%d outside [1, %d]", nq,
+               SV_MAX_QUBITS);
+    QS_REQUIRE(ng >= 0 && (ng == 0 || gates), QS_E_ARG, "bad gate array");
+    SvPlan plan;
+    int st = sv_build_plan(nq, gates, ng, &plan);
+    if (st) return st;
+    *npasses = (int)plan.passes.size();
+    return QS_OK;
+}
+
+int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
+                    size_t* needed) {
+    clear_error();
+    QS_REQUIRE(nq >= 1 && nq <= SV_MAX_QUBITS, QS_E_RANGE, "nq=%d outside [1, %d]", nq,
+               SV_MAX_QUBITS);
+    QS_REQUIRE(ng >= 0 && (ng == 0 || gates), QS_E_ARG, "bad gate array");
+    SvPlan plan;
+    int st = sv_build_plan(nq, gates, ng, &plan);
+    if (st) return st;
+    std::ostringstream os;
+    os.precision(17);
+    os << "{\"passes\":[";
+    for (size_t p = 0; p < plan.passes.size(); ++p) {
+        const DevPass& P = plan.passes[p];
+        os << (p ? "," : "") << "{\"K\":" << P.K << ",\"R\":" << P.R << ",\"tq\":[";
+        for (int i = 0; i < P.K; ++i) os << (i ? "," : "") << P.tq[i];
+        os << "],\"stages\":[";
+        for (int s = 0; s < P.nstage; ++s) {
+            const DevStage& S = plan.stages[P.stage0 + s];
+            os << (s ? "," : "") << "{\"regs\":[";
+            for (int i = 0; i < P.R; ++i) os << (i ? "," : "") << S.regs[i];
+            os << "],\"gates\":[";
+            for (int gi = S.g0; gi < S.g1; ++gi) {
+                const DevGate& G = plan.gates[P.gate0 + gi];
+                os << (gi > S.g0 ? "," : "") << "{\"kind\":" << G.kind << ",\"r0\":" << G.r0
+                   << ",\"r1\":" << G.r1 << ",\"qa\":" << G.qa << ",\"qb\":" << G.qb
+                   << ",\"cmask\":" << G.cmask << ",\"m\":[";
+                for (int k = 0; k < 32; ++k) os << (k ? "," : "") << G.m[k];
+                os << "]}";
+            }
+            os << "]}";
+        }
+        os << "]}";
+    }
+    os << "]}";
+    const std::string s = os.str();
+    if (needed) *needed = s.size() + 1;
+    if (buf && cap > s.size()) std::memcpy(buf, s.c_str(), s.size() + 1);
+    return QS_OK;
+}
+
+int qs_sv_norm_sq(const void* state, int nq, double* out_host, void* stream) {
+    clear_error();
+    int st = check_state(state, nq);
+    if (st) return st;
+    QS_REQUIRE(out_host != nullptr, QS_E_ARG, "output pointer is null");
+    return sv_norm_sq(static_cast<const double2*>(state), nq, out_host, as_stream(stream));
+}
+
+int qs_sv_scale(void* state, int nq, double re, double im, void* stream) {
+    clear_error();
+    int st = check_state(state, nq);
+    if (st) return st;
+    return sv_scale(static_cast<double2*>(state), nq, re, im, as_stream(stream));
+}
+
+int qs_sv_reset_qubit(void* state, int nq, int q, void* stream) {
+    clear_error();
+    int st = check_state(state, nq);
+    if (st) return st;
+    QS_REQUIRE(q >= 0 && q < nq, QS_E_ARG, "qubit %d out of range [0, %d)", q, nq);
+    double2* s = static_cast<double2*>(state);
+    hipStream_t hs = as_stream(stream);
+    st = sv_project(s, nq, q, hs);
+    if (st) return st;
+    double nsq = 0.0;
+    st = sv_norm_sq(s, nq, &nsq, hs);
+    if (st) return st;
+    const double nb = std::sqrt(nsq);
+    if (nb < 1e-12) return sv_fill_zero(s, nq, hs);
+    return sv_scale(s, nq, 1.0 / nb, 0.0, hs);
+}
+
+int qs_sv_probabilities(const void* state, int nq, double* probs, void* stream) {
+    clear_error();
+    int st = check_state(state, nq);
+    if (st) return st;
+    QS_REQUIRE(probs != nullptr, QS_E_ARG, "probs pointer is null");
+    return sv_probs(static_cast<const double2*>(state), nq, probs, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,425 @@
+// Statevector kernels for gfx950: fused tile passes, fill, norm, scale, reset.
+//
+// The reference runs one full-state pass per kernel op (opt_kernel.cpp:71-195:
+// dense 2x2 over 2^(n-1) pairs, CX swap and CZ negate over 2^(n-2) indices).
+// Here one launch = one HBM pass that applies many gates:
+//   * the pass owns a tile of K qubits (qubits 0..3 always among them, so every
+//     workgroup streams 256-byte runs); workgroup b owns the 2^K amplitudes that
+//     share the bits of b on the other nq-K qubits,
+//   * each thread keeps 2^R amplitudes in registers; a "stage" names which R tile
+//     qubits are the register qubits, gates on them run in registers, and a change
+//     of stage is a transpose through LDS,
+//   * diagonal gates and control qubits need no register placement: their bits are
+//     read off each amplitude's global index.
+#include "qsim_internal.h"
+
+namespace qs {
+
+namespace {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+
+__device__ __forceinline__ double2 cmadd(double2 acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+    return acc;
+}
+
+__device__ __forceinline__ int lds_slot(int l) { return l + (l >> 4); }
+
+constexpr int kLdsSlots = (1 << SV_TILE_BITS) + ((1 << SV_TILE_BITS) >> 4);
+
+// Per-stage addressing: local index and global index of register j of this thread.
+template <int R>
+struct Layout {
+    int loc_t;                 // local index bits contributed by the thread id
+    uint64_t glob_t;           // global bits of base | thread part
+    int lreg[R];               // local bit of register bit i
+    uint64_t greg[R];          // global bit of register bit i
+
+    __device__ __forceinline__ int loc(int j) const {
+        int l = loc_t;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (j & (1 << i)) l |= lreg[i];
+        return l;
+    }
+    __device__ __forceinline__ uint64_t glob(int j) const {
+        uint64_t g = glob_t;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (j & (1 << i)) g |= greg[i];
+        return g;
+    }
+};
+
+struct Regs {
+    int v[4];
+};
+
+template <int R>
+__device__ __forceinline__ void make_layout(Layout<R>& L, const Regs regs, const DevPass* P,
+                                            uint64_t base, int t) {
+    int regmask = 0;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        L.lreg[i] = 1 << regs.v[i];
+        L.greg[i] = 1ull << P->tq[regs.v[i]];
+        regmask |= 1 << regs.v[i];
+    }
+    // Thread bits fill the non-register local positions in ascending order.
+    int loc = 0;
+    uint64_t g = base;
+    int tb = 0;
+    const int K = P->K;
+    for (int p = 0; p < K; ++p) {
+        if ((regmask >> p) & 1) continue;
+        const int bit = (t >> tb) & 1;
+        loc |= bit << p;
+        g |= (uint64_t)bit << P->tq[p];
+        ++tb;
+    }
+    L.loc_t = loc;
+    L.glob_t = g;
+}
+
+template <int R, int B>
+__device__ __forceinline__ void apply_u1_bit(double2 (&a)[1 << R], const DevGate& G,
+                                             const Layout<R>& L) {
+    const double2 m00 = make_double2(G.m[0], G.m[1]), m01 = make_double2(G.m[2], G.m[3]);
+    const double2 m10 = make_double2(G.m[4], G.m[5]), m11 = make_double2(G.m[6], G.m[7]);
+    const uint64_t cm = G.cmask;
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        if (j & (1 << B)) continue;
+        const int j1 = j | (1 << B);
+        if (cm != 0 && (L.glob(j) & cm) != cm) continue;
+        const double2 x0 = a[j], x1 = a[j1];
+        a[j] = cmadd(cmul(m00, x0), m01, x1);
+        a[j1] = cmadd(cmul(m10, x0), m11, x1);
+    }
+}
+
+template <int R, int B0, int B1>
+__device__ __forceinline__ void apply_u2_bits(double2 (&a)[1 << R], const DevGate& G,
+                                              const Layout<R>& L) {
+    const uint64_t cm = G.cmask;
+    const double2* m = reinterpret_cast<const double2*>(G.m);
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        if (j & ((1 << B0) | (1 << B1))) continue;
+        if (cm != 0 && (L.glob(j) & cm) != cm) continue;
+        constexpr int b0 = 1 << B0, b1 = 1 << B1;
+        const double2 x0 = a[j], x1 = a[j | b1], x2 = a[j | b0], x3 = a[j | b0 | b1];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double2 y = cmadd(cmadd(cmadd(cmul(m[4 * r], x0), m[4 * r + 1], x1),
+                                          m[4 * r + 2], x2), m[4 * r + 3], x3);
+            if (r == 0) a[j] = y;
+            else if (r == 1) a[j | b1] = y;
+            else if (r == 2) a[j | b0] = y;
+            else a[j | b0 | b1] = y;
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void apply_u1(double2 (&a)[1 << R], const DevGate& G,
+                                         const Layout<R>& L) {
+    switch (G.r0) {
+        case 0: apply_u1_bit<R, 0>(a, G, L); break;
+        case 1: if constexpr (R > 1) apply_u1_bit<R, (R > 1 ? 1 : 0)>(a, G, L); break;
+        case 2: if constexpr (R > 2) apply_u1_bit<R, (R > 2 ? 2 : 0)>(a, G, L); break;
+        case 3: if constexpr (R > 3) apply_u1_bit<R, (R > 3 ? 3 : 0)>(a, G, L); break;
+        default: break;
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void apply_u2(double2 (&a)[1 << R], const DevGate& G,
+                                         const Layout<R>& L) {
+    constexpr int S1 = R > 1 ? 1 : 0, S2 = R > 2 ? 2 : 0, S3 = R > 3 ? 3 : 0;
+    const int code = G.r0 * 4 + G.r1;
+    if constexpr (R >= 2) {
+        switch (code) {
+            case 0 * 4 + 1: apply_u2_bits<R, 0, S1>(a, G, L); return;
+            case 1 * 4 + 0: apply_u2_bits<R, S1, 0>(a, G, L); return;
+            default: break;
+        }
+    }
+    if constexpr (R >= 3) {
+        switch (code) {
+            case 0 * 4 + 2: apply_u2_bits<R, 0, S2>(a, G, L); return;
+            case 2 * 4 + 0: apply_u2_bits<R, S2, 0>(a, G, L); return;
+            case 1 * 4 + 2: apply_u2_bits<R, S1, S2>(a, G, L); return;
+            case 2 * 4 + 1: apply_u2_bits<R, S2, S1>(a, G, L); return;
+            default: break;
+        }
+    }
+    if constexpr (R >= 4) {
+        switch (code) {
+            case 0 * 4 + 3: apply_u2_bits<R, 0, S3>(a, G, L); return;
+            case 3 * 4 + 0: apply_u2_bits<R, S3, 0>(a, G, L); return;
+            case 1 * 4 + 3: apply_u2_bits<R, S1, S3>(a, G, L); return;
+            case 3 * 4 + 1: apply_u2_bits<R, S3, S1>(a, G, L); return;
+            case 2 * 4 + 3: apply_u2_bits<R, S2, S3>(a, G, L); return;
+            case 3 * 4 + 2: apply_u2_bits<R, S3, S2>(a, G, L); return;
+            default: break;
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void apply_diag(double2 (&a)[1 << R], const DevGate& G,
+                                           const Layout<R>& L) {
+    const uint64_t cm = G.cmask;
+    const int qa = G.qa, qb = G.qb;
+    if (G.kind == DG_SIGN2) {
+#pragma unroll
+        for (int j = 0; j < (1 << R); ++j) {
+            const uint64_t g = L.glob(j);
+            const bool hit = ((g >> qa) & (g >> qb) & 1ull) != 0;
+            if (hit && (cm == 0 || (g & cm) == cm)) a[j] = make_double2(-a[j].x, -a[j].y);
+        }
+        return;
+    }
+    if (G.kind == DG_DIAG1) {
+        const double2 d0 = make_double2(G.m[0], G.m[1]), d1 = make_double2(G.m[2], G.m[3]);
+#pragma unroll
+        for (int j = 0; j < (1 << R); ++j) {
+            const uint64_t g = L.glob(j);
+            if (cm != 0 && (g & cm) != cm) continue;
+            a[j] = cmul(((g >> qa) & 1ull) ? d1 : d0, a[j]);
+        }
+        return;
+    }
+    // DG_DIAG2: index = 2*bit(qa) + bit(qb)
+    const double2 d00 = make_double2(G.m[0], G.m[1]), d01 = make_double2(G.m[2], G.m[3]);
+    const double2 d10 = make_double2(G.m[4], G.m[5]), d11 = make_double2(G.m[6], G.m[7]);
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        const uint64_t g = L.glob(j);
+        if (cm != 0 && (g & cm) != cm) continue;
+        const bool ba = (g >> qa) & 1ull, bb = (g >> qb) & 1ull;
+        const double2 d = ba ? (bb ? d11 : d10) : (bb ? d01 : d00);
+        a[j] = cmul(d, a[j]);
+    }
+}
+
+template <int R>
+__global__ void __launch_bounds__(1 << (SV_TILE_BITS - SV_REG_BITS))
+sv_pass_kernel(double2* __restrict__ state, int nq, const DevPass* __restrict__ P,
+               const DevStage* __restrict__ stages, const DevGate* __restrict__ gates) {
+    constexpr int NR = 1 << R;
+    __shared__ double2 lds[kLdsSlots];
+
+    const int t = threadIdx.x;
+    const int K = P->K;
+    const int nstage = P->nstage;
+    const DevStage* st = stages + P->stage0;
+    const DevGate* gt = gates + P->gate0;
+    // Block base: bits of blockIdx.x deposited into the non-tile qubits, ascending.
+    uint64_t base = 0;
+    if (!P->tile_is_all) {
+        const uint64_t tmask = P->tile_mask;
+        uint64_t b = blockIdx.x;
+        for (int q = 0; q < nq && b; ++q) {
+            if ((tmask >> q) & 1ull) continue;
+            base |= (b & 1ull) << q;
+            b >>= 1;
+        }
+    }
+
+    // Load layout: register bits are the top R local positions; thread bits are the
+    // low ones, so lanes walk consecutive addresses on qubits 0..3.
+    Regs load_regs{{0, 0, 0, 0}};
+#pragma unroll
+    for (int i = 0; i < R; ++i) load_regs.v[i] = K - R + i;
+    Layout<R> Lcur;
+    make_layout<R>(Lcur, load_regs, P, base, t);
+    Regs cur = load_regs;
+
+    double2 a[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) a[j] = state[Lcur.glob(j)];
+
+    for (int s = 0; s < nstage; ++s) {
+        Regs want{{0, 0, 0, 0}};
+        bool same = true;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            want.v[i] = st[s].regs[i];
+            same = same && (want.v[i] == cur.v[i]);
+        }
+        if (!same) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j) lds[lds_slot(Lcur.loc(j))] = a[j];
+            __syncthreads();
+            make_layout<R>(Lcur, want, P, base, t);
+            cur = want;
+#pragma unroll
+            for (int j = 0; j < NR; ++j) a[j] = lds[lds_slot(Lcur.loc(j))];
+            __syncthreads();
+        }
+        const int g1 = st[s].g1;
+        for (int gi = st[s].g0; gi < g1; ++gi) {
+            const DevGate& G = gt[gi];
+            switch (G.kind) {
+                case DG_U1: apply_u1<R>(a, G, Lcur); break;
+                case DG_U2: apply_u2<R>(a, G, Lcur); break;
+                default: apply_diag<R>(a, G, Lcur); break;
+            }
+        }
+    }
+
+    bool back = false;
+#pragma unroll
+    for (int i = 0; i < R; ++i) back = back || (cur.v[i] != load_regs.v[i]);
+    if (back) {
+#pragma unroll
+        for (int j = 0; j < NR; ++j) lds[lds_slot(Lcur.loc(j))] = a[j];
+        __syncthreads();
+        make_layout<R>(Lcur, load_regs, P, base, t);
+#pragma unroll
+        for (int j = 0; j < NR; ++j) a[j] = lds[lds_slot(Lcur.loc(j))];
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) state[Lcur.glob(j)] = a[j];
+}
+
+__global__ void sv_fill_zero_kernel(double2* __restrict__ state, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        state[i] = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
+}
+
+__global__ void sv_scale_kernel(double2* __restrict__ state, uint64_t n, double2 f) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        state[i] = cmul(f, state[i]);
+}
+
+// Zero the amplitudes whose bit q is set.
+__global__ void sv_project_kernel(double2* __restrict__ state, uint64_t n, int q) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        if ((i >> q) & 1ull) state[i] = make_double2(0.0, 0.0);
+}
+
+__global__ void sv_probs_kernel(const double2* __restrict__ state, uint64_t n,
+                                double* __restrict__ p) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double2 v = state[i];
+        p[i] = fma(v.x, v.x, v.y * v.y);
+    }
+}
+
+constexpr int kNormBlocks = 1024;
+constexpr int kNormThreads = 256;
+
+// Per-block partial sums of |a|^2 (fixed grid, fixed order: deterministic).
+__global__ void __launch_bounds__(kNormThreads)
+sv_norm_partial_kernel(const double2* __restrict__ state, uint64_t n, double* __restrict__ part) {
+    __shared__ double red[kNormThreads];
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double2 v = state[i];
+        acc = fma(v.x, v.x, fma(v.y, v.y, acc));
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kNormThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+int grid_for(uint64_t n, int threads) {
+    uint64_t b = (n + threads - 1) / threads;
+    if (b > 65536) b = 65536;
+    return (int)(b == 0 ? 1 : b);
+}
+
+}  // namespace
+
+int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_blob,
+                   hipStream_t stream) {
+    const char* blob = reinterpret_cast<const char*>(dev_blob);
+    const size_t pbytes = sv_blob_offset(plan.passes.size() * sizeof(DevPass));
+    const size_t sbytes = sv_blob_offset(plan.stages.size() * sizeof(DevStage));
+    const DevPass* dpasses = reinterpret_cast<const DevPass*>(blob);
+    const DevStage* dstages = reinterpret_cast<const DevStage*>(blob + pbytes);
+    const DevGate* dgates = reinterpret_cast<const DevGate*>(blob + pbytes + sbytes);
+    for (size_t p = 0; p < plan.passes.size(); ++p) {
+        const DevPass& P = plan.passes[p];
+        const DevPass* dP = dpasses + p;
+        const unsigned blocks = P.tile_is_all ? 1u : (1u << (nq - P.K));
+        const unsigned threads = 1u << (P.K - P.R);
+        switch (P.R) {
+            case 1: hipLaunchKernelGGL(sv_pass_kernel<1>, dim3(blocks), dim3(threads), 0, stream,
+                                       state, nq, dP, dstages, dgates); break;
+            case 2: hipLaunchKernelGGL(sv_pass_kernel<2>, dim3(blocks), dim3(threads), 0, stream,
+                                       state, nq, dP, dstages, dgates); break;
+            case 3: hipLaunchKernelGGL(sv_pass_kernel<3>, dim3(blocks), dim3(threads), 0, stream,
+                                       state, nq, dP, dstages, dgates); break;
+            default: hipLaunchKernelGGL(sv_pass_kernel<4>, dim3(blocks), dim3(threads), 0, stream,
+                                        state, nq, dP, dstages, dgates); break;
+        }
+        QS_HIP(hipGetLastError());
+    }
+    return QS_OK;
+}
+
+int sv_fill_zero(double2* state, int nq, hipStream_t stream) {
+    const uint64_t n = 1ull << nq;
+    hipLaunchKernelGGL(sv_fill_zero_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, state, n);
+    QS_HIP(hipGetLastError());
+    return QS_OK;
+}
+
+int sv_scale(double2* state, int nq, double re, double im, hipStream_t stream) {
+    const uint64_t n = 1ull << nq;
+    hipLaunchKernelGGL(sv_scale_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, state, n,
+                       make_double2(re, im));
+    QS_HIP(hipGetLastError());
+    return QS_OK;
+}
+
+int sv_project(double2* state, int nq, int q, hipStream_t stream) {
+    const uint64_t n = 1ull << nq;
+    hipLaunchKernelGGL(sv_project_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, state, n, q);
+    QS_HIP(hipGetLastError());
+    return QS_OK;
+}
+
+int sv_probs(const double2* state, int nq, double* p, hipStream_t stream) {
+    const uint64_t n = 1ull << nq;
+    hipLaunchKernelGGL(sv_probs_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, state, n, p);
+    QS_HIP(hipGetLastError());
+    return QS_OK;
+}
+
+int sv_norm_sq(const double2* state, int nq, double* out, hipStream_t stream) {
+    const uint64_t n = 1ull << nq;
+    double* part = nullptr;
+    QS_HIP(hipMallocAsync(reinterpret_cast<void**>(&part), kNormBlocks * sizeof(double), stream));
+    hipLaunchKernelGGL(sv_norm_partial_kernel, dim3(kNormBlocks), dim3(kNormThreads), 0, stream,
+                       state, n, part);
+    QS_HIP(hipGetLastError());
+    std::vector<double> h(kNormBlocks);
+    QS_HIP(hipMemcpyAsync(h.data(), part, kNormBlocks * sizeof(double), hipMemcpyDeviceToHost,
+                          stream));
+    QS_HIP(hipFreeAsync(part, stream));
+    QS_HIP(hipStreamSynchronize(stream));
+    double s = 0.0;
+    for (double v : h) s += v;
+    *out = s;
+    return QS_OK;
+}
+
+}  // namespace qs

@@ -1,0 +1,57 @@
+"""Build recipe for libqsim_hip.so (hipcc, gfx950 only).
+
+    python -m qsim_amd.build          (from fpga-mps-simulator_amd/)
+
+The shared library is written next to this file, in-tree, so that it travels
+with the source tree to the machine that runs it.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ROOT_DIR = os.path.dirname(PKG_DIR)
+CSRC = os.path.join(ROOT_DIR, 'csrc')
+LIB_NAME = 'libqsim_hip.so'
+LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
+
+SOURCES = ['abi.hip', 'sv_kernels.hip', 'sv_plan.cpp', 'mps_kernels.hip']
+
+HIPCC_FLAGS = [
+    '--offload-arch=gfx950',
+    '-O3', '-std=c++17', '-fPIC', '-shared',
+    '-Wall', '-Wno-unused-function',
+    # Packed FP32 ops (v_pk_*_f32) are kept out of the generated code.
+    '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+]
+
+
+def hipcc() -> str:
+    path = shutil.which('hipcc') or '/opt/rocm/bin/hipcc'
+    if not os.path.exists(path):
+        raise RuntimeError('hipcc not found: the HIP library cannot be built')
+    return path
+
+
+def build(verbose: bool = False) -> str:
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = LIB_PATH + '.tmp'
+    cmd = [hipcc()] + HIPCC_FLAGS + ['-o', tmp] + srcs
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f'hipcc failed with exit code {res.returncode}')
+    if verbose and res.stderr.strip():
+        sys.stderr.write(res.stderr)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == '__main__':
+    print(build(verbose=True))

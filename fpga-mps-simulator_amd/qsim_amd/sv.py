@@ -1,0 +1,138 @@
+"""Statevector engine over libqsim_hip.so.
+
+The state lives in HBM as one complex128 torch tensor of 2^nq amplitudes
+(index bit q = qubit q).  Gates go to the library as qs_gate records and are
+applied by fused passes (one HBM read + write of the state per pass).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from .gates import sv_gate_records
+
+
+def records_to_array(records: Sequence[tuple]):
+    arr = (_lib.QsGate * max(len(records), 1))()
+    for i, (kind, q0, q1, mask, vals) in enumerate(records):
+        g = arr[i]
+        g.kind = kind
+        g.q[0] = q0
+        g.q[1] = q1
+        g.ctrl_mask = mask
+        g.m[:] = [float(v) for v in vals]
+    return arr
+
+
+def gates_to_records(gates: Iterable[tuple], nq: int) -> Tuple[list, complex]:
+    """Logical (name, qubits, params) tuples -> qs_gate records + total phase."""
+    recs: list = []
+    phase = 1.0 + 0.0j
+    for g in gates:
+        name, qs = g[0], g[1]
+        ps = g[2] if len(g) > 2 else None
+        r, ph = sv_gate_records(name, qs, ps, nq)
+        recs.extend(r)
+        phase *= ph
+    return recs, phase
+
+
+class StateVector:
+    """A 2^nq complex128 state on one GPU."""
+
+    def __init__(self, nq: int, device: int = 0, init_zero: bool = True):
+        import torch
+        _lib.ensure_device(device)
+        maxq = _lib.lib().qs_sv_max_qubits()
+        if not 1 <= nq <= maxq:
+            raise ValueError(f"num_qubits must be in [1,{maxq}]")
+        self.nq = nq
+        self.device = device
+        self.tensor = torch.empty(1 << nq, dtype=torch.complex128, device=f'cuda:{device}')
+        if init_zero:
+            self.set_zero()
+
+    # -- plumbing ---------------------------------------------------------
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return ctypes.c_void_p(self.tensor.data_ptr())
+
+    def _stream(self):
+        return _lib.current_stream(self.device)
+
+    # -- state management -----------------------------------------------------
+    def set_zero(self) -> None:
+        _lib.call('qs_sv_set_zero', self.ptr, self.nq, self._stream())
+
+    def load(self, sv: np.ndarray) -> None:
+        import torch
+        arr = np.ascontiguousarray(np.asarray(sv, dtype=np.complex128).reshape(-1))
+        if arr.size != (1 << self.nq):
+            raise ValueError(f"state has {arr.size} amplitudes, expected {1 << self.nq}")
+        self.tensor.copy_(torch.from_numpy(arr))
+
+    def to_numpy(self) -> np.ndarray:
+        return self.tensor.cpu().numpy()
+
+    # -- gates -------------------------------------------------------------
+    def apply_records(self, records: Sequence[tuple]) -> None:
+        if not records:
+            return
+        arr = records_to_array(records)
+        _lib.call('qs_sv_apply', self.ptr, self.nq, arr, len(records), self._stream())
+
+    def apply(self, gates: Iterable[tuple]) -> complex:
+        """Apply logical gates; returns the reference-lowering global phase."""
+        recs, phase = gates_to_records(gates, self.nq)
+        self.apply_records(recs)
+        return phase
+
+    def apply_words(self, words: np.ndarray) -> None:
+        """Apply gates in the reference's 8-word instruction format."""
+        w = np.ascontiguousarray(np.asarray(words, dtype=np.int32).reshape(-1))
+        if w.size % 8:
+            raise ValueError("gate_sequence length must be a multiple of 8")
+        _lib.call('qs_sv_apply_words', self.ptr, self.nq,
+                  w.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), w.size // 8,
+                  self._stream())
+
+    def norm_sq(self) -> float:
+        out = ctypes.c_double(0.0)
+        _lib.call('qs_sv_norm_sq', self.ptr, self.nq, ctypes.byref(out), self._stream())
+        return out.value
+
+    def scale(self, factor: complex) -> None:
+        f = complex(factor)
+        _lib.call('qs_sv_scale', self.ptr, self.nq, f.real, f.imag, self._stream())
+
+    def reset_qubit(self, q: int) -> None:
+        _lib.call('qs_sv_reset_qubit', self.ptr, self.nq, int(q), self._stream())
+
+    def probabilities(self):
+        import torch
+        p = torch.empty(1 << self.nq, dtype=torch.float64, device=self.tensor.device)
+        _lib.call('qs_sv_probabilities', self.ptr, self.nq, ctypes.c_void_p(p.data_ptr()),
+                  self._stream())
+        return p
+
+
+def plan_passes(nq: int, records: Sequence[tuple]) -> int:
+    arr = records_to_array(records)
+    n = ctypes.c_int(0)
+    _lib.call('qs_sv_plan_passes', nq, arr, len(records), ctypes.byref(n))
+    return n.value
+
+
+def plan_json(nq: int, records: Sequence[tuple]) -> dict:
+    """The fused-pass plan (host only; usable without a GPU)."""
+    import json
+    arr = records_to_array(records)
+    need = ctypes.c_size_t(0)
+    _lib.call('qs_sv_plan_json', nq, arr, len(records), None, 0, ctypes.byref(need))
+    buf = ctypes.create_string_buffer(need.value)
+    _lib.call('qs_sv_plan_json', nq, arr, len(records), buf, need.value, ctypes.byref(need))
+    return json.loads(buf.value.decode())

@@ -1,0 +1,179 @@
+/*
+ * qsim.h -- C ABI of the MI355X (gfx950) gate-application library libqsim_hip.so.
+ *
+ * This library replaces the device-runtime boundary of nasir26/fpga-mps-simulator:
+ * the pyxrt buffer objects and the two HLS kernels they launch.
+ *
+ *   statevector : quantum_fpga_kernel_complex128(double* state_real, double* state_imag,
+ *                   int* gate_sequence, int num_gates, int num_qubits)
+ *                   (opt_kernel.cpp:200-206), launched by QuantumFPGACircuit._launch_kernel
+ *                   (fpga_simulator_v07.py:486-488)
+ *   MPS         : fpga_mps_simulator(fp64* bank0..bank7, int* gate_seq, int num_gates,
+ *                   int nq, int chi_max) (fpga_mps_simulator.cpp:392-404), launched from
+ *                   _run_circuit_fpga (fpga_mps_simulator.py:512, 594), followed there by a
+ *                   host scipy SVD (fpga_mps_simulator.py:597-634)
+ *
+ * Conventions
+ *  - Every function returns a status: QS_OK (0) or a negative QS_E_* code.  The message of
+ *    the last failure on the calling thread is qs_last_error().  Nothing is clamped
+ *    silently: an argument beyond a documented limit is an error (the HLS kernels clamp
+ *    chi to 64 and gates to 512 without telling the host).
+ *  - Device pointers are plain device addresses (e.g. torch.Tensor.data_ptr() on ROCm).
+ *  - `stream` is a hipStream_t passed as void*; NULL is the null stream.  Calls are
+ *    asynchronous on that stream unless documented as synchronous.
+ *  - Calls may run concurrently from several threads but must not be given overlapping
+ *    buffers at the same time.
+ *  - Statevectors are complex128, interleaved (re, im), index bit q = qubit q
+ *    (little-endian, as the reference's collected statevector).
+ *  - MPS site tensors are complex128 (chi_l, 2, chi_r) in C order, as mps_core.MPSState.
+ */
+#ifndef QSIM_H
+#define QSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QS_ABI_VERSION 1
+
+#define QS_OK         0
+#define QS_E_ARG     -1   /* bad argument (null pointer, qubit out of range, ...) */
+#define QS_E_CHI     -2   /* bond dimension beyond the supported maximum */
+#define QS_E_HIP     -3   /* HIP runtime error */
+#define QS_E_NOCONV  -4   /* Jacobi SVD did not converge within the sweep limit */
+#define QS_E_NODEV   -5   /* no usable gfx950 device */
+#define QS_E_RANGE   -6   /* size beyond a documented limit (qubits, gates, workspace) */
+
+/* ---------------------------------------------------------------- lifecycle */
+
+/* ABI version of the loaded library (QS_ABI_VERSION). */
+int qs_abi_version(void);
+
+/* Select and initialise `device` (hipSetDevice + capability check).  Replaces
+ * xrt.device(i)/load_xclbin/xrt.kernel (fpga_simulator_v07.py:399-404,
+ * fpga_mps_simulator.py:675-681).  Fails with QS_E_NODEV when the device is not gfx950. */
+int qs_init(int device);
+
+/* Thread-local message of the last failure ("" if none). */
+const char* qs_last_error(void);
+
+/* Maximum qubits of the statevector path (state must fit one allocation). */
+int qs_sv_max_qubits(void);
+
+/* ------------------------------------------------------------- statevector */
+
+/* Gate kinds of qs_gate.  Matrices are complex, row-major, (re, im) pairs. */
+#define QS_GATE_U1    0   /* 2x2 on q[0]                                              */
+#define QS_GATE_U2    1   /* 4x4 on (q[0], q[1]); row index = 2*bit(q[0]) + bit(q[1])   */
+#define QS_GATE_DIAG1 2   /* diagonal (d0, d1) on q[0]                                 */
+#define QS_GATE_DIAG2 3   /* diagonal (d00, d01, d10, d11) on (q[0], q[1]), same order */
+
+/* One gate.  `ctrl_mask` lists control qubits: the gate acts only on the amplitudes
+ * whose index has all those bits set (controls must not overlap the targets). */
+typedef struct qs_gate {
+    int32_t  kind;
+    int32_t  q[2];
+    int32_t  reserved;
+    uint64_t ctrl_mask;
+    double   m[32];
+} qs_gate;
+
+/* |0...0> into a 2^nq complex128 state.  Replaces _distribute_statevector
+ * (fpga_simulator_v07.py:430-436). */
+int qs_sv_set_zero(void* state, int nq, void* stream);
+
+/* Apply `ng` gates in order.  Replaces the per-chunk upload + launch of
+ * quantum_fpga_kernel_complex128 (opt_kernel.cpp:200-353,
+ * fpga_simulator_v07.py:475-500).  The host plans fused passes: each pass reads
+ * and writes the state once and applies every gate whose targets fall in its
+ * tile of qubits; diagonal gates and controls may lie anywhere.  No gate-count limit. */
+int qs_sv_apply(void* state, int nq, const qs_gate* gates, int ng, void* stream);
+
+/* Apply gates given in the reference's 8-word instruction format
+ * [op, q0, q1, q2, f32 p0, f32 p1, f32 p2, 0] with the SV opcode table
+ * (id0 h1 x2 y3 z4 s5 t6 sdg7 tdg8 sx9 sxdg10 rx11 ry12 rz13 p14 u2 15 u3 16
+ * cx17 cz18; fpga_simulator_v07.py:69-78, 456-473; opt_kernel.cpp:39-58, 240-353).
+ * Parameters are decoded from float32 exactly as the HLS kernel does.  Unlike the
+ * kernel, num_gates is not clamped to 512: any count is applied. */
+int qs_sv_apply_words(void* state, int nq, const int32_t* gate_sequence, int num_gates,
+                      void* stream);
+
+/* Number of fused passes qs_sv_apply would launch for this gate list (host only). */
+int qs_sv_plan_passes(int nq, const qs_gate* gates, int ng, int* npasses);
+
+/* Diagnostics (host only, no device needed): the fused-pass plan as JSON
+ * {"passes":[{"K","R","tq":[...],"stages":[{"regs":[...],"gates":[...]}]}]}.
+ * Writes at most `cap` bytes into `buf` (NUL-terminated when it fits) and the size
+ * required into *needed.  No counterpart in the reference; it lets the planner be
+ * tested without a GPU. */
+int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
+                    size_t* needed);
+
+/* sum |a_i|^2 into *out_host.  Synchronous. */
+int qs_sv_norm_sq(const void* state, int nq, double* out_host, void* stream);
+
+/* state *= (re + i im). */
+int qs_sv_scale(void* state, int nq, double re, double im, void* stream);
+
+/* Reset qubit q: zero the |1> half, renormalise, |0..0> if nothing is left
+ * (fpga_simulator_v07.py:502-513, on the device).  Synchronous (reads the norm). */
+int qs_sv_reset_qubit(void* state, int nq, int q, void* stream);
+
+/* probs[i] = |a_i|^2 (float64, device buffer of 2^nq). */
+int qs_sv_probabilities(const void* state, int nq, double* probs, void* stream);
+
+/* --------------------------------------------------------------------- MPS */
+
+#define QS_PREC_FP64       0   /* contraction FP64, SVD FP64                        */
+#define QS_PREC_FP32_MFMA  1   /* contraction complex FP32 on MFMA, SVD FP64        */
+
+/* Largest chi_max the two-site update supports. */
+int qs_mps_max_chi(void);
+
+/* One two-site update of a chain: sites i and i+1 of the chain whose slots start at
+ * `sites` (slot k at sites + k*site_stride complex elements, each slot holding a
+ * (chi_l, 2, chi_r) tensor in C order) and whose bond dimensions are `bond`
+ * (device int32[n+1], bond[k] = left dimension of site k, bond[0] = bond[n] = 1).
+ * The update contracts the two sites with the gate, SVDs the (2chi_l x 2chi_r) matrix,
+ * keeps clip(#{S > cutoff*S[0]}, 1, chi_max) values, renormalises them, writes
+ * U*sqrt(S) into site i and sqrt(S)*Vh into site i+1, stores the kept rank in
+ * bond[i+1] and the norm_log increment in *dlog (mps_core.py:277-300, 326-350). */
+typedef struct qs_mps_task {
+    double*  sites;        /* device, complex128 slots                         */
+    int32_t* bond;         /* device, int32[n+1]                               */
+    double*  dlog;         /* device, receives the norm_log increment           */
+    int64_t  site_stride;  /* complex elements per slot, >= chi_max*2*chi_max  */
+    int32_t  site;         /* left site i                                      */
+    int32_t  chi_max;
+    double   cutoff;
+    double   gate[32];     /* 4x4, row index = 2*sigma_i + sigma_{i+1}         */
+} qs_mps_task;
+
+/* Workspace bytes qs_mps_update_batch needs for `ntasks` tasks of at most chi_max. */
+size_t qs_mps_workspace_bytes(int ntasks, int chi_max);
+
+/* Run `ntasks` independent two-site updates in one launch sequence (no two tasks may
+ * touch the same site).  Replaces the per-gate kernel launch + theta read-back + host
+ * SVD + site write-back of _run_circuit_fpga (fpga_mps_simulator.py:566-634) and the
+ * device contraction contract_and_apply_2q (fpga_mps_simulator.cpp:287-371).
+ * `workspace` is a device buffer of at least qs_mps_workspace_bytes(ntasks, max chi_max).
+ * When `status_dev` is non-null it receives per-task statuses (device int32[ntasks]):
+ * 0 or QS_E_NOCONV. */
+int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision,
+                        void* workspace, size_t ws_bytes, int32_t* status_dev,
+                        void* stream);
+
+/* A'[a, s', b] = sum_s m[s', s] A[a, s, b] on one site, dims read from bond[site],
+ * bond[site+1] on the device (fpga_mps_simulator.cpp:232-276, mps_core.py:267-273).
+ * m: 2x2 complex row-major, host memory (copied into the launch). */
+int qs_mps_apply_1q(double* sites, const int32_t* bond, int64_t site_stride, int site,
+                    const double m[8], void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSIM_H */
