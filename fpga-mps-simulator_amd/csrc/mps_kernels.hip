@@ -1,0 +1,611 @@
+// MPS two-site update on gfx950: contraction (FP64 or complex FP32 on MFMA), one-sided
+// Jacobi SVD in FP64, truncation, renormalisation and sqrt(S) absorption.
+//
+// Reference semantics: mps_core.MPSState._apply_adjacent_2q + _svd_truncate
+// (mps_core.py:277-350) and the driver's copy of the same rule
+// (fpga_mps_simulator.py:597-634); device contraction layout of
+// contract_and_apply_2q (fpga_mps_simulator.cpp:287-371).
+//
+// One update = one workgroup per kernel:
+//   contraction: theta[(a,s0),(s1,d)] = sum_pq G[2 s0 + s1, 2 p + q] sum_b A[a,p,b] B[b,q,d]
+//                written as a (2chi_l x 2chi_r) complex128 matrix into the workspace;
+//   svd:         1024 threads = 64 teams of 16 lanes; team t holds two columns of theta
+//                (8 rows per lane) in registers and orthogonalises them with a complex
+//                Jacobi rotation; columns travel between teams in a round-robin
+//                tournament through LDS; after convergence column norms are the
+//                singular values, the kept columns (normalised) are U, and
+//                Vh = S^-1 U^H theta is formed from theta in the workspace.
+#include <cmath>
+
+#include "qsim_internal.h"
+
+namespace qs {
+
+namespace {
+
+constexpr int kMaxChi = 64;              // register SVD holds 2*chi <= 128 rows/cols
+constexpr int kTeamLanes = 16;
+constexpr int kRowsPerLane = 8;          // 16 lanes x 8 rows = 128 rows
+constexpr int kSvdThreads = 1024;        // 64 teams
+constexpr int kMaxSweeps = 60;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cmadd(double2 acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+    return acc;
+}
+// acc += conj(a) * b
+__device__ __forceinline__ double2 cjmadd(double2 acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, fma(a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(-a.y, b.x, acc.y));
+    return acc;
+}
+
+struct Dims {
+    int cl, cm, cr;
+};
+
+__device__ __forceinline__ bool load_dims(const qs_mps_task& T, Dims& d) {
+    d.cl = T.bond[T.site];
+    d.cm = T.bond[T.site + 1];
+    d.cr = T.bond[T.site + 2];
+    return d.cl >= 1 && d.cm >= 1 && d.cr >= 1 && d.cl <= T.chi_max && d.cm <= T.chi_max &&
+           d.cr <= T.chi_max && T.chi_max <= kMaxChi;
+}
+
+__device__ __forceinline__ double2* site_ptr(const qs_mps_task& T, int k) {
+    return reinterpret_cast<double2*>(T.sites) + (size_t)k * (size_t)T.site_stride;
+}
+
+// ------------------------------------------------------------------ FP64 contraction
+__global__ void __launch_bounds__(256)
+mps_contract_fp64_kernel(const qs_mps_task* __restrict__ tasks, double2* __restrict__ ws,
+                         size_t theta_slot, int32_t* __restrict__ status) {
+    const qs_mps_task& T = tasks[blockIdx.x];
+    Dims d;
+    if (!load_dims(T, d)) {
+        if (threadIdx.x == 0 && status) status[blockIdx.x] = QS_E_CHI;
+        return;
+    }
+    const double2* A = site_ptr(T, T.site);
+    const double2* B = site_ptr(T, T.site + 1);
+    double2* th = ws + (size_t)blockIdx.x * theta_slot;
+    double2 g[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) g[k] = make_double2(T.gate[2 * k], T.gate[2 * k + 1]);
+    const int cl = d.cl, cm = d.cm, cr = d.cr;
+    const int ldt = 2 * cr;
+    for (int pr = threadIdx.x; pr < cl * cr; pr += blockDim.x) {
+        const int a = pr / cr, e = pr - a * cr;
+        double2 t00 = make_double2(0, 0), t01 = t00, t10 = t00, t11 = t00;
+        const double2* a0 = A + (size_t)(2 * a) * cm;
+        const double2* a1 = a0 + cm;
+        for (int b = 0; b < cm; ++b) {
+            const double2 x0 = a0[b], x1 = a1[b];
+            const double2 y0 = B[(size_t)(2 * b) * cr + e], y1 = B[(size_t)(2 * b + 1) * cr + e];
+            t00 = cmadd(t00, x0, y0);
+            t01 = cmadd(t01, x0, y1);
+            t10 = cmadd(t10, x1, y0);
+            t11 = cmadd(t11, x1, y1);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            double2 v = cmul(g[4 * s + 0], t00);
+            v = cmadd(v, g[4 * s + 1], t01);
+            v = cmadd(v, g[4 * s + 2], t10);
+            v = cmadd(v, g[4 * s + 3], t11);
+            const int s0 = s >> 1, s1 = s & 1;
+            th[(size_t)(2 * a + s0) * ldt + s1 * cr + e] = v;
+        }
+    }
+}
+
+// ------------------------------------------------- complex FP32 contraction on MFMA
+// GEMM T = A (M x K) * B (K x N), M = 2 chi_l rows (a,p), K = chi_m, N = 2 chi_r
+// columns (q,d), in complex FP32 with FP32 accumulation on v_mfma_f32_16x16x4f32
+// (4 real products per complex product).  Operands are staged through LDS as split
+// real/imaginary FP32 planes; T goes back through LDS and the 4x4 gate mixes the
+// (p,q) pairs in FP64 when theta is written.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaThreads = 256;                 // 4 waves, one 64x64 quadrant each
+constexpr int kMaxM = 2 * kMaxChi;                // 128
+constexpr int kMaxK = kMaxChi;                    // 64
+
+__global__ void __launch_bounds__(kMfmaThreads)
+mps_contract_f32_mfma_kernel(const qs_mps_task* __restrict__ tasks, double2* __restrict__ ws,
+                             size_t theta_slot, int32_t* __restrict__ status) {
+    // Operand planes: A as [M][K] (re, im), B as [K][N] (re, im); K padded to 64,
+    // M and N padded to 128.  2 * 128*64*4 B * 2 = 128 KiB; T reuses the same space.
+    __shared__ float lds[2 * kMaxM * kMaxK * 2];
+    float* Are = lds;
+    float* Aim = Are + kMaxM * kMaxK;
+    float* Bre = Aim + kMaxM * kMaxK;
+    float* Bim = Bre + kMaxK * kMaxM;
+
+    const qs_mps_task& T = tasks[blockIdx.x];
+    Dims d;
+    if (!load_dims(T, d)) {
+        if (threadIdx.x == 0 && status) status[blockIdx.x] = QS_E_CHI;
+        return;
+    }
+    const int M = 2 * d.cl, K = d.cm, N = 2 * d.cr;
+    const int Kp = (K + 3) & ~3;
+    const double2* A = site_ptr(T, T.site);
+    const double2* B = site_ptr(T, T.site + 1);
+    const int tid = threadIdx.x;
+
+    for (int idx = tid; idx < kMaxM * Kp; idx += kMfmaThreads) {
+        const int r = idx / Kp, k = idx - r * Kp;
+        double2 v = make_double2(0, 0);
+        if (r < M && k < K) v = A[(size_t)r * K + k];
+        Are[r * kMaxK + k] = (float)v.x;
+        Aim[r * kMaxK + k] = (float)v.y;
+    }
+    for (int idx = tid; idx < Kp * kMaxM; idx += kMfmaThreads) {
+        const int k = idx / kMaxM, c = idx - k * kMaxM;
+        double2 v = make_double2(0, 0);
+        if (k < K && c < N) v = B[(size_t)k * N + c];
+        Bre[k * kMaxM + c] = (float)v.x;
+        Bim[k * kMaxM + c] = (float)v.y;
+    }
+    __syncthreads();
+
+    const int wave = tid >> 6, lane = tid & 63;
+    const int qm = (wave >> 1) * 64, qn = (wave & 1) * 64;   // quadrant origin
+    const int lr = lane & 15, lk = lane >> 4;                 // operand lane mapping
+    floatx4 cre[4][4], cim[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cre[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+            cim[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+    const bool active = qm < M && qn < N;
+    if (active) {
+        for (int k0 = 0; k0 < Kp; k0 += 4) {
+            float ar[4], ai[4], nai[4], br[4], bi[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = qm + 16 * i + lr;
+                ar[i] = Are[row * kMaxK + k0 + lk];
+                ai[i] = Aim[row * kMaxK + k0 + lk];
+                nai[i] = -ai[i];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int col = qn + 16 * j + lr;
+                br[j] = Bre[(k0 + lk) * kMaxM + col];
+                bi[j] = Bim[(k0 + lk) * kMaxM + col];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], br[j], cre[i][j], 0, 0, 0);
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[i], bi[j], cre[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bi[j], cim[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], br[j], cim[i][j], 0, 0, 0);
+                }
+        }
+    }
+    __syncthreads();
+    // T back to LDS as [M][N] (re, im) planes: 2 * 128*128*4 B = 128 KiB.
+    float* Tre = lds;
+    float* Tim = lds + kMaxM * kMaxM;
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    // 16x16x4 f32 output: lane holds rows 4*(lane/16)+e, column lane%16
+                    const int row = qm + 16 * i + 4 * lk + e;
+                    const int col = qn + 16 * j + lr;
+                    Tre[row * kMaxM + col] = cre[i][j][e];
+                    Tim[row * kMaxM + col] = cim[i][j][e];
+                }
+    }
+    __syncthreads();
+    double2 g[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) g[k] = make_double2(T.gate[2 * k], T.gate[2 * k + 1]);
+    double2* th = ws + (size_t)blockIdx.x * theta_slot;
+    const int cl = d.cl, cr = d.cr, ldt = 2 * cr;
+    for (int pr = tid; pr < cl * cr; pr += kMfmaThreads) {
+        const int a = pr / cr, e = pr - a * cr;
+        const int r0 = 2 * a, r1 = 2 * a + 1, c0 = e, c1 = cr + e;
+        const double2 t00 = make_double2(Tre[r0 * kMaxM + c0], Tim[r0 * kMaxM + c0]);
+        const double2 t01 = make_double2(Tre[r0 * kMaxM + c1], Tim[r0 * kMaxM + c1]);
+        const double2 t10 = make_double2(Tre[r1 * kMaxM + c0], Tim[r1 * kMaxM + c0]);
+        const double2 t11 = make_double2(Tre[r1 * kMaxM + c1], Tim[r1 * kMaxM + c1]);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            double2 v = cmul(g[4 * s + 0], t00);
+            v = cmadd(v, g[4 * s + 1], t01);
+            v = cmadd(v, g[4 * s + 2], t10);
+            v = cmadd(v, g[4 * s + 3], t11);
+            th[(size_t)(2 * a + (s >> 1)) * ldt + (s & 1) * cr + e] = v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Jacobi SVD
+__device__ __forceinline__ double team_sum(double v) {
+    v += __shfl_xor(v, 8, kTeamLanes);
+    v += __shfl_xor(v, 4, kTeamLanes);
+    v += __shfl_xor(v, 2, kTeamLanes);
+    v += __shfl_xor(v, 1, kTeamLanes);
+    return v;
+}
+
+struct SvdShared {
+    double2 buf[(kMaxChi + 1) * 2 * kMaxChi];   // 64 exchange columns + spare, 130 KiB
+    double sval[2 * kMaxChi];
+    int order[2 * kMaxChi];
+    int rank[2 * kMaxChi];
+    double fro_part[kSvdThreads / kTeamLanes];
+    double fro;
+    int rotated;
+    int keep;
+    double scale;
+};
+
+__global__ void __launch_bounds__(kSvdThreads)
+mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict__ ws,
+               size_t theta_slot, int32_t* __restrict__ status) {
+    __shared__ SvdShared sh;
+    const qs_mps_task& T = tasks[blockIdx.x];
+    Dims d;
+    if (!load_dims(T, d)) {
+        if (threadIdx.x == 0 && status) status[blockIdx.x] = QS_E_CHI;
+        return;
+    }
+    const int m = 2 * d.cl, n = 2 * d.cr;
+    const int P = n / 2;                         // column pairs = active teams
+    const int tid = threadIdx.x;
+    const int team = tid / kTeamLanes, l16 = tid % kTeamLanes;
+    const bool act = team < P;
+    const double2* th = ws + (size_t)blockIdx.x * theta_slot;
+    double2* slot0 = sh.buf;                     // P columns of m
+    double2* spare = sh.buf + kMaxChi * 2 * kMaxChi;
+
+    double2 ca[kRowsPerLane], cb[kRowsPerLane];
+#pragma unroll
+    for (int k = 0; k < kRowsPerLane; ++k) {
+        const int r = l16 + kTeamLanes * k;
+        ca[k] = (act && r < m) ? th[(size_t)r * n + 2 * team] : make_double2(0, 0);
+        cb[k] = (act && r < m) ? th[(size_t)r * n + 2 * team + 1] : make_double2(0, 0);
+    }
+    // Frobenius norm^2 of theta: columns below 1e-14 * ||theta||_F are numerical zeros
+    // and are not rotated (a wide matrix has n - m of them).
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < kRowsPerLane; ++k)
+            s += ca[k].x * ca[k].x + ca[k].y * ca[k].y + cb[k].x * cb[k].x + cb[k].y * cb[k].y;
+        s = team_sum(s);
+        if (l16 == 0) sh.fro_part[team] = s;
+        __syncthreads();
+        if (tid == 0) {
+            double f = 0.0;
+            for (int t = 0; t < kSvdThreads / kTeamLanes; ++t) f += sh.fro_part[t];
+            sh.fro = f;
+        }
+        __syncthreads();
+    }
+    const double zero_sq = 1e-28 * sh.fro;
+    const double tol = 2.220446049250313e-16 * sqrt((double)m);
+
+    int sweeps = 0;
+    for (; sweeps < kMaxSweeps; ++sweeps) {
+        if (tid == 0) sh.rotated = 0;
+        __syncthreads();
+        const int rounds = (P > 1) ? (2 * P - 1) : 1;
+        for (int rd = 0; rd < rounds; ++rd) {
+            if (act) {
+                double al = 0, be = 0, gr = 0, gi = 0;
+#pragma unroll
+                for (int k = 0; k < kRowsPerLane; ++k) {
+                    al = fma(ca[k].x, ca[k].x, fma(ca[k].y, ca[k].y, al));
+                    be = fma(cb[k].x, cb[k].x, fma(cb[k].y, cb[k].y, be));
+                    gr = fma(ca[k].x, cb[k].x, fma(ca[k].y, cb[k].y, gr));
+                    gi = fma(ca[k].x, cb[k].y, fma(-ca[k].y, cb[k].x, gi));
+                }
+                al = team_sum(al);
+                be = team_sum(be);
+                gr = team_sum(gr);
+                gi = team_sum(gi);
+                const double g = sqrt(gr * gr + gi * gi);
+                if (al > zero_sq && be > zero_sq && g > tol * sqrt(al) * sqrt(be)) {
+                    const double zeta = (be - al) / (2.0 * g);
+                    double t;
+                    if (fabs(zeta) > 1e150) {
+                        t = 0.5 / zeta;
+                    } else {
+                        t = 1.0 / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        if (zeta < 0) t = -t;
+                    }
+                    const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                    const double er = gr / g, ei = gi / g;          // e^{i phi}
+                    // a' = c a - s e^{-i phi} b ;  b' = s e^{i phi} a + c b
+                    const double2 sem = make_double2(s * er, -s * ei);
+                    const double2 sep = make_double2(s * er, s * ei);
+#pragma unroll
+                    for (int k = 0; k < kRowsPerLane; ++k) {
+                        const double2 x = ca[k], y = cb[k];
+                        const double2 sy = cmul(sem, y), sx = cmul(sep, x);
+                        ca[k] = make_double2(c * x.x - sy.x, c * x.y - sy.y);
+                        cb[k] = make_double2(c * y.x + sx.x, c * y.y + sx.y);
+                    }
+                    if (l16 == 0) sh.rotated = 1;
+                }
+            }
+            if (P > 1) {
+                // Round-robin move: bottoms shift towards team 0, tops away from it,
+                // team 0 keeps its top, bot[0] becomes top[1], top[P-1] becomes bot[P-1].
+                if (act) {
+#pragma unroll
+                    for (int k = 0; k < kRowsPerLane; ++k) {
+                        const int r = l16 + kTeamLanes * k;
+                        slot0[team * 128 + r] = cb[k];
+                        if (team == 0) spare[r] = cb[k];
+                    }
+                }
+                __syncthreads();
+                if (act) {
+                    if (team < P - 1) {
+#pragma unroll
+                        for (int k = 0; k < kRowsPerLane; ++k)
+                            cb[k] = slot0[(team + 1) * 128 + l16 + kTeamLanes * k];
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < kRowsPerLane; ++k) cb[k] = ca[k];
+                    }
+                }
+                __syncthreads();
+                if (act) {
+#pragma unroll
+                    for (int k = 0; k < kRowsPerLane; ++k)
+                        slot0[team * 128 + l16 + kTeamLanes * k] = ca[k];
+                }
+                __syncthreads();
+                if (act) {
+                    if (team >= 2) {
+#pragma unroll
+                        for (int k = 0; k < kRowsPerLane; ++k)
+                            ca[k] = slot0[(team - 1) * 128 + l16 + kTeamLanes * k];
+                    } else if (team == 1) {
+#pragma unroll
+                        for (int k = 0; k < kRowsPerLane; ++k) ca[k] = spare[l16 + kTeamLanes * k];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        const int any = sh.rotated;
+        __syncthreads();
+        if (!any) break;
+    }
+    // Column identities after `sweeps` complete tournaments are not needed: the
+    // columns are ranked by norm below, and only norms and vectors are used.
+
+    // Final column norms.
+    double na = 0, nb = 0;
+#pragma unroll
+    for (int k = 0; k < kRowsPerLane; ++k) {
+        na = fma(ca[k].x, ca[k].x, fma(ca[k].y, ca[k].y, na));
+        nb = fma(cb[k].x, cb[k].x, fma(cb[k].y, cb[k].y, nb));
+    }
+    na = sqrt(team_sum(na));
+    nb = sqrt(team_sum(nb));
+    // Column slot ids: team t owns slots 2t (a) and 2t+1 (b) for the ranking.
+    if (act && l16 == 0) {
+        sh.sval[2 * team] = na;
+        sh.sval[2 * team + 1] = nb;
+    }
+    __syncthreads();
+    if (tid < n) {
+        const double v = sh.sval[tid];
+        int r = 0;
+        for (int j = 0; j < n; ++j) {
+            const double w = sh.sval[j];
+            r += (w > v) || (w == v && j < tid);
+        }
+        sh.rank[tid] = r;
+        sh.order[r] = tid;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int kmin = m < n ? m : n;
+        const double s0 = sh.sval[sh.order[0]];
+        const double thr = T.cutoff * s0;
+        int cnt = 0;
+        double total = 0.0;
+        for (int k = 0; k < kmin; ++k) {
+            const double s = sh.sval[sh.order[k]];
+            cnt += (s > thr);
+            total += s * s;
+        }
+        int keep = cnt < 1 ? 1 : cnt;
+        if (keep > T.chi_max) keep = T.chi_max;
+        double dlog = 0.0;
+        if (keep < kmin) {
+            double dropped = 0.0;
+            for (int k = keep; k < kmin; ++k) {
+                const double s = sh.sval[sh.order[k]];
+                dropped += s * s;
+            }
+            if (dropped > 0.0) dlog += 0.5 * log(1.0 - dropped / fmax(total, 1e-300));
+        }
+        double kept = 0.0;
+        for (int k = 0; k < keep; ++k) {
+            const double s = sh.sval[sh.order[k]];
+            kept += s * s;
+        }
+        const double nrm = sqrt(kept);
+        double scale = 1.0;
+        if (nrm > 1e-15) {
+            dlog += log(nrm);
+            scale = 1.0 / nrm;
+        }
+        sh.keep = keep;
+        sh.scale = scale;
+        *T.dlog = dlog;
+        T.bond[T.site + 1] = keep;
+        if (status) status[blockIdx.x] = (sweeps >= kMaxSweeps) ? QS_E_NOCONV : QS_OK;
+    }
+    __syncthreads();
+    const int keep = sh.keep;
+    const double scale = sh.scale;
+    // Kept columns, normalised (U), into LDS at their rank: U[rank][row].
+    if (act) {
+        const int ra = sh.rank[2 * team], rb = sh.rank[2 * team + 1];
+        const double ia = na > 0 ? 1.0 / na : 0.0, ib = nb > 0 ? 1.0 / nb : 0.0;
+#pragma unroll
+        for (int k = 0; k < kRowsPerLane; ++k) {
+            const int r = l16 + kTeamLanes * k;
+            if (ra < keep) sh.buf[ra * 128 + r] = make_double2(ca[k].x * ia, ca[k].y * ia);
+            if (rb < keep) sh.buf[rb * 128 + r] = make_double2(cb[k].x * ib, cb[k].y * ib);
+        }
+    }
+    __syncthreads();
+    // A' = U sqrt(S_t), (chi_l, 2, keep) = m x keep, row-major.
+    double2* Ao = site_ptr(T, T.site);
+    for (int idx = tid; idx < m * keep; idx += kSvdThreads) {
+        const int r = idx / keep, k = idx - r * keep;
+        const double f = sqrt(sh.sval[sh.order[k]] * scale);
+        const double2 u = sh.buf[k * 128 + r];
+        Ao[idx] = make_double2(u.x * f, u.y * f);
+    }
+    // B' = sqrt(S_t) Vh = sqrt(S_t)/S * U^H theta, (keep, 2, chi_r) = keep x n.
+    double2* Bo = site_ptr(T, T.site + 1);
+    {
+        const int c = tid % 128, kg = tid / 128;    // 8 groups of rows of B'
+        if (c < n) {
+            double2 acc[kMaxChi / 8];
+#pragma unroll
+            for (int q = 0; q < kMaxChi / 8; ++q) acc[q] = make_double2(0, 0);
+            for (int r = 0; r < m; ++r) {
+                const double2 t = th[(size_t)r * n + c];
+#pragma unroll
+                for (int q = 0; q < kMaxChi / 8; ++q) {
+                    const int k = kg + 8 * q;
+                    if (k < keep) acc[q] = cjmadd(acc[q], sh.buf[k * 128 + r], t);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kMaxChi / 8; ++q) {
+                const int k = kg + 8 * q;
+                if (k < keep) {
+                    const double s = sh.sval[sh.order[k]];
+                    const double f = s > 0 ? sqrt(s * scale) / s : 0.0;
+                    Bo[(size_t)k * n + c] = make_double2(acc[q].x * f, acc[q].y * f);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ one-site gate
+__global__ void mps_apply_1q_kernel(double2* __restrict__ sites, const int32_t* __restrict__ bond,
+                                    int64_t stride, int site, double2 m00, double2 m01,
+                                    double2 m10, double2 m11) {
+    const int cl = bond[site], cr = bond[site + 1];
+    double2* A = sites + (size_t)site * stride;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < cl * cr;
+         idx += gridDim.x * blockDim.x) {
+        const int a = idx / cr, b = idx - a * cr;
+        double2* p0 = A + (size_t)(2 * a) * cr + b;
+        double2* p1 = p0 + cr;
+        const double2 x0 = *p0, x1 = *p1;
+        *p0 = cmadd(cmul(m00, x0), m01, x1);
+        *p1 = cmadd(cmul(m10, x0), m11, x1);
+    }
+}
+
+}  // namespace
+}  // namespace qs
+
+using namespace qs;
+
+extern "C" {
+
+int qs_mps_max_chi(void) { return kMaxChi; }
+
+size_t qs_mps_workspace_bytes(int ntasks, int chi_max) {
+    if (ntasks <= 0 || chi_max <= 0) return 0;
+    const size_t head = sv_blob_offset((size_t)ntasks * sizeof(qs_mps_task));
+    const size_t slot = (size_t)4 * chi_max * chi_max;
+    return head + (size_t)ntasks * slot * sizeof(double2);
+}
+
+int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, void* workspace,
+                        size_t ws_bytes, int32_t* status_dev, void* stream) {
+    clear_error();
+    QS_REQUIRE(ntasks >= 0, QS_E_ARG, "negative task count");
+    if (ntasks == 0) return QS_OK;
+    QS_REQUIRE(tasks != nullptr && workspace != nullptr, QS_E_ARG, "null tasks or workspace");
+    QS_REQUIRE(precision == QS_PREC_FP64 || precision == QS_PREC_FP32_MFMA, QS_E_ARG,
+               "unknown precision %d", precision);
+    int chi = 1;
+    for (int i = 0; i < ntasks; ++i) {
+        const qs_mps_task& t = tasks[i];
+        QS_REQUIRE(t.sites && t.bond && t.dlog, QS_E_ARG, "task %d: null pointer", i);
+        QS_REQUIRE(t.chi_max >= 1, QS_E_ARG, "task %d: chi_max %d < 1", i, t.chi_max);
+        QS_REQUIRE(t.chi_max <= kMaxChi, QS_E_CHI, "task %d: chi_max %d > %d", i, t.chi_max,
+                   kMaxChi);
+        QS_REQUIRE(t.site_stride >= (int64_t)2 * t.chi_max * t.chi_max, QS_E_ARG,
+                   "task %d: site_stride %lld < 2*chi_max^2", i, (long long)t.site_stride);
+        QS_REQUIRE(t.site >= 0, QS_E_ARG, "task %d: negative site", i);
+        QS_REQUIRE(std::isfinite(t.cutoff) && t.cutoff >= 0, QS_E_ARG, "task %d: bad cutoff", i);
+        if (t.chi_max > chi) chi = t.chi_max;
+        for (int j = 0; j < i; ++j) {
+            const qs_mps_task& u = tasks[j];
+            if (u.sites == t.sites && std::abs(u.site - t.site) < 2)
+                QS_REQUIRE(false, QS_E_ARG, "tasks %d and %d share a site", j, i);
+        }
+    }
+    const size_t need = qs_mps_workspace_bytes(ntasks, chi);
+    QS_REQUIRE(ws_bytes >= need, QS_E_RANGE, "workspace %zu bytes < %zu needed", ws_bytes, need);
+    hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
+    char* ws = static_cast<char*>(workspace);
+    qs_mps_task* dtasks = reinterpret_cast<qs_mps_task*>(ws);
+    double2* theta = reinterpret_cast<double2*>(ws + sv_blob_offset((size_t)ntasks * sizeof(qs_mps_task)));
+    const size_t slot = (size_t)4 * chi * chi;
+    QS_HIP(hipMemcpyAsync(dtasks, tasks, (size_t)ntasks * sizeof(qs_mps_task),
+                          hipMemcpyHostToDevice, hs));
+    if (precision == QS_PREC_FP32_MFMA)
+        hipLaunchKernelGGL(mps_contract_f32_mfma_kernel, dim3(ntasks), dim3(kMfmaThreads), 0, hs,
+                           dtasks, theta, slot, status_dev);
+    else
+        hipLaunchKernelGGL(mps_contract_fp64_kernel, dim3(ntasks), dim3(256), 0, hs, dtasks,
+                           theta, slot, status_dev);
+    QS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mps_svd_kernel, dim3(ntasks), dim3(kSvdThreads), 0, hs, dtasks, theta, slot,
+                       status_dev);
+    QS_HIP(hipGetLastError());
+    return QS_OK;
+}
+
+int qs_mps_apply_1q(double* sites, const int32_t* bond, int64_t site_stride, int site,
+                    const double m[8], void* stream) {
+    clear_error();
+    QS_REQUIRE(sites && bond && m, QS_E_ARG, "null pointer");
+    QS_REQUIRE(site >= 0 && site_stride > 0, QS_E_ARG, "bad site/stride");
+    hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(mps_apply_1q_kernel, dim3(16), dim3(256), 0, hs,
+                       reinterpret_cast<double2*>(sites), bond, site_stride, site,
+                       make_double2(m[0], m[1]), make_double2(m[2], m[3]),
+                       make_double2(m[4], m[5]), make_double2(m[6], m[7]));
+    QS_HIP(hipGetLastError());
+    return QS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,371 @@
+"""Drop-in for the reference's MPS driver (fpga_mps_simulator.py), MI355X edition.
+
+Same public names, constructor checks, result object and phase-time keys as the
+reference; the XRT kernel (contraction on the FPGA) plus host scipy SVD are
+replaced by libqsim_hip.so's two-site update (contraction + Jacobi SVD +
+truncation + sqrt(S) absorption on the GPU).  Site tensors stay in HBM for the
+whole circuit; two-site gates on disjoint bonds that are adjacent in the gate
+list are submitted as one batch.
+
+Deviations (each deliberate):
+  * gates are FP64 matrices (the FPGA receives float32 gate words and float32
+    1q parameters, fpga_mps_simulator.py:204-258);
+  * no worker process: circuits run in the calling process, so there is no
+    pickling of circuits and no process respawn; get_card_stats() reports this
+    process;
+  * chi_max may be up to qs_mps_max_chi() (64 on this build), the same bound
+    the reference enforces (CHI_MAX_SVD_HW = 64);
+  * xclbin_path / FPGA_MPS_XCLBIN are accepted and ignored;
+  * the keyword `precision` ('fp64' default, or 'fp32' for the complex-FP32
+    MFMA contraction) has no counterpart in the reference.
+"""
+
+from __future__ import annotations
+
+import logging
+import math
+import os
+import time
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from qsim_amd import _lib
+from qsim_amd.gates import (ONE_QUBIT_GATES, TWO_QUBIT_GATES, matrix_1q, matrix_2q,
+                            swap_qubit_order)
+from qsim_amd.mps import MPSChain, max_chi
+
+_lvl = logging.DEBUG if os.environ.get('FPGA_MPS_LOG', '').upper() == 'DEBUG' else logging.WARNING
+logger = logging.getLogger('fpga_mps')
+logger.setLevel(_lvl)
+
+NUM_CARDS = 1
+NUM_HBM_BANKS = 8
+KERNEL_NAME = 'fpga_mps_simulator'
+DEFAULT_XCLBIN = 'fpga_mps_simulator.xclbin'
+MAX_GATES_BRAM = 512
+CHI_MAX_SVD_HW = 64
+XRT_AVAILABLE = False
+QISKIT_AVAILABLE = False
+
+_OP1Q: Dict[str, int] = {
+    'id': 0, 'h': 1, 'x': 2, 'y': 3, 'z': 4, 's': 5, 'sdg': 6, 't': 7, 'tdg': 8, 'sx': 9,
+    'sxdg': 10, 'rx': 11, 'ry': 12, 'rz': 13, 'p': 14, 'phase': 14, 'u1': 14, 'u2': 15,
+    'u3': 16, 'u': 16,
+}
+_OP2Q: Dict[str, int] = {
+    'cx': 17, 'cnot': 17, 'cy': 18, 'cz': 19, 'ch': 20, 'cp': 21, 'cphase': 21, 'crx': 22,
+    'cry': 23, 'crz': 24, 'csx': 25, 'swap': 26, 'iswap': 27, 'dcx': 28, 'ecr': 29,
+    'rxx': 30, 'ryy': 31, 'rzz': 32,
+}
+_SINGLE_QUBIT_GATES = ONE_QUBIT_GATES
+_TWO_QUBIT_GATES = TWO_QUBIT_GATES | {'cnot'}
+_SWAP = matrix_2q('swap')
+
+
+def _qubit_index(circuit, qubit) -> int:
+    try:
+        return circuit.find_bit(qubit).index
+    except Exception:
+        return circuit.qubits.index(qubit)
+
+
+def _decompose_ccx(c0: int, c1: int, tgt: int) -> list:
+    """Clifford+T Toffoli used by the reference compiler (fpga_mps_simulator.py:272-282)."""
+    return [('h', [tgt], None), ('cx', [c1, tgt], None), ('tdg', [tgt], None),
+            ('cx', [c0, tgt], None), ('t', [tgt], None), ('cx', [c1, tgt], None),
+            ('tdg', [tgt], None), ('cx', [c0, tgt], None), ('t', [c1], None), ('t', [tgt], None),
+            ('h', [tgt], None), ('cx', [c0, c1], None), ('t', [c0], None), ('tdg', [c1], None),
+            ('cx', [c0, c1], None)]
+
+
+def _mat_2q(name: str, params) -> np.ndarray:
+    return matrix_2q('cx' if name == 'cnot' else name, params)
+
+
+def compile_circuit(circuit, n_qubits: int) -> Tuple[list, list]:
+    """Circuit -> nearest-neighbour gate list + measured sites (fpga_mps_simulator.py:285-410).
+
+    Gate dicts carry 'type' ('1q' | '2q' | 'reset'), 'gate_name', 'sites', 'params',
+    'opcode' and 'gate_mat'.  Non-adjacent two-qubit gates are routed by SWAP chains
+    that walk the far qubit next to the near one and back again.
+    """
+    perm = list(range(n_qubits))
+    out: list = []
+    measured: list = []
+
+    def emit_swap(a: int, b: int):
+        lo, hi = min(a, b), max(a, b)
+        out.append({'type': '2q', 'gate_name': 'swap', 'sites': [lo, hi], 'params': None,
+                    'opcode': _OP2Q['swap'], 'gate_mat': _SWAP})
+        for k in range(n_qubits):
+            if perm[k] == lo:
+                perm[k] = hi
+            elif perm[k] == hi:
+                perm[k] = lo
+
+    def emit_2q(pa: int, pb: int, nm: str, ps):
+        g = _mat_2q(nm, ps)
+        if pa > pb:
+            g = swap_qubit_order(g)
+        lo, hi = min(pa, pb), max(pa, pb)
+        for k in range(hi, lo + 1, -1):
+            emit_swap(k - 1, k)
+        out.append({'type': '2q', 'gate_name': nm, 'sites': [lo, lo + 1], 'params': ps,
+                    'opcode': _OP2Q.get(nm, 17), 'gate_mat': g})
+        for k in range(lo + 1, hi):
+            emit_swap(k, k + 1)
+
+    for inst in circuit.data:
+        nm = inst.operation.name.lower()
+        qs = [_qubit_index(circuit, q) for q in inst.qubits]
+        ps = [float(p) for p in getattr(inst.operation, 'params', [])] or None
+        if nm == 'measure':
+            for q in qs:
+                if perm[q] not in measured:
+                    measured.append(perm[q])
+            continue
+        if nm in ('barrier', 'snapshot', 'delay', 'id'):
+            continue
+        if nm == 'reset':
+            out.append({'type': 'reset', 'gate_name': 'reset', 'sites': [perm[qs[0]]],
+                        'params': None, 'opcode': -1, 'gate_mat': None})
+            continue
+        if nm in _SINGLE_QUBIT_GATES:
+            out.append({'type': '1q', 'gate_name': nm, 'sites': [perm[qs[0]]], 'params': ps,
+                        'opcode': _OP1Q.get(nm, 0), 'gate_mat': matrix_1q(nm, ps)})
+            continue
+        if nm in _TWO_QUBIT_GATES:
+            emit_2q(perm[qs[0]], perm[qs[1]], nm, ps)
+            continue
+        if nm in ('ccx', 'toffoli', 'ccnot'):
+            c0, c1, tgt = perm[qs[0]], perm[qs[1]], perm[qs[2]]
+            for snm, sqs, sps in _decompose_ccx(c0, c1, tgt):
+                if snm in _SINGLE_QUBIT_GATES:
+                    out.append({'type': '1q', 'gate_name': snm, 'sites': [sqs[0]],
+                                'params': sps, 'opcode': _OP1Q.get(snm, 0),
+                                'gate_mat': matrix_1q(snm, sps)})
+                else:
+                    emit_2q(sqs[0], sqs[1], snm, sps)
+            continue
+        raise ValueError(f"Unknown gate for the MPS path: {nm}")
+    if not measured:
+        measured = list(range(n_qubits))
+    return out, sorted(measured)
+
+
+def _host_reset(t: np.ndarray) -> np.ndarray:
+    """Reset of one site tensor (fpga_mps_simulator.py:530-552)."""
+    t = t.copy()
+    t[:, 1, :] = 0.0
+    nm = np.linalg.norm(t)
+    if nm > 1e-15:
+        return t / nm
+    t = np.zeros_like(t)
+    t[0, 0, 0] = 1.0
+    return t
+
+
+def run_gate_list(gate_list, n: int, chi_max: int, svd_cutoff: float, precision: str = 'fp64',
+                  device: int = 0, phase_times: Optional[Dict[str, float]] = None) -> MPSChain:
+    """Execute a compiled gate list on the GPU (the gate loop of _run_circuit_fpga).
+
+    Two-site gates accumulate into a batch while they touch pairwise-disjoint
+    bonds; a gate that touches a batched site flushes the batch first, so the
+    order of non-commuting gates is the order of the list.
+    """
+    import torch
+    chain = MPSChain(n, chi_max, svd_cutoff, device, precision)
+    batch: List[Tuple[int, np.ndarray]] = []
+    busy = set()
+    t_kernel = 0.0
+    t_xfer = 0.0
+
+    def flush():
+        nonlocal batch, busy, t_kernel
+        if batch:
+            t0 = time.perf_counter()
+            chain.apply_2q_batch(batch)
+            t_kernel += time.perf_counter() - t0
+        batch = []
+        busy = set()
+
+    for g in gate_list:
+        typ = g['type']
+        s = g['sites'][0]
+        if typ == '2q':
+            a, b = g['sites']
+            if b != a + 1:
+                raise AssertionError(f"Non-adjacent 2q gate after routing: sites {a},{b}")
+            if a in busy or b in busy:
+                flush()
+            batch.append((a, g['gate_mat']))
+            busy.update((a, b))
+        elif typ == '1q':
+            if s in busy:
+                flush()
+            t0 = time.perf_counter()
+            chain.apply_1q(s, g['gate_mat'])
+            t_kernel += time.perf_counter() - t0
+        elif typ == 'reset':
+            if s in busy:
+                flush()
+            t0 = time.perf_counter()
+            host = chain.site_tensors()[s]
+            chain.set_site(s, _host_reset(host))
+            t_xfer += time.perf_counter() - t0
+    flush()
+    t0 = time.perf_counter()
+    torch.cuda.synchronize(device)
+    t_kernel += time.perf_counter() - t0
+    chain.check_status()
+    if phase_times is not None:
+        phase_times['fpga_kernel'] += t_kernel
+        phase_times['hbm_transfer'] += t_xfer
+    return chain
+
+
+def _run_circuit(circuit, shots: int, chi_max: int, svd_cutoff: float, seed: Optional[int],
+                 precision: str, device: int):
+    """One circuit: compile, run on the GPU, sample (fpga_mps_simulator.py:417-654)."""
+    t_total0 = time.perf_counter()
+    pt = {'fpga_kernel': 0.0, 'hbm_transfer': 0.0, 'cpu_svd': 0.0, 'cpu_qr': 0.0,
+          'cpu_sampling': 0.0}
+    nq = circuit.num_qubits
+    gate_list, measured = compile_circuit(circuit, nq)
+    chain = run_gate_list(gate_list, nq, chi_max, svd_cutoff, precision, device, pt)
+    t0 = time.perf_counter()
+    host = chain.to_host()
+    pt['hbm_transfer'] += time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
+    counts = host.get_counts(shots, measured, rng)
+    pt['cpu_sampling'] += time.perf_counter() - t0
+    total = time.perf_counter() - t_total0
+    pt['host_other'] = max(0.0, total - sum(pt[k] for k in ('fpga_kernel', 'hbm_transfer',
+                                                            'cpu_svd', 'cpu_qr',
+                                                            'cpu_sampling')))
+    return host, counts, pt
+
+
+class FPGAMPSResult:
+    """Result container (fpga_mps_simulator.py:733-771)."""
+
+    def __init__(self, results: list, shots: int, num_cards: int, elapsed: float, mode: str):
+        self._results = results
+        self.shots = shots
+        self.num_cards = num_cards
+        self.total_time = elapsed
+        self.mode = mode
+
+    def get_counts(self, idx: int = 0) -> Dict[str, int]:
+        return self._results[idx]['counts']
+
+    def get_statevector(self, idx: int = 0) -> np.ndarray:
+        sv = self._results[idx].get('statevector')
+        if sv is None:
+            raise ValueError('Statevector not available (n > 24 or not computed).')
+        return sv
+
+    def get_mps(self, idx: int = 0):
+        return self._results[idx].get('mps')
+
+    def get_max_bond(self, idx: int = 0) -> int:
+        return self._results[idx].get('max_bond', -1)
+
+    def get_bond_dims(self, idx: int = 0) -> List[int]:
+        return self._results[idx].get('bond_dims', [])
+
+    def get_phase_times(self, idx: int = 0) -> Dict[str, float]:
+        return self._results[idx].get('phase_times', {})
+
+    def success(self) -> bool:
+        return True
+
+
+class FPGAMPSSimulator:
+    """MPS simulator with the reference's interface, running on one MI355X."""
+
+    def __init__(self, num_cards: int = NUM_CARDS, chi_max: int = CHI_MAX_SVD_HW,
+                 svd_cutoff: float = 1e-12, engine: str = 'fpga', mode: str = 'shots',
+                 xclbin_path: Optional[str] = None, precision: str = 'fp64', device: int = 0):
+        if num_cards != 1:
+            raise ValueError(f'Single-card verification build: num_cards must be 1, got {num_cards}. '
+                             'Multi-card operation is not permitted in this environment.')
+        if mode != 'shots':
+            raise ValueError(f"Single-card verification build: mode must be 'shots', got {mode!r}. "
+                             "Distributed/multi-card mode has been removed from this environment.")
+        if engine != 'fpga':
+            raise ValueError(f"Single-card verification build: engine must be 'fpga', got {engine!r}. "
+                             "There is no CPU engine in this environment — CPU fallback is forbidden.")
+        limit = max_chi()
+        if not (1 <= chi_max <= limit):
+            raise ValueError(f'chi_max must be in [1, {limit}], got {chi_max}.')
+        if precision not in ('fp64', 'fp32', 'fp32_mfma'):
+            raise ValueError(f"precision must be 'fp64' or 'fp32', got {precision!r}")
+        try:
+            _lib.ensure_device(device)
+        except Exception as e:
+            raise RuntimeError(f'GPU path unavailable: could not open device {device} ({e}). '
+                               'No CPU fallback is permitted in this environment.') from e
+        self.num_cards = num_cards
+        self.chi_max = chi_max
+        self.svd_cutoff = svd_cutoff
+        self.mode = mode
+        self.precision = precision
+        self.device = device
+        self._engine = 'fpga'
+        self._xclbin = xclbin_path or os.environ.get('FPGA_MPS_XCLBIN')
+
+    def run(self, circuits, shots: int = 1024, seed: Optional[int] = None) -> FPGAMPSResult:
+        if not isinstance(circuits, list):
+            circuits = [circuits]
+        t0 = time.time()
+        results = []
+        for ti, circuit in enumerate(circuits):
+            s = (seed + ti) if seed is not None else None
+            try:
+                host, counts, pt = _run_circuit(circuit, shots, self.chi_max, self.svd_cutoff, s,
+                                                self.precision, self.device)
+            except (ValueError, AssertionError) as e:
+                raise RuntimeError(f'Card 0 error: {e}') from e
+            except RuntimeError as e:
+                raise RuntimeError(f'Card 0 error: {e}') from e
+            sv = None
+            if circuit.num_qubits <= 24:
+                try:
+                    sv = host.get_statevector()
+                except Exception:
+                    sv = None
+            results.append({'counts': counts, 'statevector': sv, 'max_bond': host.max_bond(),
+                            'bond_dims': host.bond_dims(), 'phase_times': pt})
+        return FPGAMPSResult(results, shots, self.num_cards, time.time() - t0, 'shots_fpga')
+
+    def get_card_stats(self) -> List[dict]:
+        return [{'card_id': 0, 'pid': os.getpid(), 'alive': True, 'engine': 'FPGA',
+                 'execution_mode': 'HIP-GFX950', 'xclbin': self._xclbin}]
+
+    def shutdown(self):
+        pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.shutdown()
+        return False
+
+
+def _pauli_expectation_sv(sv: np.ndarray, pauli: str) -> float:
+    """<psi|P|psi> for a Pauli string; character k acts on qubit n-1-k
+    (fpga_mps_simulator.py:1091-1108)."""
+    n = int(round(math.log2(len(sv))))
+    mats = {'X': np.array([[0, 1], [1, 0]], dtype=np.complex128),
+            'Y': np.array([[0, -1j], [1j, 0]], dtype=np.complex128),
+            'Z': np.array([[1, 0], [0, -1]], dtype=np.complex128)}
+    psi = np.asarray(sv, dtype=np.complex128).reshape([2] * n)
+    for k, ch in enumerate(pauli.upper()):
+        if ch == 'I':
+            continue
+        psi = np.moveaxis(np.tensordot(mats.get(ch, np.eye(2)), psi, axes=([1], [k])), 0, k)
+    return float(np.real(np.vdot(np.asarray(sv).reshape(-1), psi.reshape(-1))))

@@ -1,0 +1,273 @@
+"""MPS engine over libqsim_hip.so.
+
+A chain of n sites lives in HBM as one complex128 tensor of n slots, each slot
+sized for a (chi_max, 2, chi_max) tensor; the actual bond dimensions stay on
+the device (int32 bond[n+1]), so consecutive two-site updates never wait for
+the host.  Two-site updates on disjoint bonds are submitted as one batch
+(one contraction launch + one SVD launch for the whole batch).
+
+Semantics follow mps_core.MPSState (mps_core.py:172-494): |0..0> start,
+sqrt(S) absorbed into both neighbours, norm_log bookkeeping of _svd_truncate.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Dict, Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+
+PRECISIONS = {'fp64': _lib.QS_PREC_FP64, 'fp32': _lib.QS_PREC_FP32_MFMA,
+              'fp32_mfma': _lib.QS_PREC_FP32_MFMA}
+
+
+def max_chi() -> int:
+    return _lib.lib().qs_mps_max_chi()
+
+
+class MPSChain:
+    """Device-resident MPS of n qubits with bond dimensions up to chi_max."""
+
+    def __init__(self, n: int, chi_max: int, cutoff: float = 1e-12, device: int = 0,
+                 precision: str = 'fp64'):
+        import torch
+        _lib.ensure_device(device)
+        if n < 1:
+            raise ValueError('n must be >= 1')
+        if not 1 <= chi_max <= max_chi():
+            raise ValueError(f'chi_max must be in [1, {max_chi()}], got {chi_max}')
+        if precision not in PRECISIONS:
+            raise ValueError(f'precision must be one of {sorted(PRECISIONS)}')
+        self.n = n
+        self.chi_max = chi_max
+        self.cutoff = float(cutoff)
+        self.device = device
+        self.precision = precision
+        self.stride = 2 * chi_max * chi_max
+        dev = f'cuda:{device}'
+        self.sites = torch.zeros(n * self.stride, dtype=torch.complex128, device=dev)
+        self.sites.view(n, self.stride)[:, 0] = 1.0
+        self.bond = torch.ones(n + 1, dtype=torch.int32, device=dev)
+        self._dlogs: List = []          # device tensors of per-update norm_log increments
+        self._ws = None
+        self._ws_tasks = 0
+        self._status = None
+        self.updates = 0
+
+    # -- plumbing ------------------------------------------------------------
+    def _stream(self):
+        return _lib.current_stream(self.device)
+
+    def _workspace(self, ntasks: int):
+        import torch
+        if self._ws is None or self._ws_tasks < ntasks:
+            nbytes = _lib.lib().qs_mps_workspace_bytes(ntasks, self.chi_max)
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.sites.device)
+            self._ws_tasks = ntasks
+            self._status = torch.zeros(ntasks, dtype=torch.int32, device=self.sites.device)
+        return self._ws, self._status
+
+    # -- gates ---------------------------------------------------------------
+    def apply_2q_batch(self, updates: Sequence[Tuple[int, np.ndarray]],
+                       precision: Optional[str] = None) -> None:
+        """Two-site updates on bonds (i, i+1), all disjoint, in one launch sequence."""
+        import torch
+        k = len(updates)
+        if k == 0:
+            return
+        prec = PRECISIONS[precision or self.precision]
+        dl = torch.zeros(k, dtype=torch.float64, device=self.sites.device)
+        tasks = (_lib.QsMpsTask * k)()
+        base = self.sites.data_ptr()
+        for j, (i, g) in enumerate(updates):
+            if not 0 <= i < self.n - 1:
+                raise ValueError(f'bond ({i},{i + 1}) outside the chain of {self.n} sites')
+            t = tasks[j]
+            t.sites = base
+            t.bond = self.bond.data_ptr()
+            t.dlog = dl.data_ptr() + 8 * j
+            t.site_stride = self.stride
+            t.site = int(i)
+            t.chi_max = self.chi_max
+            t.cutoff = self.cutoff
+            gf = np.asarray(g, dtype=np.complex128).reshape(16)
+            vals = np.empty(32)
+            vals[0::2] = gf.real
+            vals[1::2] = gf.imag
+            t.gate[:] = vals.tolist()
+        ws, status = self._workspace(k)
+        _lib.call('qs_mps_update_batch', tasks, k, prec, ctypes.c_void_p(ws.data_ptr()),
+                  ws.numel(), ctypes.c_void_p(status.data_ptr()), self._stream())
+        self._dlogs.append(dl)
+        self.updates += k
+
+    def apply_2q(self, i: int, g: np.ndarray) -> None:
+        self.apply_2q_batch([(i, g)])
+
+    def apply_1q(self, site: int, u: np.ndarray) -> None:
+        if not 0 <= site < self.n:
+            raise ValueError(f'site {site} outside the chain')
+        m = np.asarray(u, dtype=np.complex128).reshape(4)
+        arr = (ctypes.c_double * 8)(*[float(v) for pair in zip(m.real, m.imag) for v in pair])
+        _lib.call('qs_mps_apply_1q', ctypes.c_void_p(self.sites.data_ptr()),
+                  ctypes.c_void_p(self.bond.data_ptr()), self.stride, int(site), arr,
+                  self._stream())
+
+    def check_status(self) -> None:
+        """Raise if any update of the last batches failed to converge."""
+        if self._status is not None:
+            bad = self._status.cpu().numpy()
+            if (bad != 0).any():
+                raise RuntimeError(f'MPS update status {int(bad[bad != 0][0])} '
+                                   f'(QS_E_NOCONV={_lib.QS_E_NOCONV}, QS_E_CHI={_lib.QS_E_CHI})')
+
+    # -- host views -----------------------------------------------------------
+    @property
+    def norm_log(self) -> float:
+        total = 0.0
+        for d in self._dlogs:
+            for v in d.cpu().numpy().tolist():
+                total += v
+        return total
+
+    def bond_dims(self) -> List[int]:
+        return self.bond.cpu().numpy()[1:self.n].astype(int).tolist()
+
+    def max_bond(self) -> int:
+        return 1 if self.n <= 1 else max(self.bond_dims())
+
+    def site_tensors(self) -> List[np.ndarray]:
+        b = self.bond.cpu().numpy()
+        flat = self.sites.view(self.n, self.stride).cpu().numpy()
+        return [flat[i, :b[i] * 2 * b[i + 1]].reshape(b[i], 2, b[i + 1]).copy()
+                for i in range(self.n)]
+
+    def set_site(self, i: int, t: np.ndarray) -> None:
+        """Overwrite site i with a (chi_l, 2, chi_r) tensor whose dims match the bonds."""
+        import torch
+        b = self.bond.cpu().numpy()
+        t = np.ascontiguousarray(np.asarray(t, dtype=np.complex128))
+        if t.shape != (b[i], 2, b[i + 1]):
+            raise ValueError(f'site {i}: shape {t.shape} != ({b[i]}, 2, {b[i + 1]})')
+        row = self.sites.view(self.n, self.stride)[i]
+        row[:t.size].copy_(torch.from_numpy(t.reshape(-1)))
+
+    def to_host(self) -> 'HostMPS':
+        return HostMPS(self.site_tensors(), self.norm_log)
+
+
+class HostMPS:
+    """Host copy of a chain with the read-outs of mps_core.MPSState."""
+
+    def __init__(self, sites: List[np.ndarray], norm_log: float):
+        self.sites = sites
+        self.n = len(sites)
+        self.norm_log = norm_log
+
+    def bond_dims(self) -> List[int]:
+        return [t.shape[2] for t in self.sites[:-1]]
+
+    def max_bond(self) -> int:
+        return 1 if self.n <= 1 else max(self.bond_dims())
+
+    def get_statevector(self) -> np.ndarray:
+        """Dense state with qubit q <-> index bit q (mps_core.py:433-461)."""
+        if self.n > 24:
+            raise ValueError(f"Statevector reconstruction for n={self.n} > 24 not supported.")
+        acc = self.sites[0][0]
+        for t in self.sites[1:]:
+            acc = np.tensordot(acc, t, axes=([1], [0])).reshape(-1, t.shape[2])
+        sv = acc[:, 0] * math.exp(self.norm_log)
+        return sv.reshape([2] * self.n).transpose(list(range(self.n - 1, -1, -1))).reshape(-1)
+
+    def right_environments(self) -> List[np.ndarray]:
+        envs: List = [None] * (self.n + 1)
+        envs[self.n] = np.ones((1, 1), dtype=np.complex128)
+        for i in range(self.n - 1, -1, -1):
+            a = self.sites[i]
+            r = np.einsum('asb,bc,dsc->ad', a, envs[i + 1], a.conj())
+            tr = float(np.real(np.trace(r)))
+            if tr > 1e-300:
+                r = r / tr
+            envs[i] = r
+        return envs
+
+    def sample(self, shots: int, measured: Optional[List[int]] = None,
+               rng: Optional[np.random.Generator] = None) -> List[str]:
+        """Sequential conditional sampling (mps_core.py:371-420)."""
+        rng = rng if rng is not None else np.random.default_rng()
+        measured = list(range(self.n)) if measured is None else list(measured)
+        full = set(measured) == set(range(self.n))
+        envs = self.right_environments()
+        out = []
+        for _ in range(shots):
+            v = np.ones(1, dtype=np.complex128)
+            bits = [0] * self.n
+            for i in range(self.n):
+                a = self.sites[i]
+                r = envs[i + 1]
+                pr = np.zeros(2)
+                for s in (0, 1):
+                    w = v @ a[:, s, :]
+                    pr[s] = np.real(w @ r @ w.conj())
+                pr = np.maximum(pr, 0.0)
+                tot = pr.sum()
+                pr = np.array([0.5, 0.5]) if tot < 1e-15 else pr / tot
+                o = int(rng.choice(2, p=pr))
+                bits[i] = o
+                v = v @ a[:, o, :]
+                nv = np.linalg.norm(v)
+                if nv > 1e-15:
+                    v = v / nv
+            if full:
+                out.append(''.join(str(bits[q]) for q in range(self.n - 1, -1, -1)))
+            else:
+                out.append(''.join(str(bits[q]) for q in sorted(measured, reverse=True)))
+        return out
+
+    def get_counts(self, shots, measured=None, rng=None) -> Dict[str, int]:
+        c: Dict[str, int] = {}
+        for s in self.sample(shots, measured, rng):
+            c[s] = c.get(s, 0) + 1
+        return c
+
+
+def overlap(bra, ket) -> complex:
+    """<bra|ket> of two chains (host copies) by transfer matrices, with exp(norm_log)."""
+    env = np.ones((1, 1), dtype=np.complex128)
+    for a, b in zip(bra.sites, ket.sites):
+        # env[x, y] a*[x, s, p] b[y, s, q] -> new env[p, q]
+        t = np.tensordot(env, b, axes=([1], [0]))              # (x, s, q)
+        env = np.tensordot(a.conj(), t, axes=([0, 1], [0, 1]))  # (p, q)
+    return complex(env[0, 0]) * math.exp(bra.norm_log + ket.norm_log)
+
+
+def smoke_check() -> None:
+    """One small brick-wall run on cuda:0 against the MPS oracle (used by smoke())."""
+    from oracle import mps_ref
+    from . import workloads
+    n, depth, chi = 10, 6, 16
+    gates = workloads.brickwall_gates(n, depth, seed=3, kind='haar')
+    ch = MPSChain(n, chi, 1e-12, precision='fp64')
+    ref = mps_ref.MPS(n, chi, 1e-12)
+    layer = None
+    batch = []
+    for lay, i, g in gates:
+        if lay != layer and batch:
+            ch.apply_2q_batch(batch)
+            batch = []
+        layer = lay
+        batch.append((i, g))
+        ref.apply_adjacent_2q(i, g)
+    ch.apply_2q_batch(batch)
+    ch.check_status()
+    got = ch.to_host()
+    sv_g, sv_r = got.get_statevector(), ref.get_statevector()
+    err = float(np.max(np.abs(sv_g - sv_r)))
+    print(f'smoke: MPS {n}q depth {depth} chi {chi}: bonds {got.bond_dims()} '
+          f'max |diff| of statevector vs oracle = {err:.3e}')
+    assert got.bond_dims() == ref.bond_dims()
+    assert err < 1e-10 * max(1.0, float(np.max(np.abs(sv_r))))

@@ -1,0 +1,127 @@
+"""MPS leg of bench.py: two-site updates/s at chi=64.
+
+Workload: 1000 qubits, |0..0>, brick-wall depth 40 of exp(-0.1 i H) gates (H seeded
+random Hermitian), chi_max 64, cutoff 1e-10: 20 x 500 + 20 x 499 = 19 980 updates.
+
+  fp32_batched : every layer is one batch (complex FP32 contraction on MFMA, FP64 SVD)
+  fp64_batched : the same batches with the FP64 contraction
+  fp64_plain   : one update per launch sequence, FP64 (mps_core semantics, gate by gate),
+                 timed on a bounded sample: the updates of layers `plain_layers` after
+                 replaying the earlier layers untimed, so the sample sees the same bond
+                 dimensions as the batched runs at those layers
+
+Per-layer times come from HIP events on the stream the kernels run on.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from typing import Dict, List
+
+import numpy as np
+
+N_SITES = 1000
+DEPTH = 40
+CHI = 64
+CUTOFF = 1e-10
+SEED = 42
+
+
+def contraction_flops(cl: int, cm: int, cr: int) -> float:
+    """Algorithmic real flops of one update's contraction: 32 chi_l chi_r (chi_m + 4)."""
+    return 32.0 * cl * cr * (cm + 4)
+
+
+def _layers(gates):
+    out: Dict[int, list] = {}
+    for lay, i, g in gates:
+        out.setdefault(lay, []).append((i, g))
+    return [out[k] for k in sorted(out)]
+
+
+def run_batched(layers, precision: str):
+    import torch
+    from .mps import MPSChain
+    ch = MPSChain(N_SITES, CHI, CUTOFF, precision=precision)
+    stream = torch.cuda.current_stream()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(layers) + 1)]
+    flops = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for k, batch in enumerate(layers):
+        ch.apply_2q_batch(batch)
+        evs[k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_layer = [evs[k].elapsed_time(evs[k + 1]) / 1e3 for k in range(len(layers))]
+    ch.check_status()
+    return ch, per_layer, wall
+
+
+def run_plain_sample(layers, sample_layers: List[int]):
+    """FP64, one update per launch; earlier layers replayed batched (untimed)."""
+    import torch
+    from .mps import MPSChain
+    ch = MPSChain(N_SITES, CHI, CUTOFF, precision='fp64')
+    first = min(sample_layers)
+    for batch in layers[:first]:
+        ch.apply_2q_batch(batch)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n = 0
+    e0.record(stream)
+    for k in sample_layers:
+        for upd in layers[k]:
+            ch.apply_2q_batch([upd])
+            n += 1
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ch.check_status()
+    return n, e0.elapsed_time(e1) / 1e3
+
+
+def run(plain_layers=(30, 31)) -> dict:
+    import torch
+    from . import workloads
+    from .mps import overlap
+    gates = workloads.brickwall_gates(N_SITES, DEPTH, SEED)
+    layers = _layers(gates)
+    nupd = sum(len(b) for b in layers)
+    # warm-up (kernel load, allocator) on a short chain
+    from .mps import MPSChain
+    w = MPSChain(8, CHI, CUTOFF, precision='fp32')
+    w.apply_2q_batch([(0, gates[0][2]), (2, gates[1][2])])
+    torch.cuda.synchronize()
+
+    ch32, t32, wall32 = run_batched(layers, 'fp32')
+    ch64, t64, wall64 = run_batched(layers, 'fp64')
+    n_plain, t_plain = run_plain_sample(layers, list(plain_layers))
+    bonds = ch64.bond_dims()
+    h32, h64 = ch32.to_host(), ch64.to_host()
+    fid = abs(overlap(h32, h64)) ** 2 / (overlap(h32, h32).real * overlap(h64, h64).real)
+    s32 = sum(t32)
+    s64 = sum(t64)
+    samp32 = sum(t32[k] for k in plain_layers)
+    samp64 = sum(t64[k] for k in plain_layers)
+    n_samp = sum(len(layers[k]) for k in plain_layers)
+    rate32 = nupd / s32
+    rate_plain = n_plain / t_plain
+    return {
+        'workload': f'{N_SITES} qubits, brick-wall depth {DEPTH} of exp(-0.1iH), chi_max {CHI}, '
+                    f'cutoff {CUTOFF:g}: {nupd} two-site updates',
+        'updates': nupd,
+        'fp32_mfma_batched_updates_per_s': rate32,
+        'fp64_batched_updates_per_s': nupd / s64,
+        'fp64_plain_updates_per_s': rate_plain,
+        'fp64_plain_sample': f'{n_plain} updates of layers {list(plain_layers)}, one per launch',
+        'fp32_batched_updates_per_s_same_layers': n_samp / samp32,
+        'fp64_batched_updates_per_s_same_layers': n_samp / samp64,
+        'speedup_fp32_batched_vs_fp64_plain_same_layers': (n_samp / samp32) / rate_plain,
+        'fp32_seconds': s32, 'fp64_seconds': s64,
+        'max_bond': max(bonds), 'mean_bond': float(np.mean(bonds)),
+        'norm_log_fp64': ch64.norm_log, 'norm_log_fp32': ch32.norm_log,
+        'infidelity_fp32_vs_fp64': float(1 - fid),
+    }

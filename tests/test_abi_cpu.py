@@ -1,0 +1,94 @@
+"""The C ABI without a GPU: the library loads, exports every symbol include/qsim.h
+declares, and rejects bad arguments with a status and a message (no device call)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from qsim_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, 'include', 'qsim.h')).read()
+    src = re.sub(r'/\*.*?\*/', '', src, flags=re.S)
+    return sorted(set(re.findall(r'\b(qs_[a-z0-9_]+)\s*\(', src)))
+
+
+def test_library_exports_every_declared_function():
+    lib = _lib.load_library()
+    names = header_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python prototypes cover the header
+    assert set(names) <= set(_lib._PROTOS), set(names) - set(_lib._PROTOS)
+    assert _lib.missing_symbols() == []
+
+
+def test_abi_version_and_limits():
+    lib = _lib.load_library()
+    assert lib.qs_abi_version() == 1
+    assert lib.qs_sv_max_qubits() == 34
+    assert lib.qs_mps_max_chi() == 64
+    ntasks, chi = 5, 64
+    head = ((ntasks * ctypes.sizeof(_lib.QsMpsTask) + 255) // 256) * 256
+    assert lib.qs_mps_workspace_bytes(ntasks, chi) == head + ntasks * 4 * chi * chi * 16
+    assert ctypes.sizeof(_lib.QsGate) == 16 + 8 + 256
+    assert ctypes.sizeof(_lib.QsMpsTask) == 8 * 3 + 8 + 4 + 4 + 8 + 256
+
+
+def test_argument_errors_are_reported_without_device():
+    lib = _lib.load_library()
+    g = (_lib.QsGate * 1)()
+    g[0].kind = _lib.QS_GATE_U1
+    g[0].q[0] = 5
+    n = ctypes.c_int(0)
+    st = lib.qs_sv_plan_passes(3, g, 1, ctypes.byref(n))
+    assert st == _lib.QS_E_ARG
+    assert b'out of range' in lib.qs_last_error()
+    st = lib.qs_sv_plan_passes(40, g, 1, ctypes.byref(n))
+    assert st == _lib.QS_E_RANGE
+    g[0].q[0] = 0
+    g[0].ctrl_mask = 1
+    assert lib.qs_sv_plan_passes(3, g, 1, ctypes.byref(n)) == _lib.QS_E_ARG
+    assert b'overlaps' in lib.qs_last_error()
+    g[0].ctrl_mask = 0
+    assert lib.qs_sv_plan_passes(3, g, 1, ctypes.byref(n)) == _lib.QS_OK and n.value == 1
+    assert lib.qs_last_error() == b''
+    # null state pointer
+    assert lib.qs_sv_set_zero(None, 3, None) == _lib.QS_E_ARG
+    # MPS: chi beyond the limit is an error, never a clamp
+    t = (_lib.QsMpsTask * 1)()
+    t[0].sites = 1
+    t[0].bond = 1
+    t[0].dlog = 1
+    t[0].chi_max = 128
+    t[0].site_stride = 2 * 128 * 128
+    assert lib.qs_mps_update_batch(t, 1, 0, ctypes.c_void_p(1), 1 << 30, None, None) == _lib.QS_E_CHI
+    t[0].chi_max = 8
+    assert lib.qs_mps_update_batch(t, 1, 7, ctypes.c_void_p(1), 1 << 30, None, None) == _lib.QS_E_ARG
+    assert lib.qs_mps_update_batch(t, 1, 0, ctypes.c_void_p(1), 16, None, None) == _lib.QS_E_RANGE
+    t2 = (_lib.QsMpsTask * 2)()
+    for k in range(2):
+        t2[k].sites, t2[k].bond, t2[k].dlog = 1, 1, 1
+        t2[k].chi_max, t2[k].site_stride, t2[k].site = 8, 128, 3 + k
+    assert lib.qs_mps_update_batch(t2, 2, 0, ctypes.c_void_p(1), 1 << 30, None, None) == _lib.QS_E_ARG
+    assert b'share a site' in lib.qs_last_error()
+
+
+def test_engines_fail_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip('a GPU is visible')
+    from qsim_amd.sv import StateVector
+    with pytest.raises(RuntimeError):
+        StateVector(3)
+    from fpga_mps_simulator import FPGAMPSSimulator
+    with pytest.raises(RuntimeError):
+        FPGAMPSSimulator(chi_max=8)
+    with pytest.raises(ValueError):
+        FPGAMPSSimulator(num_cards=2)

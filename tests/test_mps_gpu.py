@@ -1,0 +1,209 @@
+"""GPU parity tests of the MPS path (libqsim_hip.so) against the MPS oracle.
+
+FP64 GPU runs must match mps_core semantics to 1e-10 with identical bond dimensions
+(golden vectors written from mps_core.py, and the oracle on larger inputs); the
+complex-FP32 MFMA contraction must stay within 1e-6 infidelity of the FP64 oracle.
+"""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mps_ref, sv_ref
+from qsim_amd import workloads
+from qsim_amd.circuit import QuantumCircuit
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def chain_run(n, chi, cutoff, updates, precision='fp64'):
+    """updates: [(layer, site, gate)] -> MPSChain after batching each layer."""
+    from qsim_amd.mps import MPSChain
+    ch = MPSChain(n, chi, cutoff, precision=precision)
+    cur, batch = None, []
+    for lay, i, g in updates:
+        if lay != cur and batch:
+            ch.apply_2q_batch(batch)
+            batch = []
+        cur = lay
+        batch.append((i, g))
+    ch.apply_2q_batch(batch)
+    ch.check_status()
+    return ch
+
+
+def as_oracle(host):
+    m = mps_ref.MPS(host.n, 1, 0)
+    m.sites = host.sites
+    m.norm_log = host.norm_log
+    return m
+
+
+def fidelity(a, b):
+    ab = mps_ref.overlap(a, b)
+    aa = mps_ref.overlap(a, a).real
+    bb = mps_ref.overlap(b, b).real
+    return abs(ab) ** 2 / (aa * bb)
+
+
+@pytest.mark.parametrize('name', ['c1_16q', 'trunc_12q', 'exact_10q'])
+def test_golden_vectors_fp64(name):
+    case = dict(np.load(os.path.join(HERE, 'golden', f'mps_core_{name}.npz')))
+    n, chi, cut = int(case['n']), int(case['chi_max']), float(case['cutoff'])
+    layers = workloads.brickwall_bonds(n, int(case['depth']))
+    lay_of = {}
+    k = 0
+    ups = []
+    for lay, bonds in enumerate(layers):
+        for i in bonds:
+            assert int(case['sites'][k]) == i
+            ups.append((lay, i, case['gates'][k]))
+            k += 1
+    ch = chain_run(n, chi, cut, ups)
+    host = ch.to_host()
+    assert host.bond_dims() == [int(v) for v in case['bond_dims']]
+    assert abs(host.norm_log - float(case['norm_log'])) < 1e-10 * max(1, abs(float(case['norm_log'])))
+    sv = host.get_statevector()
+    p = np.abs(sv) ** 2
+    assert abs(p.sum() - float(case['norm_sq'])) < 1e-9
+    idx = np.arange(sv.size)
+    z = np.array([np.sum(p * (1 - 2 * ((idx >> q) & 1))) / p.sum() for q in range(n)])
+    assert np.allclose(z, case['z_expect'], atol=1e-10)
+    orc = as_oracle(host)
+    for b in range(n - 1):
+        s = orc.bond_spectrum(b)
+        assert np.allclose(s, case['bond_spectra'][b][:s.size], rtol=1e-8, atol=1e-11)
+    if 'statevector' in case:
+        assert np.max(np.abs(sv - case['statevector'])) < 1e-10
+
+
+def test_near_identity_chain_fp64_matches_oracle():
+    n, depth, chi, cut = 48, 14, 64, 1e-10
+    ups = workloads.brickwall_gates(n, depth, seed=17)
+    ch = chain_run(n, chi, cut, ups)
+    ref = mps_ref.MPS(n, chi, cut)
+    for _, i, g in ups:
+        ref.apply_adjacent_2q(i, g)
+    host = ch.to_host()
+    assert host.bond_dims() == ref.bond_dims()
+    assert abs(host.norm_log - ref.norm_log) < 1e-10 * max(1.0, abs(ref.norm_log))
+    assert 1 - fidelity(as_oracle(host), ref) < 1e-10
+
+
+def test_haar_saturated_chi64_fp64_matches_oracle():
+    n, depth, chi, cut = 16, 14, 64, 1e-12
+    ups = workloads.brickwall_gates(n, depth, seed=5, kind='haar')
+    ch = chain_run(n, chi, cut, ups)
+    ref = mps_ref.MPS(n, chi, cut)
+    for _, i, g in ups:
+        ref.apply_adjacent_2q(i, g)
+    host = ch.to_host()
+    assert host.bond_dims() == ref.bond_dims()
+    assert max(host.bond_dims()) == 64
+    assert abs(host.norm_log - ref.norm_log) < 1e-10 * max(1.0, abs(ref.norm_log))
+    sv, rv = host.get_statevector(), ref.get_statevector()
+    assert np.max(np.abs(sv - rv)) < 1e-10 * np.max(np.abs(rv))
+
+
+def test_fp32_mfma_fidelity():
+    n, depth, chi, cut = 64, 16, 64, 1e-10
+    ups = workloads.brickwall_gates(n, depth, seed=23)
+    ch = chain_run(n, chi, cut, ups, precision='fp32')
+    ref = mps_ref.MPS(n, chi, cut)
+    for _, i, g in ups:
+        ref.apply_adjacent_2q(i, g)
+    host = ch.to_host()
+    infid = 1 - fidelity(as_oracle(host), ref)
+    assert infid < 1e-6, infid
+    assert abs(host.norm_log - ref.norm_log) < 1e-5 * max(1.0, abs(ref.norm_log))
+
+
+def _circ(n, ops):
+    qc = QuantumCircuit(n)
+    for nm, qs, ps in ops:
+        qc.append(nm, qs, ps)
+    return qc
+
+
+DIAG_KATS = [  # fpga_diag.py:28-57
+    (2, [('h', [0], None), ('cx', [0, 1], None)]),
+    (4, [('h', [0], None), ('cx', [0, 1], None), ('cx', [1, 2], None), ('cx', [2, 3], None)]),
+    (4, [('h', [0], None), ('cx', [0, 3], None)]),
+    (1, [('u', [0], [0.7, 1.3, -0.4])]),
+    (2, [('u', [0], [0.7, 1.3, -0.4]), ('u', [1], [-0.9, 0.2, 2.1]), ('cx', [0, 1], None),
+         ('u', [0], [0.3, -0.6, 1.1])]),
+    (4, [('u', [0], [0.7, 1.3, -0.4]), ('u', [2], [-0.9, 0.2, 2.1]), ('cx', [0, 2], None)]),
+]
+
+
+@pytest.fixture(scope='module')
+def sim():
+    from fpga_mps_simulator import FPGAMPSSimulator
+    return FPGAMPSSimulator(num_cards=1, chi_max=64, engine='fpga', mode='shots')
+
+
+@pytest.mark.parametrize('n,ops', DIAG_KATS)
+def test_diag_kats(sim, n, ops):
+    res = sim.run(_circ(n, ops), shots=256, seed=1)
+    sv = res.get_statevector()
+    ref = sv_ref.run_circuit(n, ops)
+    fid = abs(np.vdot(sv, ref)) ** 2 / (np.vdot(sv, sv).real * np.vdot(ref, ref).real)
+    assert fid > 1 - 1e-12
+    assert res.mode == 'shots_fpga'
+    assert set(res.get_phase_times()) >= {'fpga_kernel', 'hbm_transfer', 'cpu_svd', 'cpu_qr',
+                                          'cpu_sampling', 'host_other'}
+
+
+def test_ghz3_self_test(sim):
+    qc = QuantumCircuit(3)
+    qc.h(0)
+    qc.cx(0, 1)
+    qc.cx(1, 2)
+    qc.measure_all()
+    res = sim.run(qc, shots=1024, seed=0)
+    c = res.get_counts()
+    assert (c.get('000', 0) + c.get('111', 0)) / 1024 > 0.95
+    assert res.get_bond_dims() == [2, 2] and res.get_max_bond() == 2
+
+
+def test_random_named_circuits_match_oracle(sim):
+    rng = np.random.default_rng(31)
+    names1 = ['h', 'x', 'y', 'z', 's', 'sdg', 't', 'tdg', 'sx', 'sxdg', 'rx', 'ry', 'rz', 'p',
+              'u2', 'u3']
+    names2 = ['cx', 'cz', 'cy', 'ch', 'swap', 'cp', 'crx', 'cry', 'crz', 'csx', 'iswap', 'dcx',
+              'ecr', 'rxx', 'ryy', 'rzz']
+    npar = {'rx': 1, 'ry': 1, 'rz': 1, 'p': 1, 'u2': 2, 'u3': 3, 'cp': 1, 'crx': 1, 'cry': 1,
+            'crz': 1, 'rxx': 1, 'ryy': 1, 'rzz': 1}
+    for trial in range(4):
+        n = 7
+        ops = []
+        for _ in range(60):
+            if rng.random() < 0.5:
+                nm = names1[rng.integers(len(names1))]
+                qs = [int(rng.integers(n))]
+            else:
+                nm = names2[rng.integers(len(names2))]
+                qs = [int(v) for v in rng.choice(n, 2, replace=False)]
+            ps = [float(v) for v in rng.uniform(-3, 3, npar.get(nm, 0))] or None
+            ops.append((nm, qs, ps))
+        if trial == 3:
+            ops.insert(20, ('reset', [2], None))
+        qc = _circ(n, ops)
+        res = sim.run(qc, shots=0, seed=3)
+        gl, _ = mps_ref.compile_ops(mps_ref.circuit_ops(qc), n)
+        ref = mps_ref.run_gate_list(n, gl, 64, 1e-12)
+        assert res.get_bond_dims() == ref.bond_dims()
+        rv = ref.get_statevector()
+        assert np.max(np.abs(res.get_statevector() - rv)) < 1e-10 * max(1, np.max(np.abs(rv)))
+
+
+def test_chi_limit_raises():
+    from fpga_mps_simulator import FPGAMPSSimulator
+    with pytest.raises(ValueError):
+        FPGAMPSSimulator(chi_max=65)
+    with pytest.raises(ValueError):
+        FPGAMPSSimulator(engine='cpu')
