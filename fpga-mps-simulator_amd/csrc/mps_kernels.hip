@@ -15,7 +15,10 @@
 //                tournament through LDS; after convergence column norms are the
 //                singular values, the kept columns (normalised) are U, and
 //                Vh = S^-1 U^H theta is formed from theta in the workspace.
+#include <algorithm>
 #include <cmath>
+#include <utility>
+#include <vector>
 
 #include "qsim_internal.h"
 
@@ -25,8 +28,6 @@ namespace {
 
 constexpr int kMaxChi = 64;              // register SVD holds 2*chi <= 128 rows/cols
 constexpr int kTeamLanes = 16;
-constexpr int kRowsPerLane = 8;          // 16 lanes x 8 rows = 128 rows
-constexpr int kSvdThreads = 1024;        // 64 teams
 constexpr int kMaxSweeps = 60;
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -48,6 +49,14 @@ struct Dims {
     int cl, cm, cr;
 };
 
+// Device copy of a task: the caller's record plus its index in the caller's order
+// (tasks are grouped by size class before upload).
+struct DevTask {
+    qs_mps_task t;
+    int32_t orig;
+    int32_t pad[3];
+};
+
 __device__ __forceinline__ bool load_dims(const qs_mps_task& T, Dims& d) {
     d.cl = T.bond[T.site];
     d.cm = T.bond[T.site + 1];
@@ -62,12 +71,12 @@ __device__ __forceinline__ double2* site_ptr(const qs_mps_task& T, int k) {
 
 // ------------------------------------------------------------------ FP64 contraction
 __global__ void __launch_bounds__(256)
-mps_contract_fp64_kernel(const qs_mps_task* __restrict__ tasks, double2* __restrict__ ws,
+mps_contract_fp64_kernel(const DevTask* __restrict__ tasks, double2* __restrict__ ws,
                          size_t theta_slot, int32_t* __restrict__ status) {
-    const qs_mps_task& T = tasks[blockIdx.x];
+    const qs_mps_task& T = tasks[blockIdx.x].t;
     Dims d;
     if (!load_dims(T, d)) {
-        if (threadIdx.x == 0 && status) status[blockIdx.x] = QS_E_CHI;
+        if (threadIdx.x == 0 && status) status[tasks[blockIdx.x].orig] = QS_E_CHI;
         return;
     }
     const double2* A = site_ptr(T, T.site);
@@ -116,7 +125,7 @@ constexpr int kMaxM = 2 * kMaxChi;                // 128
 constexpr int kMaxK = kMaxChi;                    // 64
 
 __global__ void __launch_bounds__(kMfmaThreads)
-mps_contract_f32_mfma_kernel(const qs_mps_task* __restrict__ tasks, double2* __restrict__ ws,
+mps_contract_f32_mfma_kernel(const DevTask* __restrict__ tasks, double2* __restrict__ ws,
                              size_t theta_slot, int32_t* __restrict__ status) {
     // Operand planes: A as [M][K] (re, im), B as [K][N] (re, im); K padded to 64,
     // M and N padded to 128.  2 * 128*64*4 B * 2 = 128 KiB; T reuses the same space.
@@ -126,10 +135,10 @@ mps_contract_f32_mfma_kernel(const qs_mps_task* __restrict__ tasks, double2* __r
     float* Bre = Aim + kMaxM * kMaxK;
     float* Bim = Bre + kMaxK * kMaxM;
 
-    const qs_mps_task& T = tasks[blockIdx.x];
+    const qs_mps_task& T = tasks[blockIdx.x].t;
     Dims d;
     if (!load_dims(T, d)) {
-        if (threadIdx.x == 0 && status) status[blockIdx.x] = QS_E_CHI;
+        if (threadIdx.x == 0 && status) status[tasks[blockIdx.x].orig] = QS_E_CHI;
         return;
     }
     const int M = 2 * d.cl, K = d.cm, N = 2 * d.cr;
@@ -244,26 +253,35 @@ __device__ __forceinline__ double team_sum(double v) {
     return v;
 }
 
+// Size class CHI (8, 16, 32, 64): theta is at most 2CHI x 2CHI; CHI teams of 16 lanes,
+// CHI/8 rows per lane, LDS exchange buffer of CHI columns of 2CHI rows plus one spare.
+template <int CHI>
 struct SvdShared {
-    double2 buf[(kMaxChi + 1) * 2 * kMaxChi];   // 64 exchange columns + spare, 130 KiB
-    double sval[2 * kMaxChi];
-    int order[2 * kMaxChi];
-    int rank[2 * kMaxChi];
-    double fro_part[kSvdThreads / kTeamLanes];
+    static constexpr int LD = 2 * CHI;
+    double2 buf[(CHI + 1) * LD];
+    double sval[2 * CHI];
+    int order[2 * CHI];
+    int rank[2 * CHI];
+    double fro_part[CHI];
     double fro;
     int rotated;
     int keep;
     double scale;
 };
 
-__global__ void __launch_bounds__(kSvdThreads)
-mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict__ ws,
+template <int CHI>
+__global__ void __launch_bounds__(16 * CHI)
+mps_svd_kernel(const DevTask* __restrict__ tasks, const double2* __restrict__ ws,
                size_t theta_slot, int32_t* __restrict__ status) {
-    __shared__ SvdShared sh;
-    const qs_mps_task& T = tasks[blockIdx.x];
+    constexpr int NT = 16 * CHI;              // threads
+    constexpr int RPL = CHI / 8;              // rows per lane (16 lanes x RPL = 2 CHI rows)
+    constexpr int LD = 2 * CHI;               // LDS column stride
+    __shared__ SvdShared<CHI> sh;
+    const DevTask& DT = tasks[blockIdx.x];
+    const qs_mps_task& T = DT.t;
     Dims d;
-    if (!load_dims(T, d)) {
-        if (threadIdx.x == 0 && status) status[blockIdx.x] = QS_E_CHI;
+    if (!load_dims(T, d) || d.cl > CHI || d.cr > CHI) {
+        if (threadIdx.x == 0 && status) status[DT.orig] = QS_E_CHI;
         return;
     }
     const int m = 2 * d.cl, n = 2 * d.cr;
@@ -272,12 +290,12 @@ mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict_
     const int team = tid / kTeamLanes, l16 = tid % kTeamLanes;
     const bool act = team < P;
     const double2* th = ws + (size_t)blockIdx.x * theta_slot;
-    double2* slot0 = sh.buf;                     // P columns of m
-    double2* spare = sh.buf + kMaxChi * 2 * kMaxChi;
+    double2* slot0 = sh.buf;                     // P columns of 2CHI rows
+    double2* spare = sh.buf + CHI * LD;
 
-    double2 ca[kRowsPerLane], cb[kRowsPerLane];
+    double2 ca[RPL], cb[RPL];
 #pragma unroll
-    for (int k = 0; k < kRowsPerLane; ++k) {
+    for (int k = 0; k < RPL; ++k) {
         const int r = l16 + kTeamLanes * k;
         ca[k] = (act && r < m) ? th[(size_t)r * n + 2 * team] : make_double2(0, 0);
         cb[k] = (act && r < m) ? th[(size_t)r * n + 2 * team + 1] : make_double2(0, 0);
@@ -287,14 +305,14 @@ mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict_
     {
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < kRowsPerLane; ++k)
+        for (int k = 0; k < RPL; ++k)
             s += ca[k].x * ca[k].x + ca[k].y * ca[k].y + cb[k].x * cb[k].x + cb[k].y * cb[k].y;
         s = team_sum(s);
         if (l16 == 0) sh.fro_part[team] = s;
         __syncthreads();
         if (tid == 0) {
             double f = 0.0;
-            for (int t = 0; t < kSvdThreads / kTeamLanes; ++t) f += sh.fro_part[t];
+            for (int t = 0; t < CHI; ++t) f += sh.fro_part[t];
             sh.fro = f;
         }
         __syncthreads();
@@ -311,7 +329,7 @@ mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict_
             if (act) {
                 double al = 0, be = 0, gr = 0, gi = 0;
 #pragma unroll
-                for (int k = 0; k < kRowsPerLane; ++k) {
+                for (int k = 0; k < RPL; ++k) {
                     al = fma(ca[k].x, ca[k].x, fma(ca[k].y, ca[k].y, al));
                     be = fma(cb[k].x, cb[k].x, fma(cb[k].y, cb[k].y, be));
                     gr = fma(ca[k].x, cb[k].x, fma(ca[k].y, cb[k].y, gr));
@@ -337,7 +355,7 @@ mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict_
                     const double2 sem = make_double2(s * er, -s * ei);
                     const double2 sep = make_double2(s * er, s * ei);
 #pragma unroll
-                    for (int k = 0; k < kRowsPerLane; ++k) {
+                    for (int k = 0; k < RPL; ++k) {
                         const double2 x = ca[k], y = cb[k];
                         const double2 sy = cmul(sem, y), sx = cmul(sep, x);
                         ca[k] = make_double2(c * x.x - sy.x, c * x.y - sy.y);
@@ -351,9 +369,9 @@ mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict_
                 // team 0 keeps its top, bot[0] becomes top[1], top[P-1] becomes bot[P-1].
                 if (act) {
 #pragma unroll
-                    for (int k = 0; k < kRowsPerLane; ++k) {
+                    for (int k = 0; k < RPL; ++k) {
                         const int r = l16 + kTeamLanes * k;
-                        slot0[team * 128 + r] = cb[k];
+                        slot0[team * LD + r] = cb[k];
                         if (team == 0) spare[r] = cb[k];
                     }
                 }
@@ -361,28 +379,28 @@ mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict_
                 if (act) {
                     if (team < P - 1) {
 #pragma unroll
-                        for (int k = 0; k < kRowsPerLane; ++k)
-                            cb[k] = slot0[(team + 1) * 128 + l16 + kTeamLanes * k];
+                        for (int k = 0; k < RPL; ++k)
+                            cb[k] = slot0[(team + 1) * LD + l16 + kTeamLanes * k];
                     } else {
 #pragma unroll
-                        for (int k = 0; k < kRowsPerLane; ++k) cb[k] = ca[k];
+                        for (int k = 0; k < RPL; ++k) cb[k] = ca[k];
                     }
                 }
                 __syncthreads();
                 if (act) {
 #pragma unroll
-                    for (int k = 0; k < kRowsPerLane; ++k)
-                        slot0[team * 128 + l16 + kTeamLanes * k] = ca[k];
+                    for (int k = 0; k < RPL; ++k)
+                        slot0[team * LD + l16 + kTeamLanes * k] = ca[k];
                 }
                 __syncthreads();
                 if (act) {
                     if (team >= 2) {
 #pragma unroll
-                        for (int k = 0; k < kRowsPerLane; ++k)
-                            ca[k] = slot0[(team - 1) * 128 + l16 + kTeamLanes * k];
+                        for (int k = 0; k < RPL; ++k)
+                            ca[k] = slot0[(team - 1) * LD + l16 + kTeamLanes * k];
                     } else if (team == 1) {
 #pragma unroll
-                        for (int k = 0; k < kRowsPerLane; ++k) ca[k] = spare[l16 + kTeamLanes * k];
+                        for (int k = 0; k < RPL; ++k) ca[k] = spare[l16 + kTeamLanes * k];
                     }
                 }
                 __syncthreads();
@@ -393,19 +411,17 @@ mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict_
         __syncthreads();
         if (!any) break;
     }
-    // Column identities after `sweeps` complete tournaments are not needed: the
-    // columns are ranked by norm below, and only norms and vectors are used.
 
-    // Final column norms.
+    // Final column norms.  Team t owns column slots 2t (a) and 2t+1 (b); columns are
+    // ranked by norm, so their original indices are not needed.
     double na = 0, nb = 0;
 #pragma unroll
-    for (int k = 0; k < kRowsPerLane; ++k) {
+    for (int k = 0; k < RPL; ++k) {
         na = fma(ca[k].x, ca[k].x, fma(ca[k].y, ca[k].y, na));
         nb = fma(cb[k].x, cb[k].x, fma(cb[k].y, cb[k].y, nb));
     }
     na = sqrt(team_sum(na));
     nb = sqrt(team_sum(nb));
-    // Column slot ids: team t owns slots 2t (a) and 2t+1 (b) for the ranking.
     if (act && l16 == 0) {
         sh.sval[2 * team] = na;
         sh.sval[2 * team + 1] = nb;
@@ -459,7 +475,7 @@ mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict_
         sh.scale = scale;
         *T.dlog = dlog;
         T.bond[T.site + 1] = keep;
-        if (status) status[blockIdx.x] = (sweeps >= kMaxSweeps) ? QS_E_NOCONV : QS_OK;
+        if (status) status[DT.orig] = (sweeps >= kMaxSweeps) ? QS_E_NOCONV : QS_OK;
     }
     __syncthreads();
     const int keep = sh.keep;
@@ -469,39 +485,39 @@ mps_svd_kernel(const qs_mps_task* __restrict__ tasks, const double2* __restrict_
         const int ra = sh.rank[2 * team], rb = sh.rank[2 * team + 1];
         const double ia = na > 0 ? 1.0 / na : 0.0, ib = nb > 0 ? 1.0 / nb : 0.0;
 #pragma unroll
-        for (int k = 0; k < kRowsPerLane; ++k) {
+        for (int k = 0; k < RPL; ++k) {
             const int r = l16 + kTeamLanes * k;
-            if (ra < keep) sh.buf[ra * 128 + r] = make_double2(ca[k].x * ia, ca[k].y * ia);
-            if (rb < keep) sh.buf[rb * 128 + r] = make_double2(cb[k].x * ib, cb[k].y * ib);
+            if (ra < keep) sh.buf[ra * LD + r] = make_double2(ca[k].x * ia, ca[k].y * ia);
+            if (rb < keep) sh.buf[rb * LD + r] = make_double2(cb[k].x * ib, cb[k].y * ib);
         }
     }
     __syncthreads();
     // A' = U sqrt(S_t), (chi_l, 2, keep) = m x keep, row-major.
     double2* Ao = site_ptr(T, T.site);
-    for (int idx = tid; idx < m * keep; idx += kSvdThreads) {
+    for (int idx = tid; idx < m * keep; idx += NT) {
         const int r = idx / keep, k = idx - r * keep;
         const double f = sqrt(sh.sval[sh.order[k]] * scale);
-        const double2 u = sh.buf[k * 128 + r];
+        const double2 u = sh.buf[k * LD + r];
         Ao[idx] = make_double2(u.x * f, u.y * f);
     }
     // B' = sqrt(S_t) Vh = sqrt(S_t)/S * U^H theta, (keep, 2, chi_r) = keep x n.
     double2* Bo = site_ptr(T, T.site + 1);
     {
-        const int c = tid % 128, kg = tid / 128;    // 8 groups of rows of B'
+        const int c = tid % LD, kg = tid / LD;      // 8 groups of rows of B'
         if (c < n) {
-            double2 acc[kMaxChi / 8];
+            double2 acc[CHI / 8];
 #pragma unroll
-            for (int q = 0; q < kMaxChi / 8; ++q) acc[q] = make_double2(0, 0);
+            for (int q = 0; q < CHI / 8; ++q) acc[q] = make_double2(0, 0);
             for (int r = 0; r < m; ++r) {
                 const double2 t = th[(size_t)r * n + c];
 #pragma unroll
-                for (int q = 0; q < kMaxChi / 8; ++q) {
+                for (int q = 0; q < CHI / 8; ++q) {
                     const int k = kg + 8 * q;
-                    if (k < keep) acc[q] = cjmadd(acc[q], sh.buf[k * 128 + r], t);
+                    if (k < keep) acc[q] = cjmadd(acc[q], sh.buf[k * LD + r], t);
                 }
             }
 #pragma unroll
-            for (int q = 0; q < kMaxChi / 8; ++q) {
+            for (int q = 0; q < CHI / 8; ++q) {
                 const int k = kg + 8 * q;
                 if (k < keep) {
                     const double s = sh.sval[sh.order[k]];
@@ -541,10 +557,28 @@ int qs_mps_max_chi(void) { return kMaxChi; }
 
 size_t qs_mps_workspace_bytes(int ntasks, int chi_max) {
     if (ntasks <= 0 || chi_max <= 0) return 0;
-    const size_t head = sv_blob_offset((size_t)ntasks * sizeof(qs_mps_task));
+    const size_t head = sv_blob_offset((size_t)ntasks * sizeof(DevTask));
     const size_t slot = (size_t)4 * chi_max * chi_max;
     return head + (size_t)ntasks * slot * sizeof(double2);
 }
+
+namespace {
+int chi_class(int chi) { return chi <= 8 ? 8 : chi <= 16 ? 16 : chi <= 32 ? 32 : 64; }
+
+void launch_svd(int cls, const DevTask* t, int n, const double2* theta, size_t slot,
+                int32_t* status, hipStream_t hs) {
+    switch (cls) {
+        case 8: hipLaunchKernelGGL(mps_svd_kernel<8>, dim3(n), dim3(16 * 8), 0, hs, t, theta, slot,
+                                   status); break;
+        case 16: hipLaunchKernelGGL(mps_svd_kernel<16>, dim3(n), dim3(16 * 16), 0, hs, t, theta,
+                                    slot, status); break;
+        case 32: hipLaunchKernelGGL(mps_svd_kernel<32>, dim3(n), dim3(16 * 32), 0, hs, t, theta,
+                                    slot, status); break;
+        default: hipLaunchKernelGGL(mps_svd_kernel<64>, dim3(n), dim3(16 * 64), 0, hs, t, theta,
+                                    slot, status); break;
+    }
+}
+}  // namespace
 
 int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, void* workspace,
                         size_t ws_bytes, int32_t* status_dev, void* stream) {
@@ -566,20 +600,38 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
         QS_REQUIRE(t.site >= 0, QS_E_ARG, "task %d: negative site", i);
         QS_REQUIRE(std::isfinite(t.cutoff) && t.cutoff >= 0, QS_E_ARG, "task %d: bad cutoff", i);
         if (t.chi_max > chi) chi = t.chi_max;
-        for (int j = 0; j < i; ++j) {
-            const qs_mps_task& u = tasks[j];
-            if (u.sites == t.sites && std::abs(u.site - t.site) < 2)
-                QS_REQUIRE(false, QS_E_ARG, "tasks %d and %d share a site", j, i);
-        }
+    }
+    {
+        // No two tasks of one chain may touch the same site.
+        std::vector<std::pair<std::pair<const void*, int>, int>> keys(ntasks);
+        for (int i = 0; i < ntasks; ++i) keys[i] = {{tasks[i].sites, tasks[i].site}, i};
+        std::sort(keys.begin(), keys.end());
+        for (int i = 1; i < ntasks; ++i)
+            if (keys[i].first.first == keys[i - 1].first.first &&
+                keys[i].first.second - keys[i - 1].first.second < 2)
+                QS_REQUIRE(false, QS_E_ARG, "tasks %d and %d share a site", keys[i - 1].second,
+                           keys[i].second);
     }
     const size_t need = qs_mps_workspace_bytes(ntasks, chi);
     QS_REQUIRE(ws_bytes >= need, QS_E_RANGE, "workspace %zu bytes < %zu needed", ws_bytes, need);
+    // Group by SVD size class (stable), keep each task's original index for statuses.
+    std::vector<DevTask> dt(ntasks);
+    std::vector<int> cls(ntasks);
+    int k = 0;
+    for (int c : {8, 16, 32, 64})
+        for (int i = 0; i < ntasks; ++i)
+            if (chi_class(tasks[i].chi_max) == c) {
+                dt[k].t = tasks[i];
+                dt[k].orig = i;
+                cls[k] = c;
+                ++k;
+            }
     hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
     char* ws = static_cast<char*>(workspace);
-    qs_mps_task* dtasks = reinterpret_cast<qs_mps_task*>(ws);
-    double2* theta = reinterpret_cast<double2*>(ws + sv_blob_offset((size_t)ntasks * sizeof(qs_mps_task)));
+    DevTask* dtasks = reinterpret_cast<DevTask*>(ws);
+    double2* theta = reinterpret_cast<double2*>(ws + sv_blob_offset((size_t)ntasks * sizeof(DevTask)));
     const size_t slot = (size_t)4 * chi * chi;
-    QS_HIP(hipMemcpyAsync(dtasks, tasks, (size_t)ntasks * sizeof(qs_mps_task),
+    QS_HIP(hipMemcpyAsync(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask),
                           hipMemcpyHostToDevice, hs));
     if (precision == QS_PREC_FP32_MFMA)
         hipLaunchKernelGGL(mps_contract_f32_mfma_kernel, dim3(ntasks), dim3(kMfmaThreads), 0, hs,
@@ -588,9 +640,13 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
         hipLaunchKernelGGL(mps_contract_fp64_kernel, dim3(ntasks), dim3(256), 0, hs, dtasks,
                            theta, slot, status_dev);
     QS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(mps_svd_kernel, dim3(ntasks), dim3(kSvdThreads), 0, hs, dtasks, theta, slot,
-                       status_dev);
-    QS_HIP(hipGetLastError());
+    for (int b = 0; b < ntasks;) {
+        int e = b;
+        while (e < ntasks && cls[e] == cls[b]) ++e;
+        launch_svd(cls[b], dtasks + b, e - b, theta + (size_t)b * slot, slot, status_dev, hs);
+        QS_HIP(hipGetLastError());
+        b = e;
+    }
     return QS_OK;
 }
 

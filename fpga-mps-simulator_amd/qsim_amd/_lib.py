@@ -72,6 +72,13 @@ _PROTOS = {
                                          ctypes.c_void_p]),
     'qs_sv_probabilities': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                            ctypes.c_void_p]),
+    'qs_sv_pauli_inner': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                         _c_int32_p, _c_int32_p, ctypes.c_int, _c_double_p,
+                                         ctypes.c_void_p]),
+    'qs_sv_inner': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_double_p,
+                                   ctypes.c_void_p]),
+    'qs_sv_apply_tfim': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_void_p]),
     'qs_mps_max_chi': (ctypes.c_int, []),
     'qs_mps_workspace_bytes': (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
     'qs_mps_update_batch': (ctypes.c_int, [ctypes.POINTER(QsMpsTask), ctypes.c_int,

@@ -125,6 +125,21 @@ int qs_sv_reset_qubit(void* state, int nq, int q, void* stream);
 /* probs[i] = |a_i|^2 (float64, device buffer of 2^nq). */
 int qs_sv_probabilities(const void* state, int nq, double* probs, void* stream);
 
+/* ------------------------------------------------- statevector: gradients */
+/* No counterpart in the reference (it has no differentiation); these serve the adjoint
+ * gradient of <psi|H|psi> for circuits of rotations exp(-i theta P / 2). */
+
+/* out[2k], out[2k+1] = Re, Im of <lam| P_k |psi>, P_k = X (0), Y (1) or Z (2) on qubits[k].
+ * Synchronous. */
+int qs_sv_pauli_inner(const void* lam, const void* psi, int nq, const int32_t* kinds,
+                      const int32_t* qubits, int count, double* out_host, void* stream);
+
+/* out_host[0], out_host[1] = Re, Im of <a|b>.  Synchronous. */
+int qs_sv_inner(const void* a, const void* b, int nq, double* out_host, void* stream);
+
+/* out = H psi, H = -J sum_i Z_i Z_{i+1} - h sum_i X_i (open chain); out != psi. */
+int qs_sv_apply_tfim(const void* psi, void* out, int nq, double J, double h, void* stream);
+
 /* --------------------------------------------------------------------- MPS */
 
 #define QS_PREC_FP64       0   /* contraction FP64, SVD FP64                        */

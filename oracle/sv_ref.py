@@ -398,3 +398,37 @@ def u3_cz_layers(n: int, depth: int, seed: int):
 def sv_benchmark_circuit(n: int = 28, depth: int = 20, seed: int = 1234):
     """depth layers of U3 + CZ ladder followed by an n-qubit QFT."""
     return u3_cz_layers(n, depth, seed) + qft_gates(n)
+
+
+# ----------------------------------------------------------------------------
+# TFIM energy (builder-added observable; no reference counterpart)
+# ----------------------------------------------------------------------------
+
+def tfim_energy_sv(state: np.ndarray, j_coupling: float = 1.0, h_field: float = 1.0) -> float:
+    """<psi|H|psi>/<psi|psi> for H = -J sum Z_i Z_{i+1} - h sum X_i (open chain)."""
+    n = int(round(math.log2(state.size)))
+    idx = np.arange(state.size, dtype=np.int64)
+    p = np.abs(state) ** 2
+    zz = np.zeros(state.size)
+    for i in range(n - 1):
+        zz += 1.0 - 2.0 * (((idx >> i) ^ (idx >> (i + 1))) & 1)
+    e = -j_coupling * float(np.dot(p, zz))
+    for i in range(n):
+        e -= h_field * float(np.real(np.vdot(state, state[idx ^ (1 << i)])))
+    return e / float(p.sum())
+
+
+def hea_gates(n: int, theta, layers: int = 4):
+    """layers x [Ry(theta) every qubit, Rz(theta) every qubit, CZ ladder]."""
+    g = []
+    k = 0
+    for _ in range(layers):
+        for q in range(n):
+            g.append(('ry', [q], [float(theta[k + q])]))
+        k += n
+        for q in range(n):
+            g.append(('rz', [q], [float(theta[k + q])]))
+        k += n
+        for q in range(n - 1):
+            g.append(('cz', [q, q + 1], None))
+    return g

@@ -35,8 +35,10 @@ def test_abi_version_and_limits():
     assert lib.qs_sv_max_qubits() == 34
     assert lib.qs_mps_max_chi() == 64
     ntasks, chi = 5, 64
-    head = ((ntasks * ctypes.sizeof(_lib.QsMpsTask) + 255) // 256) * 256
-    assert lib.qs_mps_workspace_bytes(ntasks, chi) == head + ntasks * 4 * chi * chi * 16
+    ws = lib.qs_mps_workspace_bytes(ntasks, chi)
+    theta = ntasks * 4 * chi * chi * 16          # one (2chi x 2chi) complex128 slot per task
+    assert theta < ws < theta + ntasks * 1024 + 256
+    assert lib.qs_mps_workspace_bytes(0, chi) == 0
     assert ctypes.sizeof(_lib.QsGate) == 16 + 8 + 256
     assert ctypes.sizeof(_lib.QsMpsTask) == 8 * 3 + 8 + 4 + 4 + 8 + 256
 
