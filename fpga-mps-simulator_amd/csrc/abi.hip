@@ -97,18 +97,20 @@ double f32_word(int32_t w) {
 // Uploads the plan's tables with a stream-ordered allocation and launches it.
 int run_plan(double2* state, int nq, const SvPlan& plan, hipStream_t stream) {
     if (plan.passes.empty()) return QS_OK;
-    const size_t pbytes = sv_blob_offset(plan.passes.size() * sizeof(DevPass));
-    const size_t sbytes = sv_blob_offset(plan.stages.size() * sizeof(DevStage));
-    const size_t gbytes = std::max<size_t>(plan.gates.size(), 1) * sizeof(DevGate);
-    std::vector<char> blob(pbytes + sbytes + gbytes, 0);
-    std::memcpy(blob.data(), plan.passes.data(), plan.passes.size() * sizeof(DevPass));
-    std::memcpy(blob.data() + pbytes, plan.stages.data(), plan.stages.size() * sizeof(DevStage));
-    std::memcpy(blob.data() + pbytes + sbytes, plan.gates.data(),
-                plan.gates.size() * sizeof(DevGate));
+    const SvBlobLayout L = sv_blob_layout(plan, nq);
+    std::vector<char> host(L.factors, 0);
+    auto put = [&](size_t off, const void* src, size_t bytes) {
+        if (bytes) std::memcpy(host.data() + off, src, bytes);
+    };
+    put(L.passes, plan.passes.data(), plan.passes.size() * sizeof(DevPass));
+    put(L.stages, plan.stages.data(), plan.stages.size() * sizeof(DevStage));
+    put(L.gates, plan.gates.data(), plan.gates.size() * sizeof(DevGate));
+    put(L.phases, plan.phases.data(), plan.phases.size() * sizeof(DevPhase));
+    put(L.terms, plan.terms.data(), plan.terms.size() * sizeof(DevTerm));
     void* dev = nullptr;
-    QS_HIP(hipMallocAsync(&dev, blob.size(), stream));
+    QS_HIP(hipMallocAsync(&dev, L.total, stream));
     // Pageable source: the runtime stages the bytes before returning.
-    QS_HIP(hipMemcpyAsync(dev, blob.data(), blob.size(), hipMemcpyHostToDevice, stream));
+    QS_HIP(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, stream));
     int st = sv_launch_plan(state, nq, plan, dev, stream);
     QS_HIP(hipFreeAsync(dev, stream));
     return st;
@@ -261,7 +263,26 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
                    << ",\"r1\":" << G.r1 << ",\"qa\":" << G.qa << ",\"qb\":" << G.qb
                    << ",\"cmask\":" << G.cmask << ",\"m\":[";
                 for (int k = 0; k < 32; ++k) os << (k ? "," : "") << G.m[k];
-                os << "]}";
+                os << "]";
+                if (G.kind == DG_PHASE) {
+                    const DevPhase& Q = plan.phases[P.phase0 + G.qa];
+                    os << ",\"phase\":{\"sign\":" << Q.sign << ",\"slot\":" << Q.slot
+                       << ",\"c\":[" << Q.c[0] << "," << Q.c[1] << "],\"kr\":[";
+                    for (int k = 0; k < 4; ++k)
+                        os << (k ? "," : "") << Q.kr[k][0] << "," << Q.kr[k][1];
+                    os << "],\"crr\":[";
+                    for (int k = 0; k < 6; ++k)
+                        os << (k ? "," : "") << Q.crr[k][0] << "," << Q.crr[k][1];
+                    os << "],\"terms\":[";
+                    const int n = Q.nt + Q.nb;
+                    for (int k = 0; k < n; ++k) {
+                        const DevTerm& T = plan.terms[P.term0 + (k < Q.nt ? Q.t0 + k : Q.b0 + k - Q.nt)];
+                        os << (k ? "," : "") << "[" << T.cls << "," << T.a << "," << T.b << ","
+                           << T.re << "," << T.im << "]";
+                    }
+                    os << "]}";
+                }
+                os << "}";
             }
             os << "]}";
         }

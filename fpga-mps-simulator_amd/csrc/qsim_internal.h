@@ -44,16 +44,49 @@ constexpr int SV_LOW_BITS = 4;         // qubits 0..3 are in every tile (256 B r
 constexpr int SV_MAX_STAGES = 24;      // register layouts per pass
 
 // Device gate of a fused pass.  U1/U2 name register bits (positions within the
-// thread's 2^R register block); DIAG names global qubits.
-enum : int32_t { DG_U1 = 0, DG_U2 = 1, DG_DIAG1 = 2, DG_DIAG2 = 3, DG_SIGN2 = 4 };
+// thread's 2^R register block); DIAG1/DIAG2 name global qubits (general diagonal,
+// evaluated per amplitude); PHASE is a group of unitary diagonal gates described by
+// a DevPhase record (index in qa).
+enum : int32_t { DG_U1 = 0, DG_U2 = 1, DG_DIAG1 = 2, DG_DIAG2 = 3, DG_PHASE = 5 };
 
 struct alignas(16) DevGate {
     int32_t kind;
     int32_t r0, r1;        // register bits (U1: r0; U2: r0 = MSB of row index)
-    int32_t qa, qb;        // DIAG: global qubits (qa = MSB for DIAG2)
+    int32_t qa, qb;        // DIAG: global qubits (qa = MSB for DIAG2); PHASE: qa = record
     int32_t pad;
     uint64_t cmask;        // global control mask
     double m[32];
+};
+
+// A unitary diagonal group is a product of phase factors f^(bit products).  Factors are
+// classified by where their qubits live in the pass: block bits (not in the tile),
+// thread bits (tile positions that index the thread) or register bits.
+//   block classes, folded per workgroup by sv_phase_block_kernel into 13 factors
+//     [A_B, lambda_0..lambda_7 (thread bits), kappa_0..kappa_3 (register bits)]:
+//     PT_BB: a, b global (b < 0: one-body)       -> A_B
+//     PT_BT: a thread bit, b global              -> lambda_a
+//     PT_BR: a register bit, b global            -> kappa_a
+//   thread classes, evaluated per thread:
+//     PT_T1: a thread bit                        -> A
+//     PT_TT: a, b thread bits                    -> A
+//     PT_TR: a register bit, b thread bit        -> kappa_a
+enum : int32_t { PT_BB = 0, PT_BT = 1, PT_BR = 2, PT_T1 = 3, PT_TT = 4, PT_TR = 5 };
+constexpr int PHASE_BLOCK_VALUES = 13;
+
+struct alignas(16) DevTerm {
+    int32_t cls, a, b, pad;
+    double re, im;
+};
+
+struct alignas(16) DevPhase {
+    int32_t t0, nt;        // thread-class terms in the pass's term table
+    int32_t b0, nb;        // block-class terms in the pass's term table
+    int32_t slot;          // block-factor slot of this group in the pass (-1: none)
+    int32_t sign;          // 1: every factor is +1 or -1 (parity arithmetic)
+    int32_t pad[2];
+    double c[2];           // constant factor
+    double kr[4][2];       // constant per-register-bit factors
+    double crr[6][2];      // register-pair factors: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
 };
 
 struct DevStage {
@@ -67,6 +100,12 @@ struct DevPass {
     uint64_t tile_mask;
     int32_t stage0;            // first stage in the stage array
     int32_t gate0;             // first gate in the gate array
+    int32_t phase0;            // first phase record
+    int32_t term0;             // first term
+    int32_t nphase;            // phase records of this pass
+    int32_t nslot;             // block-factor slots of this pass
+    int32_t slot0;             // first slot in the block-factor table
+    int32_t pad;
 };
 
 // Host-side logical gate, after name -> matrix conversion.
@@ -81,11 +120,20 @@ struct SvPlan {
     std::vector<DevPass> passes;
     std::vector<DevStage> stages;
     std::vector<DevGate> gates;
+    std::vector<DevPhase> phases;
+    std::vector<DevTerm> terms;
+    int nslot_total = 0;       // block-factor slots over all passes
 };
 
-// Device blob layout of an uploaded plan: [passes][stages][gates], each table
-// starting on a 256-byte boundary.
+// Device blob layout of an uploaded plan: [passes][stages][gates][phases][terms], each
+// table starting on a 256-byte boundary, followed by the block-factor table
+// (nslot_total x 2^(nq-K) x 13 complex) written by the prologue kernels.
 inline size_t sv_blob_offset(size_t bytes) { return ((bytes + 255) / 256) * 256; }
+
+struct SvBlobLayout {
+    size_t passes, stages, gates, phases, terms, factors, total;
+};
+SvBlobLayout sv_blob_layout(const SvPlan& plan, int nq);
 
 // Builds the fused-pass plan.  Returns QS_OK or an error code (message set).
 int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan);

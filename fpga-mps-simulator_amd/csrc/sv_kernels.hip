@@ -176,15 +176,6 @@ __device__ __forceinline__ void apply_diag(double2 (&a)[1 << R], const DevGate& 
                                            const Layout<R>& L) {
     const uint64_t cm = G.cmask;
     const int qa = G.qa, qb = G.qb;
-    if (G.kind == DG_SIGN2) {
-#pragma unroll
-        for (int j = 0; j < (1 << R); ++j) {
-            const uint64_t g = L.glob(j);
-            const bool hit = ((g >> qa) & (g >> qb) & 1ull) != 0;
-            if (hit && (cm == 0 || (g & cm) == cm)) a[j] = make_double2(-a[j].x, -a[j].y);
-        }
-        return;
-    }
     if (G.kind == DG_DIAG1) {
         const double2 d0 = make_double2(G.m[0], G.m[1]), d1 = make_double2(G.m[2], G.m[3]);
 #pragma unroll
@@ -208,10 +199,140 @@ __device__ __forceinline__ void apply_diag(double2 (&a)[1 << R], const DevGate& 
     }
 }
 
+// Mask of register indices j (bit j set) whose register bit r is 1.
+__device__ __forceinline__ constexpr unsigned reg_col_mask(int r) {
+    return r == 0 ? 0xAAAAu : r == 1 ? 0xCCCCu : r == 2 ? 0xF0F0u : 0xFF00u;
+}
+
+__device__ __forceinline__ bool is_unit(double2 z) { return z.x == 1.0 && z.y == 0.0; }
+
+// Applies a unitary diagonal group (DevPhase).  Block-class factors come from the
+// prologue's table, thread-class factors are folded per thread, register factors are
+// applied to the halves/quarters of the register block they select.
+template <int R>
+__device__ __forceinline__ void apply_phase(double2 (&a)[1 << R], const DevPhase& Q,
+                                            const DevTerm* __restrict__ terms,
+                                            const double2* __restrict__ bf, int t) {
+    constexpr int NR = 1 << R;
+    double2 A = make_double2(Q.c[0], Q.c[1]);
+    double2 kap[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) kap[r] = make_double2(1.0, 0.0);
+    if (bf) {
+        A = cmul(A, bf[0]);
+        for (int q = 0; q < 8; ++q)
+            if ((t >> q) & 1) A = cmul(A, bf[1 + q]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) kap[r] = bf[9 + r];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) kap[r] = cmul(kap[r], make_double2(Q.kr[r][0], Q.kr[r][1]));
+    for (int k = 0; k < Q.nt; ++k) {
+        const DevTerm& T = terms[Q.t0 + k];
+        const double2 f = make_double2(T.re, T.im);
+        if (T.cls == PT_T1) {
+            if ((t >> T.a) & 1) A = cmul(A, f);
+        } else if (T.cls == PT_TT) {
+            if (((t >> T.a) & (t >> T.b)) & 1) A = cmul(A, f);
+        } else {  // PT_TR
+            if ((t >> T.b) & 1) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (r == T.a) kap[r] = cmul(kap[r], f);
+            }
+        }
+    }
+    if (Q.sign) {
+        // every factor is +-1: build the 16-bit parity mask of the register block
+        unsigned M = (A.x < 0.0) ? 0xFFFFu : 0u;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (kap[r].x < 0.0) M ^= reg_col_mask(r);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int s2 = r + 1; s2 < R; ++s2) {
+                const int pi = r == 0 ? s2 - 1 : (r == 1 ? s2 + 1 : 5);
+                if (Q.crr[pi][0] < 0.0) M ^= reg_col_mask(r) & reg_col_mask(s2);
+            }
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+            if ((M >> j) & 1u) a[j] = make_double2(-a[j].x, -a[j].y);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) a[j] = cmul(A, a[j]);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+            if (j & (1 << r)) a[j] = cmul(kap[r], a[j]);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int s2 = r + 1; s2 < R; ++s2) {
+            const int pi = r == 0 ? s2 - 1 : (r == 1 ? s2 + 1 : 5);
+            const double2 f = make_double2(Q.crr[pi][0], Q.crr[pi][1]);
+            if (!is_unit(f)) {
+#pragma unroll
+                for (int j = 0; j < NR; ++j)
+                    if ((j >> r) & (j >> s2) & 1) a[j] = cmul(f, a[j]);
+            }
+        }
+}
+
+// Block-class phase factors of every phase group of one pass, one thread per
+// (slot, workgroup): [A_B, lambda_0..7, kappa_0..3].
+__global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
+                                      const DevPhase* __restrict__ phases,
+                                      const DevTerm* __restrict__ terms,
+                                      double2* __restrict__ factors, uint32_t nblocks) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nslot = P->nslot;
+    if (gid >= (uint64_t)nslot * nblocks) return;
+    const int slot = (int)(gid / nblocks);
+    const uint32_t blk = (uint32_t)(gid - (uint64_t)slot * nblocks);
+    const DevPhase* Q = nullptr;
+    for (int k = 0; k < P->nphase; ++k)
+        if (phases[P->phase0 + k].slot == slot) Q = &phases[P->phase0 + k];
+    uint64_t base = 0;
+    {
+        const uint64_t tmask = P->tile_mask;
+        uint64_t b = blk;
+        for (int q = 0; q < nq && b; ++q) {
+            if ((tmask >> q) & 1ull) continue;
+            base |= (b & 1ull) << q;
+            b >>= 1;
+        }
+    }
+    double2 v[PHASE_BLOCK_VALUES];
+#pragma unroll
+    for (int k = 0; k < PHASE_BLOCK_VALUES; ++k) v[k] = make_double2(1.0, 0.0);
+    const DevTerm* tm = terms + P->term0 + Q->b0;
+    for (int k = 0; k < Q->nb; ++k) {
+        const DevTerm& T = tm[k];
+        const double2 f = make_double2(T.re, T.im);
+        if (T.cls == PT_BB) {
+            const bool on = ((base >> T.a) & 1ull) && (T.b < 0 || ((base >> T.b) & 1ull));
+            if (on) v[0] = cmul(v[0], f);
+        } else if ((base >> T.b) & 1ull) {
+            const int idx = (T.cls == PT_BT) ? 1 + T.a : 9 + T.a;
+#pragma unroll
+            for (int i = 1; i < PHASE_BLOCK_VALUES; ++i)
+                if (i == idx) v[i] = cmul(v[i], f);
+        }
+    }
+    double2* out = factors + ((size_t)(P->slot0 + slot) * nblocks + blk) * PHASE_BLOCK_VALUES;
+#pragma unroll
+    for (int k = 0; k < PHASE_BLOCK_VALUES; ++k) out[k] = v[k];
+}
+
 template <int R>
 __global__ void __launch_bounds__(1 << (SV_TILE_BITS - SV_REG_BITS))
 sv_pass_kernel(double2* __restrict__ state, int nq, const DevPass* __restrict__ P,
-               const DevStage* __restrict__ stages, const DevGate* __restrict__ gates) {
+               const DevStage* __restrict__ stages, const DevGate* __restrict__ gates,
+               const DevPhase* __restrict__ phases, const DevTerm* __restrict__ terms,
+               const double2* __restrict__ factors) {
     constexpr int NR = 1 << R;
     __shared__ double2 lds[kLdsSlots];
 
@@ -220,6 +341,9 @@ sv_pass_kernel(double2* __restrict__ state, int nq, const DevPass* __restrict__ 
     const int nstage = P->nstage;
     const DevStage* st = stages + P->stage0;
     const DevGate* gt = gates + P->gate0;
+    const DevPhase* ph = phases + P->phase0;
+    const DevTerm* tm = terms + P->term0;
+    const uint32_t nblocks = gridDim.x;
     // Block base: bits of blockIdx.x deposited into the non-tile qubits, ascending.
     uint64_t base = 0;
     if (!P->tile_is_all) {
@@ -269,6 +393,15 @@ sv_pass_kernel(double2* __restrict__ state, int nq, const DevPass* __restrict__ 
             switch (G.kind) {
                 case DG_U1: apply_u1<R>(a, G, Lcur); break;
                 case DG_U2: apply_u2<R>(a, G, Lcur); break;
+                case DG_PHASE: {
+                    const DevPhase& Q = ph[G.qa];
+                    const double2* bf =
+                        Q.slot >= 0 ? factors + ((size_t)(P->slot0 + Q.slot) * nblocks + blockIdx.x) *
+                                                    PHASE_BLOCK_VALUES
+                                    : nullptr;
+                    apply_phase<R>(a, Q, tm, bf, t);
+                    break;
+                }
                 default: apply_diag<R>(a, G, Lcur); break;
             }
         }
@@ -350,25 +483,33 @@ int grid_for(uint64_t n, int threads) {
 int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_blob,
                    hipStream_t stream) {
     const char* blob = reinterpret_cast<const char*>(dev_blob);
-    const size_t pbytes = sv_blob_offset(plan.passes.size() * sizeof(DevPass));
-    const size_t sbytes = sv_blob_offset(plan.stages.size() * sizeof(DevStage));
-    const DevPass* dpasses = reinterpret_cast<const DevPass*>(blob);
-    const DevStage* dstages = reinterpret_cast<const DevStage*>(blob + pbytes);
-    const DevGate* dgates = reinterpret_cast<const DevGate*>(blob + pbytes + sbytes);
+    const SvBlobLayout L = sv_blob_layout(plan, nq);
+    const DevPass* dpasses = reinterpret_cast<const DevPass*>(blob + L.passes);
+    const DevStage* dstages = reinterpret_cast<const DevStage*>(blob + L.stages);
+    const DevGate* dgates = reinterpret_cast<const DevGate*>(blob + L.gates);
+    const DevPhase* dphases = reinterpret_cast<const DevPhase*>(blob + L.phases);
+    const DevTerm* dterms = reinterpret_cast<const DevTerm*>(blob + L.terms);
+    double2* dfactors = reinterpret_cast<double2*>(const_cast<char*>(blob) + L.factors);
     for (size_t p = 0; p < plan.passes.size(); ++p) {
         const DevPass& P = plan.passes[p];
         const DevPass* dP = dpasses + p;
         const unsigned blocks = P.tile_is_all ? 1u : (1u << (nq - P.K));
         const unsigned threads = 1u << (P.K - P.R);
+        if (P.nslot > 0) {
+            const uint64_t work = (uint64_t)P.nslot * blocks;
+            hipLaunchKernelGGL(sv_phase_block_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256),
+                               0, stream, nq, dP, dphases, dterms, dfactors, blocks);
+            QS_HIP(hipGetLastError());
+        }
         switch (P.R) {
             case 1: hipLaunchKernelGGL(sv_pass_kernel<1>, dim3(blocks), dim3(threads), 0, stream,
-                                       state, nq, dP, dstages, dgates); break;
+                                       state, nq, dP, dstages, dgates, dphases, dterms, dfactors); break;
             case 2: hipLaunchKernelGGL(sv_pass_kernel<2>, dim3(blocks), dim3(threads), 0, stream,
-                                       state, nq, dP, dstages, dgates); break;
+                                       state, nq, dP, dstages, dgates, dphases, dterms, dfactors); break;
             case 3: hipLaunchKernelGGL(sv_pass_kernel<3>, dim3(blocks), dim3(threads), 0, stream,
-                                       state, nq, dP, dstages, dgates); break;
+                                       state, nq, dP, dstages, dgates, dphases, dterms, dfactors); break;
             default: hipLaunchKernelGGL(sv_pass_kernel<4>, dim3(blocks), dim3(threads), 0, stream,
-                                        state, nq, dP, dstages, dgates); break;
+                                        state, nq, dP, dstages, dgates, dphases, dterms, dfactors); break;
         }
         QS_HIP(hipGetLastError());
     }

@@ -10,6 +10,7 @@
 // must lie in the tile; diagonal gates and controls may use any qubit.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 
 #include "qsim_internal.h"
@@ -26,13 +27,6 @@ struct Item {
     uint64_t touch;     // every qubit the gate reads: targets, diag qubits, controls
     uint64_t targets;   // qubits that must be tile qubits (non-diagonal targets)
 };
-
-bool is_sign_cz(const HostGate& g) {
-    // diag(1, 1, 1, -1) exactly: a CZ, applied as a conditional negation.
-    const double* m = g.m;
-    return g.kind == QS_GATE_DIAG2 && m[0] == 1.0 && m[1] == 0.0 && m[2] == 1.0 && m[3] == 0.0 &&
-           m[4] == 1.0 && m[5] == 0.0 && m[6] == -1.0 && m[7] == 0.0;
-}
 
 void mat2_mul(const double* b, const double* a, double* out) {  // out = b @ a (2x2)
     double r[8];
@@ -80,6 +74,159 @@ int local_pos(const int* tq, int K, int q) {
     return -1;
 }
 
+// ------------------------------------------------------------------ phase groups
+typedef std::complex<double> cd;
+
+struct PTerm {
+    int qa, qb;            // global qubits; qb < 0: one-body; qa < 0: constant
+    cd f;
+};
+
+bool unit_modulus(const double* m, int n) {
+    for (int k = 0; k < n; ++k) {
+        const double a = std::hypot(m[2 * k], m[2 * k + 1]);
+        if (std::fabs(a - 1.0) > 1e-13) return false;
+    }
+    return true;
+}
+
+// Decomposes a unitary diagonal gate into phase factors; false if it has to be
+// applied element by element (non-unit entries or more than one control).
+bool phase_terms(const HostGate& g, std::vector<PTerm>& out) {
+    auto c_of = [](const double* m, int k) { return cd(m[2 * k], m[2 * k + 1]); };
+    int a, b;
+    cd d00, d01, d10, d11;
+    if (g.kind == QS_GATE_DIAG1) {
+        if (!unit_modulus(g.m, 2)) return false;
+        const int nc = __builtin_popcountll(g.cmask);
+        if (nc == 0) {
+            const cd d0 = c_of(g.m, 0), d1 = c_of(g.m, 1);
+            out.push_back({-1, -1, d0});
+            out.push_back({g.q0, -1, d1 * std::conj(d0)});
+            return true;
+        }
+        if (nc != 1) return false;
+        a = __builtin_ctzll(g.cmask);     // control is the MSB of a 2-qubit diagonal
+        b = g.q0;
+        d00 = 1.0; d01 = 1.0; d10 = c_of(g.m, 0); d11 = c_of(g.m, 1);
+    } else {
+        if (g.cmask != 0 || !unit_modulus(g.m, 4)) return false;
+        a = g.q0;
+        b = g.q1;
+        d00 = c_of(g.m, 0); d01 = c_of(g.m, 1); d10 = c_of(g.m, 2); d11 = c_of(g.m, 3);
+    }
+    out.push_back({-1, -1, d00});
+    out.push_back({a, -1, d10 * std::conj(d00)});
+    out.push_back({b, -1, d01 * std::conj(d00)});
+    out.push_back({a, b, d11 * d00 * std::conj(d10) * std::conj(d01)});
+    return true;
+}
+
+bool is_one(const cd& z) { return z.real() == 1.0 && z.imag() == 0.0; }
+bool is_sign(const cd& z) { return (z.real() == 1.0 || z.real() == -1.0) && z.imag() == 0.0; }
+
+int pair_index(int r, int s) {          // r < s < 4
+    static const int idx[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
+    return idx[r][s];
+}
+
+// Builds one DevPhase record (plus its terms) for diagonal gates `diag` executed under
+// register layout `regs` of pass P.
+void emit_phase_group(const std::vector<const HostGate*>& diag, const int* regs, DevPass& P,
+                      SvPlan* plan) {
+    const int K = P.K, R = P.R;
+    int ridx[SV_TILE_BITS], tidx[SV_TILE_BITS];
+    int nt = 0;
+    for (int p = 0; p < K; ++p) {
+        ridx[p] = -1;
+        tidx[p] = -1;
+    }
+    for (int i = 0; i < R; ++i) ridx[regs[i]] = i;
+    for (int p = 0; p < K; ++p)
+        if (ridx[p] < 0) tidx[p] = nt++;
+    auto local = [&](int q) {
+        for (int i = 0; i < K; ++i)
+            if (P.tq[i] == q) return i;
+        return -1;
+    };
+    cd c = 1.0, kr[4] = {1.0, 1.0, 1.0, 1.0}, crr[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+    std::vector<DevTerm> bt, tt;       // block-class and thread-class terms
+    auto add = [&](std::vector<DevTerm>& v, int cls, int a, int b, const cd& f) {
+        DevTerm t{};
+        t.cls = cls;
+        t.a = a;
+        t.b = b;
+        t.re = f.real();
+        t.im = f.imag();
+        v.push_back(t);
+    };
+    std::vector<PTerm> pts;
+    for (const HostGate* g : diag) phase_terms(*g, pts);
+    for (const PTerm& t : pts) {
+        if (is_one(t.f)) continue;
+        if (t.qa < 0) {
+            c *= t.f;
+            continue;
+        }
+        const int la = local(t.qa);
+        if (t.qb < 0) {
+            if (la < 0) add(bt, PT_BB, t.qa, -1, t.f);
+            else if (ridx[la] >= 0) kr[ridx[la]] *= t.f;
+            else add(tt, PT_T1, tidx[la], -1, t.f);
+            continue;
+        }
+        int lb = local(t.qb);
+        int qa = t.qa, qb = t.qb;
+        int la2 = la;
+        // order the pair so that register > thread > block in the first slot
+        auto rank = [&](int l) { return l < 0 ? 0 : (ridx[l] >= 0 ? 2 : 1); };
+        if (rank(lb) > rank(la2)) {
+            std::swap(la2, lb);
+            std::swap(qa, qb);
+        }
+        const int ra = rank(la2), rb = rank(lb);
+        if (ra == 0) add(bt, PT_BB, qa, qb, t.f);                         // block x block
+        else if (ra == 1 && rb == 0) add(bt, PT_BT, tidx[la2], qb, t.f);  // thread x block
+        else if (ra == 1) add(tt, PT_TT, tidx[la2], tidx[lb], t.f);       // thread x thread
+        else if (rb == 0) add(bt, PT_BR, ridx[la2], qb, t.f);             // register x block
+        else if (rb == 1) add(tt, PT_TR, ridx[la2], tidx[lb], t.f);       // register x thread
+        else {                                                             // register x register
+            const int r = std::min(ridx[la2], ridx[lb]), s2 = std::max(ridx[la2], ridx[lb]);
+            crr[pair_index(r, s2)] *= t.f;
+        }
+    }
+    DevPhase ph{};
+    bool sign = is_sign(c);
+    for (int i = 0; i < 4; ++i) sign = sign && is_sign(kr[i]);
+    for (int i = 0; i < 6; ++i) sign = sign && is_sign(crr[i]);
+    for (const DevTerm& t : bt) sign = sign && is_sign(cd(t.re, t.im));
+    for (const DevTerm& t : tt) sign = sign && is_sign(cd(t.re, t.im));
+    ph.sign = sign ? 1 : 0;
+    ph.c[0] = c.real();
+    ph.c[1] = c.imag();
+    for (int i = 0; i < 4; ++i) {
+        ph.kr[i][0] = kr[i].real();
+        ph.kr[i][1] = kr[i].imag();
+    }
+    for (int i = 0; i < 6; ++i) {
+        ph.crr[i][0] = crr[i].real();
+        ph.crr[i][1] = crr[i].imag();
+    }
+    ph.t0 = (int)plan->terms.size() - P.term0;
+    ph.nt = (int)tt.size();
+    plan->terms.insert(plan->terms.end(), tt.begin(), tt.end());
+    ph.b0 = (int)plan->terms.size() - P.term0;
+    ph.nb = (int)bt.size();
+    plan->terms.insert(plan->terms.end(), bt.begin(), bt.end());
+    ph.slot = bt.empty() ? -1 : P.nslot++;
+    DevGate d{};
+    d.kind = DG_PHASE;
+    d.qa = (int)plan->phases.size() - P.phase0;
+    d.qb = -1;
+    plan->phases.push_back(ph);
+    plan->gates.push_back(d);
+}
+
 void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan) {
     const int K = P.K, R = P.R;
     int load[4] = {0, 0, 0, 0};
@@ -87,16 +234,25 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
     int cur[4] = {load[0], load[1], load[2], load[3]};
     P.stage0 = (int)plan->stages.size();
     P.gate0 = (int)plan->gates.size();
+    P.phase0 = (int)plan->phases.size();
+    P.term0 = (int)plan->terms.size();
+    P.nslot = 0;
     int nstage = 0;
     bool open = false;
     DevStage st{};
+    std::vector<const HostGate*> diag;   // pending unitary diagonal gates of the stage
 
     auto regmask_of = [&](const int* r) {
         int m = 0;
         for (int i = 0; i < R; ++i) m |= 1 << r[i];
         return m;
     };
+    auto flush_diag = [&]() {
+        if (!diag.empty()) emit_phase_group(diag, cur, P, plan);
+        diag.clear();
+    };
     auto open_stage = [&](const int* regs) {
+        flush_diag();
         if (open) {
             st.g1 = (int)plan->gates.size() - P.gate0;
             plan->stages.push_back(st);
@@ -118,28 +274,19 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
         const HostGate& g = it.g;
         if (it.diag) {
             if (!open) open_stage(cur);
-            // Merge with a directly preceding one-qubit diagonal on the same qubit.
-            if (g.kind == QS_GATE_DIAG1 && (int)plan->gates.size() > P.gate0 + st.g0) {
-                DevGate& prev = plan->gates.back();
-                if (prev.kind == DG_DIAG1 && prev.qa == g.q0 && prev.cmask == g.cmask) {
-                    double a[8] = {prev.m[0], prev.m[1], 0, 0, 0, 0, prev.m[2], prev.m[3]};
-                    double b[8] = {g.m[0], g.m[1], 0, 0, 0, 0, g.m[2], g.m[3]};
-                    double r[8];
-                    mat2_mul(b, a, r);
-                    prev.m[0] = r[0]; prev.m[1] = r[1]; prev.m[2] = r[6]; prev.m[3] = r[7];
-                    continue;
-                }
+            std::vector<PTerm> probe;
+            if (phase_terms(g, probe)) {
+                diag.push_back(&it.g);
+                continue;
             }
+            // General diagonal (non-unit entries or several controls): per amplitude.
+            flush_diag();
             DevGate d{};
             d.cmask = g.cmask;
             d.qa = g.q0;
             d.qb = (g.kind == QS_GATE_DIAG2) ? g.q1 : -1;
-            if (is_sign_cz(g)) {
-                d.kind = DG_SIGN2;
-            } else {
-                d.kind = (g.kind == QS_GATE_DIAG1) ? DG_DIAG1 : DG_DIAG2;
-                std::memcpy(d.m, g.m, sizeof(double) * (g.kind == QS_GATE_DIAG1 ? 4 : 8));
-            }
+            d.kind = (g.kind == QS_GATE_DIAG1) ? DG_DIAG1 : DG_DIAG2;
+            std::memcpy(d.m, g.m, sizeof(double) * (g.kind == QS_GATE_DIAG1 ? 4 : 8));
             plan->gates.push_back(d);
             continue;
         }
@@ -164,12 +311,15 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
                 }
                 for (int i = 0; i < R && popcount64(nm) < R; ++i) nm |= 1 << cur[i];
                 for (int p = K - 1; p >= 0 && popcount64(nm) < R; --p) nm |= 1 << p;
+                // the pending diagonal group runs under the old layout
+                flush_diag();
                 int k = 0;
                 for (int p = 0; p < K; ++p)
                     if ((nm >> p) & 1) cur[k++] = p;
             }
             open_stage(cur);
         }
+        flush_diag();
         DevGate d{};
         d.cmask = g.cmask;
         d.qa = d.qb = -1;
@@ -193,12 +343,16 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
         }
         plan->gates.push_back(d);
     }
+    flush_diag();
     if (open) {
         st.g1 = (int)plan->gates.size() - P.gate0;
         plan->stages.push_back(st);
         ++nstage;
     }
     P.nstage = nstage;
+    P.nphase = (int)plan->phases.size() - P.phase0;
+    P.slot0 = plan->nslot_total;
+    plan->nslot_total += P.nslot;
 }
 
 }  // namespace
@@ -207,6 +361,9 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
     plan->passes.clear();
     plan->stages.clear();
     plan->gates.clear();
+    plan->phases.clear();
+    plan->terms.clear();
+    plan->nslot_total = 0;
     std::vector<Item> items;
     items.reserve(ng);
     for (int i = 0; i < ng; ++i) {
@@ -282,6 +439,32 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
         }
     }
     return QS_OK;
+}
+
+}  // namespace qs
+
+namespace qs {
+
+SvBlobLayout sv_blob_layout(const SvPlan& plan, int nq) {
+    SvBlobLayout L{};
+    size_t off = 0;
+    L.passes = off;
+    off += sv_blob_offset(plan.passes.size() * sizeof(DevPass));
+    L.stages = off;
+    off += sv_blob_offset(plan.stages.size() * sizeof(DevStage));
+    L.gates = off;
+    off += sv_blob_offset(std::max<size_t>(plan.gates.size(), 1) * sizeof(DevGate));
+    L.phases = off;
+    off += sv_blob_offset(std::max<size_t>(plan.phases.size(), 1) * sizeof(DevPhase));
+    L.terms = off;
+    off += sv_blob_offset(std::max<size_t>(plan.terms.size(), 1) * sizeof(DevTerm));
+    L.factors = off;
+    size_t nblocks = 1;
+    if (!plan.passes.empty() && nq > plan.passes[0].K) nblocks = (size_t)1 << (nq - plan.passes[0].K);
+    off += sv_blob_offset((size_t)std::max(plan.nslot_total, 1) * nblocks * PHASE_BLOCK_VALUES *
+                          2 * sizeof(double));
+    L.total = off;
+    return L;
 }
 
 }  // namespace qs

@@ -8,7 +8,9 @@ target outside the tile) shows up as a mismatch against the oracle.
 
 import numpy as np
 
-DG_U1, DG_U2, DG_DIAG1, DG_DIAG2, DG_SIGN2 = 0, 1, 2, 3, 4
+DG_U1, DG_U2, DG_DIAG1, DG_DIAG2, DG_PHASE = 0, 1, 2, 3, 5
+PT_BB, PT_BT, PT_BR, PT_T1, PT_TT, PT_TR = 0, 1, 2, 3, 4, 5
+PAIRS = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
 
 
 def _cm(vals, n):
@@ -54,9 +56,8 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                     out = m @ v
                     for k in range(4):
                         st[ii[k]] = out[k]
-                elif g['kind'] == DG_SIGN2:
-                    hit = sel & (((idx >> g['qa']) & (idx >> g['qb']) & 1) == 1)
-                    st[hit] *= -1
+                elif g['kind'] == DG_PHASE:
+                    st *= phase_factor(g['phase'], tq, regs, idx)
                 elif g['kind'] == DG_DIAG1:
                     d = _cm(g['m'], 2)
                     b = (idx >> g['qa']) & 1
@@ -71,3 +72,46 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                 if g['kind'] in (DG_U1, DG_U2):
                     assert (tile >> tq[regs[g['r0']]]) & 1
     return st
+
+
+def phase_factor(ph, tq, regs, idx):
+    """Per-amplitude factor of a phase group, resolving each term class to qubits."""
+    K = len(tq)
+    thread_pos = [p for p in range(K) if p not in regs]      # thread bit i -> local position
+    bit = lambda q: (idx >> q) & 1
+    tq_thread = lambda a: tq[thread_pos[a]]
+    tq_reg = lambda a: tq[regs[a]]
+    f = np.full(idx.size, complex(*ph['c']))
+    vals = []
+    for r in range(len(regs)):
+        vals.append((bit(tq_reg(r)), complex(ph['kr'][2 * r], ph['kr'][2 * r + 1])))
+    for k, (r, s2) in enumerate(PAIRS):
+        if s2 < len(regs):
+            vals.append((bit(tq_reg(r)) & bit(tq_reg(s2)),
+                         complex(ph['crr'][2 * k], ph['crr'][2 * k + 1])))
+    for cls, a, b, re, im in ph['terms']:
+        if cls == PT_BB:
+            on = bit(a) & (bit(b) if b >= 0 else 1)
+        elif cls == PT_BT:
+            on = bit(tq_thread(a)) & bit(b)
+        elif cls == PT_BR:
+            on = bit(tq_reg(a)) & bit(b)
+        elif cls == PT_T1:
+            on = bit(tq_thread(a))
+        elif cls == PT_TT:
+            on = bit(tq_thread(a)) & bit(tq_thread(b))
+        elif cls == PT_TR:
+            on = bit(tq_reg(a)) & bit(tq_thread(b))
+        else:
+            raise AssertionError(cls)
+        # block-class terms must name non-tile qubits, thread/register ones tile qubits
+        if cls in (PT_BB,):
+            assert a not in tq and (b < 0 or b not in tq)
+        if cls in (PT_BT, PT_BR):
+            assert b not in tq
+        vals.append((on, complex(re, im)))
+    for on, z in vals:
+        if ph['sign']:
+            assert z in (1, -1), z
+        f = np.where(on == 1, f * z, f)
+    return f
