@@ -30,16 +30,16 @@ constexpr int kMaxChi = 64;              // register SVD holds 2*chi <= 128 rows
 constexpr int kTeamLanes = 16;
 constexpr int kMaxSweeps = 60;
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+__device__ __forceinline__ cd2 cmul(cd2 a, cd2 b) {
+    return mk2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
-__device__ __forceinline__ double2 cmadd(double2 acc, double2 a, double2 b) {
+__device__ __forceinline__ cd2 cmadd(cd2 acc, cd2 a, cd2 b) {
     acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
     acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
     return acc;
 }
 // acc += conj(a) * b
-__device__ __forceinline__ double2 cjmadd(double2 acc, double2 a, double2 b) {
+__device__ __forceinline__ cd2 cjmadd(cd2 acc, cd2 a, cd2 b) {
     acc.x = fma(a.x, b.x, fma(a.y, b.y, acc.x));
     acc.y = fma(a.x, b.y, fma(-a.y, b.x, acc.y));
     return acc;
@@ -65,13 +65,13 @@ __device__ __forceinline__ bool load_dims(const qs_mps_task& T, Dims& d) {
            d.cr <= T.chi_max && T.chi_max <= kMaxChi;
 }
 
-__device__ __forceinline__ double2* site_ptr(const qs_mps_task& T, int k) {
-    return reinterpret_cast<double2*>(T.sites) + (size_t)k * (size_t)T.site_stride;
+__device__ __forceinline__ cd2* site_ptr(const qs_mps_task& T, int k) {
+    return reinterpret_cast<cd2*>(T.sites) + (size_t)k * (size_t)T.site_stride;
 }
 
 // ------------------------------------------------------------------ FP64 contraction
 __global__ void __launch_bounds__(256)
-mps_contract_fp64_kernel(const DevTask* __restrict__ tasks, double2* __restrict__ ws,
+mps_contract_fp64_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws,
                          size_t theta_slot, int32_t* __restrict__ status) {
     const qs_mps_task& T = tasks[blockIdx.x].t;
     Dims d;
@@ -79,22 +79,22 @@ mps_contract_fp64_kernel(const DevTask* __restrict__ tasks, double2* __restrict_
         if (threadIdx.x == 0 && status) status[tasks[blockIdx.x].orig] = QS_E_CHI;
         return;
     }
-    const double2* A = site_ptr(T, T.site);
-    const double2* B = site_ptr(T, T.site + 1);
-    double2* th = ws + (size_t)blockIdx.x * theta_slot;
-    double2 g[16];
+    const cd2* A = site_ptr(T, T.site);
+    const cd2* B = site_ptr(T, T.site + 1);
+    cd2* th = ws + (size_t)blockIdx.x * theta_slot;
+    cd2 g[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) g[k] = make_double2(T.gate[2 * k], T.gate[2 * k + 1]);
+    for (int k = 0; k < 16; ++k) g[k] = mk2(T.gate[2 * k], T.gate[2 * k + 1]);
     const int cl = d.cl, cm = d.cm, cr = d.cr;
     const int ldt = 2 * cr;
     for (int pr = threadIdx.x; pr < cl * cr; pr += blockDim.x) {
         const int a = pr / cr, e = pr - a * cr;
-        double2 t00 = make_double2(0, 0), t01 = t00, t10 = t00, t11 = t00;
-        const double2* a0 = A + (size_t)(2 * a) * cm;
-        const double2* a1 = a0 + cm;
+        cd2 t00 = mk2(0, 0), t01 = t00, t10 = t00, t11 = t00;
+        const cd2* a0 = A + (size_t)(2 * a) * cm;
+        const cd2* a1 = a0 + cm;
         for (int b = 0; b < cm; ++b) {
-            const double2 x0 = a0[b], x1 = a1[b];
-            const double2 y0 = B[(size_t)(2 * b) * cr + e], y1 = B[(size_t)(2 * b + 1) * cr + e];
+            const cd2 x0 = a0[b], x1 = a1[b];
+            const cd2 y0 = B[(size_t)(2 * b) * cr + e], y1 = B[(size_t)(2 * b + 1) * cr + e];
             t00 = cmadd(t00, x0, y0);
             t01 = cmadd(t01, x0, y1);
             t10 = cmadd(t10, x1, y0);
@@ -102,7 +102,7 @@ mps_contract_fp64_kernel(const DevTask* __restrict__ tasks, double2* __restrict_
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            double2 v = cmul(g[4 * s + 0], t00);
+            cd2 v = cmul(g[4 * s + 0], t00);
             v = cmadd(v, g[4 * s + 1], t01);
             v = cmadd(v, g[4 * s + 2], t10);
             v = cmadd(v, g[4 * s + 3], t11);
@@ -125,7 +125,7 @@ constexpr int kMaxM = 2 * kMaxChi;                // 128
 constexpr int kMaxK = kMaxChi;                    // 64
 
 __global__ void __launch_bounds__(kMfmaThreads)
-mps_contract_f32_mfma_kernel(const DevTask* __restrict__ tasks, double2* __restrict__ ws,
+mps_contract_f32_mfma_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws,
                              size_t theta_slot, int32_t* __restrict__ status) {
     // Operand planes: A as [M][K] (re, im), B as [K][N] (re, im); K padded to 64,
     // M and N padded to 128.  2 * 128*64*4 B * 2 = 128 KiB; T reuses the same space.
@@ -143,20 +143,20 @@ mps_contract_f32_mfma_kernel(const DevTask* __restrict__ tasks, double2* __restr
     }
     const int M = 2 * d.cl, K = d.cm, N = 2 * d.cr;
     const int Kp = (K + 3) & ~3;
-    const double2* A = site_ptr(T, T.site);
-    const double2* B = site_ptr(T, T.site + 1);
+    const cd2* A = site_ptr(T, T.site);
+    const cd2* B = site_ptr(T, T.site + 1);
     const int tid = threadIdx.x;
 
     for (int idx = tid; idx < kMaxM * Kp; idx += kMfmaThreads) {
         const int r = idx / Kp, k = idx - r * Kp;
-        double2 v = make_double2(0, 0);
+        cd2 v = mk2(0, 0);
         if (r < M && k < K) v = A[(size_t)r * K + k];
         Are[r * kMaxK + k] = (float)v.x;
         Aim[r * kMaxK + k] = (float)v.y;
     }
     for (int idx = tid; idx < Kp * kMaxM; idx += kMfmaThreads) {
         const int k = idx / kMaxM, c = idx - k * kMaxM;
-        double2 v = make_double2(0, 0);
+        cd2 v = mk2(0, 0);
         if (k < K && c < N) v = B[(size_t)k * N + c];
         Bre[k * kMaxM + c] = (float)v.x;
         Bim[k * kMaxM + c] = (float)v.y;
@@ -221,21 +221,21 @@ mps_contract_f32_mfma_kernel(const DevTask* __restrict__ tasks, double2* __restr
                 }
     }
     __syncthreads();
-    double2 g[16];
+    cd2 g[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) g[k] = make_double2(T.gate[2 * k], T.gate[2 * k + 1]);
-    double2* th = ws + (size_t)blockIdx.x * theta_slot;
+    for (int k = 0; k < 16; ++k) g[k] = mk2(T.gate[2 * k], T.gate[2 * k + 1]);
+    cd2* th = ws + (size_t)blockIdx.x * theta_slot;
     const int cl = d.cl, cr = d.cr, ldt = 2 * cr;
     for (int pr = tid; pr < cl * cr; pr += kMfmaThreads) {
         const int a = pr / cr, e = pr - a * cr;
         const int r0 = 2 * a, r1 = 2 * a + 1, c0 = e, c1 = cr + e;
-        const double2 t00 = make_double2(Tre[r0 * kMaxM + c0], Tim[r0 * kMaxM + c0]);
-        const double2 t01 = make_double2(Tre[r0 * kMaxM + c1], Tim[r0 * kMaxM + c1]);
-        const double2 t10 = make_double2(Tre[r1 * kMaxM + c0], Tim[r1 * kMaxM + c0]);
-        const double2 t11 = make_double2(Tre[r1 * kMaxM + c1], Tim[r1 * kMaxM + c1]);
+        const cd2 t00 = mk2(Tre[r0 * kMaxM + c0], Tim[r0 * kMaxM + c0]);
+        const cd2 t01 = mk2(Tre[r0 * kMaxM + c1], Tim[r0 * kMaxM + c1]);
+        const cd2 t10 = mk2(Tre[r1 * kMaxM + c0], Tim[r1 * kMaxM + c0]);
+        const cd2 t11 = mk2(Tre[r1 * kMaxM + c1], Tim[r1 * kMaxM + c1]);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            double2 v = cmul(g[4 * s + 0], t00);
+            cd2 v = cmul(g[4 * s + 0], t00);
             v = cmadd(v, g[4 * s + 1], t01);
             v = cmadd(v, g[4 * s + 2], t10);
             v = cmadd(v, g[4 * s + 3], t11);
@@ -258,7 +258,7 @@ __device__ __forceinline__ double team_sum(double v) {
 template <int CHI>
 struct SvdShared {
     static constexpr int LD = 2 * CHI;
-    double2 buf[(CHI + 1) * LD];
+    cd2 buf[(CHI + 1) * LD];
     double sval[2 * CHI];
     int order[2 * CHI];
     int rank[2 * CHI];
@@ -271,7 +271,7 @@ struct SvdShared {
 
 template <int CHI>
 __global__ void __launch_bounds__(16 * CHI)
-mps_svd_kernel(const DevTask* __restrict__ tasks, const double2* __restrict__ ws,
+mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
                size_t theta_slot, int32_t* __restrict__ status) {
     constexpr int NT = 16 * CHI;              // threads
     constexpr int RPL = CHI / 8;              // rows per lane (16 lanes x RPL = 2 CHI rows)
@@ -289,16 +289,16 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const double2* __restrict__ ws
     const int tid = threadIdx.x;
     const int team = tid / kTeamLanes, l16 = tid % kTeamLanes;
     const bool act = team < P;
-    const double2* th = ws + (size_t)blockIdx.x * theta_slot;
-    double2* slot0 = sh.buf;                     // P columns of 2CHI rows
-    double2* spare = sh.buf + CHI * LD;
+    const cd2* th = ws + (size_t)blockIdx.x * theta_slot;
+    cd2* slot0 = sh.buf;                     // P columns of 2CHI rows
+    cd2* spare = sh.buf + CHI * LD;
 
-    double2 ca[RPL], cb[RPL];
+    cd2 ca[RPL], cb[RPL];
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
         const int r = l16 + kTeamLanes * k;
-        ca[k] = (act && r < m) ? th[(size_t)r * n + 2 * team] : make_double2(0, 0);
-        cb[k] = (act && r < m) ? th[(size_t)r * n + 2 * team + 1] : make_double2(0, 0);
+        ca[k] = (act && r < m) ? th[(size_t)r * n + 2 * team] : mk2(0, 0);
+        cb[k] = (act && r < m) ? th[(size_t)r * n + 2 * team + 1] : mk2(0, 0);
     }
     // Frobenius norm^2 of theta: columns below 1e-14 * ||theta||_F are numerical zeros
     // and are not rotated (a wide matrix has n - m of them).
@@ -352,14 +352,14 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const double2* __restrict__ ws
                     const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
                     const double er = gr / g, ei = gi / g;          // e^{i phi}
                     // a' = c a - s e^{-i phi} b ;  b' = s e^{i phi} a + c b
-                    const double2 sem = make_double2(s * er, -s * ei);
-                    const double2 sep = make_double2(s * er, s * ei);
+                    const cd2 sem = mk2(s * er, -s * ei);
+                    const cd2 sep = mk2(s * er, s * ei);
 #pragma unroll
                     for (int k = 0; k < RPL; ++k) {
-                        const double2 x = ca[k], y = cb[k];
-                        const double2 sy = cmul(sem, y), sx = cmul(sep, x);
-                        ca[k] = make_double2(c * x.x - sy.x, c * x.y - sy.y);
-                        cb[k] = make_double2(c * y.x + sx.x, c * y.y + sx.y);
+                        const cd2 x = ca[k], y = cb[k];
+                        const cd2 sy = cmul(sem, y), sx = cmul(sep, x);
+                        ca[k] = mk2(c * x.x - sy.x, c * x.y - sy.y);
+                        cb[k] = mk2(c * y.x + sx.x, c * y.y + sx.y);
                     }
                     if (l16 == 0) sh.rotated = 1;
                 }
@@ -475,7 +475,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const double2* __restrict__ ws
         sh.scale = scale;
         *T.dlog = dlog;
         T.bond[T.site + 1] = keep;
-        if (status) status[DT.orig] = (sweeps >= kMaxSweeps) ? QS_E_NOCONV : QS_OK;
+        if (status) status[DT.orig] = (sweeps >= kMaxSweeps) ? QS_E_NOCONV : sweeps + 1;
     }
     __syncthreads();
     const int keep = sh.keep;
@@ -487,29 +487,29 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const double2* __restrict__ ws
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
             const int r = l16 + kTeamLanes * k;
-            if (ra < keep) sh.buf[ra * LD + r] = make_double2(ca[k].x * ia, ca[k].y * ia);
-            if (rb < keep) sh.buf[rb * LD + r] = make_double2(cb[k].x * ib, cb[k].y * ib);
+            if (ra < keep) sh.buf[ra * LD + r] = mk2(ca[k].x * ia, ca[k].y * ia);
+            if (rb < keep) sh.buf[rb * LD + r] = mk2(cb[k].x * ib, cb[k].y * ib);
         }
     }
     __syncthreads();
     // A' = U sqrt(S_t), (chi_l, 2, keep) = m x keep, row-major.
-    double2* Ao = site_ptr(T, T.site);
+    cd2* Ao = site_ptr(T, T.site);
     for (int idx = tid; idx < m * keep; idx += NT) {
         const int r = idx / keep, k = idx - r * keep;
         const double f = sqrt(sh.sval[sh.order[k]] * scale);
-        const double2 u = sh.buf[k * LD + r];
-        Ao[idx] = make_double2(u.x * f, u.y * f);
+        const cd2 u = sh.buf[k * LD + r];
+        Ao[idx] = mk2(u.x * f, u.y * f);
     }
     // B' = sqrt(S_t) Vh = sqrt(S_t)/S * U^H theta, (keep, 2, chi_r) = keep x n.
-    double2* Bo = site_ptr(T, T.site + 1);
+    cd2* Bo = site_ptr(T, T.site + 1);
     {
         const int c = tid % LD, kg = tid / LD;      // 8 groups of rows of B'
         if (c < n) {
-            double2 acc[CHI / 8];
+            cd2 acc[CHI / 8];
 #pragma unroll
-            for (int q = 0; q < CHI / 8; ++q) acc[q] = make_double2(0, 0);
+            for (int q = 0; q < CHI / 8; ++q) acc[q] = mk2(0, 0);
             for (int r = 0; r < m; ++r) {
-                const double2 t = th[(size_t)r * n + c];
+                const cd2 t = th[(size_t)r * n + c];
 #pragma unroll
                 for (int q = 0; q < CHI / 8; ++q) {
                     const int k = kg + 8 * q;
@@ -522,7 +522,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const double2* __restrict__ ws
                 if (k < keep) {
                     const double s = sh.sval[sh.order[k]];
                     const double f = s > 0 ? sqrt(s * scale) / s : 0.0;
-                    Bo[(size_t)k * n + c] = make_double2(acc[q].x * f, acc[q].y * f);
+                    Bo[(size_t)k * n + c] = mk2(acc[q].x * f, acc[q].y * f);
                 }
             }
         }
@@ -530,17 +530,17 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const double2* __restrict__ ws
 }
 
 // ------------------------------------------------------------------ one-site gate
-__global__ void mps_apply_1q_kernel(double2* __restrict__ sites, const int32_t* __restrict__ bond,
-                                    int64_t stride, int site, double2 m00, double2 m01,
-                                    double2 m10, double2 m11) {
+__global__ void mps_apply_1q_kernel(cd2* __restrict__ sites, const int32_t* __restrict__ bond,
+                                    int64_t stride, int site, cd2 m00, cd2 m01,
+                                    cd2 m10, cd2 m11) {
     const int cl = bond[site], cr = bond[site + 1];
-    double2* A = sites + (size_t)site * stride;
+    cd2* A = sites + (size_t)site * stride;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < cl * cr;
          idx += gridDim.x * blockDim.x) {
         const int a = idx / cr, b = idx - a * cr;
-        double2* p0 = A + (size_t)(2 * a) * cr + b;
-        double2* p1 = p0 + cr;
-        const double2 x0 = *p0, x1 = *p1;
+        cd2* p0 = A + (size_t)(2 * a) * cr + b;
+        cd2* p1 = p0 + cr;
+        const cd2 x0 = *p0, x1 = *p1;
         *p0 = cmadd(cmul(m00, x0), m01, x1);
         *p1 = cmadd(cmul(m10, x0), m11, x1);
     }
@@ -559,13 +559,13 @@ size_t qs_mps_workspace_bytes(int ntasks, int chi_max) {
     if (ntasks <= 0 || chi_max <= 0) return 0;
     const size_t head = sv_blob_offset((size_t)ntasks * sizeof(DevTask));
     const size_t slot = (size_t)4 * chi_max * chi_max;
-    return head + (size_t)ntasks * slot * sizeof(double2);
+    return head + (size_t)ntasks * slot * sizeof(cd2);
 }
 
 namespace {
 int chi_class(int chi) { return chi <= 8 ? 8 : chi <= 16 ? 16 : chi <= 32 ? 32 : 64; }
 
-void launch_svd(int cls, const DevTask* t, int n, const double2* theta, size_t slot,
+void launch_svd(int cls, const DevTask* t, int n, const cd2* theta, size_t slot,
                 int32_t* status, hipStream_t hs) {
     switch (cls) {
         case 8: hipLaunchKernelGGL(mps_svd_kernel<8>, dim3(n), dim3(16 * 8), 0, hs, t, theta, slot,
@@ -629,7 +629,7 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
     hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
     char* ws = static_cast<char*>(workspace);
     DevTask* dtasks = reinterpret_cast<DevTask*>(ws);
-    double2* theta = reinterpret_cast<double2*>(ws + sv_blob_offset((size_t)ntasks * sizeof(DevTask)));
+    cd2* theta = reinterpret_cast<cd2*>(ws + sv_blob_offset((size_t)ntasks * sizeof(DevTask)));
     const size_t slot = (size_t)4 * chi * chi;
     QS_HIP(hipMemcpyAsync(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask),
                           hipMemcpyHostToDevice, hs));
@@ -657,9 +657,8 @@ int qs_mps_apply_1q(double* sites, const int32_t* bond, int64_t site_stride, int
     QS_REQUIRE(site >= 0 && site_stride > 0, QS_E_ARG, "bad site/stride");
     hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(mps_apply_1q_kernel, dim3(16), dim3(256), 0, hs,
-                       reinterpret_cast<double2*>(sites), bond, site_stride, site,
-                       make_double2(m[0], m[1]), make_double2(m[2], m[3]),
-                       make_double2(m[4], m[5]), make_double2(m[6], m[7]));
+                       reinterpret_cast<cd2*>(sites), bond, site_stride, site,
+                       mk2(m[0], m[1]), mk2(m[2], m[3]), mk2(m[4], m[5]), mk2(m[6], m[7]));
     QS_HIP(hipGetLastError());
     return QS_OK;
 }

@@ -13,6 +13,17 @@
 
 namespace qs {
 
+// Complex double as a native 2-vector.  HIP's double2 is a struct whose copies lower to
+// memcpy, which keeps register arrays of it in scratch memory; this type stays in VGPRs.
+typedef double cd2 __attribute__((ext_vector_type(2)));
+
+__host__ __device__ __forceinline__ cd2 mk2(double re, double im) {
+    cd2 r;
+    r.x = re;
+    r.y = im;
+    return r;
+}
+
 // Thread-local error message, returned by qs_last_error().
 void set_error(const char* fmt, ...);
 void clear_error();

@@ -17,11 +17,11 @@ namespace qs {
 
 namespace {
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+__device__ __forceinline__ cd2 cmul(cd2 a, cd2 b) {
+    return mk2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
 
-__device__ __forceinline__ double2 cmadd(double2 acc, double2 a, double2 b) {
+__device__ __forceinline__ cd2 cmadd(cd2 acc, cd2 a, cd2 b) {
     acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
     acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
     return acc;
@@ -86,36 +86,47 @@ __device__ __forceinline__ void make_layout(Layout<R>& L, const Regs regs, const
 }
 
 template <int R, int B>
-__device__ __forceinline__ void apply_u1_bit(double2 (&a)[1 << R], const DevGate& G,
+__device__ __forceinline__ void apply_u1_bit(cd2 (&a)[1 << R], const DevGate& G,
                                              const Layout<R>& L) {
-    const double2 m00 = make_double2(G.m[0], G.m[1]), m01 = make_double2(G.m[2], G.m[3]);
-    const double2 m10 = make_double2(G.m[4], G.m[5]), m11 = make_double2(G.m[6], G.m[7]);
+    const cd2 m00 = mk2(G.m[0], G.m[1]), m01 = mk2(G.m[2], G.m[3]);
+    const cd2 m10 = mk2(G.m[4], G.m[5]), m11 = mk2(G.m[6], G.m[7]);
     const uint64_t cm = G.cmask;
+    if (cm == 0) {
+#pragma unroll
+        for (int j = 0; j < (1 << R); ++j) {
+            if ((j >> B) & 1) continue;           // compile-time after unrolling
+            const cd2 x0 = a[j], x1 = a[j | (1 << B)];
+            a[j] = cmadd(cmul(m00, x0), m01, x1);
+            a[j | (1 << B)] = cmadd(cmul(m10, x0), m11, x1);
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
-        if (j & (1 << B)) continue;
-        const int j1 = j | (1 << B);
-        if (cm != 0 && (L.glob(j) & cm) != cm) continue;
-        const double2 x0 = a[j], x1 = a[j1];
-        a[j] = cmadd(cmul(m00, x0), m01, x1);
-        a[j1] = cmadd(cmul(m10, x0), m11, x1);
+        if ((j >> B) & 1) continue;
+        const bool ok = (L.glob(j) & cm) == cm;
+        const cd2 x0 = a[j], x1 = a[j | (1 << B)];
+        const cd2 y0 = cmadd(cmul(m00, x0), m01, x1);
+        const cd2 y1 = cmadd(cmul(m10, x0), m11, x1);
+        a[j] = ok ? y0 : x0;
+        a[j | (1 << B)] = ok ? y1 : x1;
     }
 }
 
 template <int R, int B0, int B1>
-__device__ __forceinline__ void apply_u2_bits(double2 (&a)[1 << R], const DevGate& G,
+__device__ __forceinline__ void apply_u2_bits(cd2 (&a)[1 << R], const DevGate& G,
                                               const Layout<R>& L) {
     const uint64_t cm = G.cmask;
-    const double2* m = reinterpret_cast<const double2*>(G.m);
+    const cd2* m = reinterpret_cast<const cd2*>(G.m);
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
         if (j & ((1 << B0) | (1 << B1))) continue;
         if (cm != 0 && (L.glob(j) & cm) != cm) continue;
         constexpr int b0 = 1 << B0, b1 = 1 << B1;
-        const double2 x0 = a[j], x1 = a[j | b1], x2 = a[j | b0], x3 = a[j | b0 | b1];
+        const cd2 x0 = a[j], x1 = a[j | b1], x2 = a[j | b0], x3 = a[j | b0 | b1];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const double2 y = cmadd(cmadd(cmadd(cmul(m[4 * r], x0), m[4 * r + 1], x1),
+            const cd2 y = cmadd(cmadd(cmadd(cmul(m[4 * r], x0), m[4 * r + 1], x1),
                                           m[4 * r + 2], x2), m[4 * r + 3], x3);
             if (r == 0) a[j] = y;
             else if (r == 1) a[j | b1] = y;
@@ -126,7 +137,7 @@ __device__ __forceinline__ void apply_u2_bits(double2 (&a)[1 << R], const DevGat
 }
 
 template <int R>
-__device__ __forceinline__ void apply_u1(double2 (&a)[1 << R], const DevGate& G,
+__device__ __forceinline__ void apply_u1(cd2 (&a)[1 << R], const DevGate& G,
                                          const Layout<R>& L) {
     switch (G.r0) {
         case 0: apply_u1_bit<R, 0>(a, G, L); break;
@@ -138,7 +149,7 @@ __device__ __forceinline__ void apply_u1(double2 (&a)[1 << R], const DevGate& G,
 }
 
 template <int R>
-__device__ __forceinline__ void apply_u2(double2 (&a)[1 << R], const DevGate& G,
+__device__ __forceinline__ void apply_u2(cd2 (&a)[1 << R], const DevGate& G,
                                          const Layout<R>& L) {
     constexpr int S1 = R > 1 ? 1 : 0, S2 = R > 2 ? 2 : 0, S3 = R > 3 ? 3 : 0;
     const int code = G.r0 * 4 + G.r1;
@@ -172,12 +183,12 @@ __device__ __forceinline__ void apply_u2(double2 (&a)[1 << R], const DevGate& G,
 }
 
 template <int R>
-__device__ __forceinline__ void apply_diag(double2 (&a)[1 << R], const DevGate& G,
+__device__ __forceinline__ void apply_diag(cd2 (&a)[1 << R], const DevGate& G,
                                            const Layout<R>& L) {
     const uint64_t cm = G.cmask;
     const int qa = G.qa, qb = G.qb;
     if (G.kind == DG_DIAG1) {
-        const double2 d0 = make_double2(G.m[0], G.m[1]), d1 = make_double2(G.m[2], G.m[3]);
+        const cd2 d0 = mk2(G.m[0], G.m[1]), d1 = mk2(G.m[2], G.m[3]);
 #pragma unroll
         for (int j = 0; j < (1 << R); ++j) {
             const uint64_t g = L.glob(j);
@@ -187,14 +198,14 @@ __device__ __forceinline__ void apply_diag(double2 (&a)[1 << R], const DevGate& 
         return;
     }
     // DG_DIAG2: index = 2*bit(qa) + bit(qb)
-    const double2 d00 = make_double2(G.m[0], G.m[1]), d01 = make_double2(G.m[2], G.m[3]);
-    const double2 d10 = make_double2(G.m[4], G.m[5]), d11 = make_double2(G.m[6], G.m[7]);
+    const cd2 d00 = mk2(G.m[0], G.m[1]), d01 = mk2(G.m[2], G.m[3]);
+    const cd2 d10 = mk2(G.m[4], G.m[5]), d11 = mk2(G.m[6], G.m[7]);
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
         const uint64_t g = L.glob(j);
         if (cm != 0 && (g & cm) != cm) continue;
         const bool ba = (g >> qa) & 1ull, bb = (g >> qb) & 1ull;
-        const double2 d = ba ? (bb ? d11 : d10) : (bb ? d01 : d00);
+        const cd2 d = ba ? (bb ? d11 : d10) : (bb ? d01 : d00);
         a[j] = cmul(d, a[j]);
     }
 }
@@ -204,50 +215,53 @@ __device__ __forceinline__ constexpr unsigned reg_col_mask(int r) {
     return r == 0 ? 0xAAAAu : r == 1 ? 0xCCCCu : r == 2 ? 0xF0F0u : 0xFF00u;
 }
 
-__device__ __forceinline__ bool is_unit(double2 z) { return z.x == 1.0 && z.y == 0.0; }
+__device__ __forceinline__ bool is_unit(cd2 z) { return z.x == 1.0 && z.y == 0.0; }
 
 // Applies a unitary diagonal group (DevPhase).  Block-class factors come from the
 // prologue's table, thread-class factors are folded per thread, register factors are
 // applied to the halves/quarters of the register block they select.
 template <int R>
-__device__ __forceinline__ void apply_phase(double2 (&a)[1 << R], const DevPhase& Q,
+__device__ __forceinline__ void apply_phase(cd2 (&a)[1 << R], const DevPhase& Q,
                                             const DevTerm* __restrict__ terms,
-                                            const double2* __restrict__ bf, int t) {
+                                            const cd2* __restrict__ bf, int t) {
     constexpr int NR = 1 << R;
-    double2 A = make_double2(Q.c[0], Q.c[1]);
-    double2 kap[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) kap[r] = make_double2(1.0, 0.0);
+    cd2 A = mk2(Q.c[0], Q.c[1]);
+    // per-register-bit factors as scalars (an indexed array would live in scratch)
+    cd2 k0 = mk2(1.0, 0.0), k1 = k0, k2 = k0, k3 = k0;
     if (bf) {
         A = cmul(A, bf[0]);
         for (int q = 0; q < 8; ++q)
             if ((t >> q) & 1) A = cmul(A, bf[1 + q]);
-#pragma unroll
-        for (int r = 0; r < R; ++r) kap[r] = bf[9 + r];
+        k0 = bf[9];
+        if (R > 1) k1 = bf[10];
+        if (R > 2) k2 = bf[11];
+        if (R > 3) k3 = bf[12];
     }
-#pragma unroll
-    for (int r = 0; r < R; ++r) kap[r] = cmul(kap[r], make_double2(Q.kr[r][0], Q.kr[r][1]));
+    k0 = cmul(k0, mk2(Q.kr[0][0], Q.kr[0][1]));
+    if (R > 1) k1 = cmul(k1, mk2(Q.kr[1][0], Q.kr[1][1]));
+    if (R > 2) k2 = cmul(k2, mk2(Q.kr[2][0], Q.kr[2][1]));
+    if (R > 3) k3 = cmul(k3, mk2(Q.kr[3][0], Q.kr[3][1]));
     for (int k = 0; k < Q.nt; ++k) {
         const DevTerm& T = terms[Q.t0 + k];
-        const double2 f = make_double2(T.re, T.im);
+        const cd2 f = mk2(T.re, T.im);
         if (T.cls == PT_T1) {
             if ((t >> T.a) & 1) A = cmul(A, f);
         } else if (T.cls == PT_TT) {
             if (((t >> T.a) & (t >> T.b)) & 1) A = cmul(A, f);
-        } else {  // PT_TR
-            if ((t >> T.b) & 1) {
-#pragma unroll
-                for (int r = 0; r < R; ++r)
-                    if (r == T.a) kap[r] = cmul(kap[r], f);
-            }
+        } else if ((t >> T.b) & 1) {  // PT_TR
+            if (T.a == 0) k0 = cmul(k0, f);
+            else if (T.a == 1) k1 = cmul(k1, f);
+            else if (T.a == 2) k2 = cmul(k2, f);
+            else k3 = cmul(k3, f);
         }
     }
     if (Q.sign) {
-        // every factor is +-1: build the 16-bit parity mask of the register block
+        // every factor is +-1: the 16-bit parity mask of the register block
         unsigned M = (A.x < 0.0) ? 0xFFFFu : 0u;
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (kap[r].x < 0.0) M ^= reg_col_mask(r);
+        if (k0.x < 0.0) M ^= reg_col_mask(0);
+        if (R > 1 && k1.x < 0.0) M ^= reg_col_mask(1);
+        if (R > 2 && k2.x < 0.0) M ^= reg_col_mask(2);
+        if (R > 3 && k3.x < 0.0) M ^= reg_col_mask(3);
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -257,22 +271,24 @@ __device__ __forceinline__ void apply_phase(double2 (&a)[1 << R], const DevPhase
             }
 #pragma unroll
         for (int j = 0; j < NR; ++j)
-            if ((M >> j) & 1u) a[j] = make_double2(-a[j].x, -a[j].y);
+            if ((M >> j) & 1u) a[j] = -a[j];
         return;
     }
 #pragma unroll
-    for (int j = 0; j < NR; ++j) a[j] = cmul(A, a[j]);
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int j = 0; j < NR; ++j)
-            if (j & (1 << r)) a[j] = cmul(kap[r], a[j]);
+    for (int j = 0; j < NR; ++j) {
+        cd2 f = A;
+        if (j & 1) f = cmul(f, k0);
+        if (R > 1 && (j & 2)) f = cmul(f, k1);
+        if (R > 2 && (j & 4)) f = cmul(f, k2);
+        if (R > 3 && (j & 8)) f = cmul(f, k3);
+        a[j] = cmul(f, a[j]);
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int s2 = r + 1; s2 < R; ++s2) {
             const int pi = r == 0 ? s2 - 1 : (r == 1 ? s2 + 1 : 5);
-            const double2 f = make_double2(Q.crr[pi][0], Q.crr[pi][1]);
+            const cd2 f = mk2(Q.crr[pi][0], Q.crr[pi][1]);
             if (!is_unit(f)) {
 #pragma unroll
                 for (int j = 0; j < NR; ++j)
@@ -286,7 +302,7 @@ __device__ __forceinline__ void apply_phase(double2 (&a)[1 << R], const DevPhase
 __global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
                                       const DevPhase* __restrict__ phases,
                                       const DevTerm* __restrict__ terms,
-                                      double2* __restrict__ factors, uint32_t nblocks) {
+                                      cd2* __restrict__ factors, uint32_t nblocks) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int nslot = P->nslot;
     if (gid >= (uint64_t)nslot * nblocks) return;
@@ -305,13 +321,13 @@ __global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
             b >>= 1;
         }
     }
-    double2 v[PHASE_BLOCK_VALUES];
+    cd2 v[PHASE_BLOCK_VALUES];
 #pragma unroll
-    for (int k = 0; k < PHASE_BLOCK_VALUES; ++k) v[k] = make_double2(1.0, 0.0);
+    for (int k = 0; k < PHASE_BLOCK_VALUES; ++k) v[k] = mk2(1.0, 0.0);
     const DevTerm* tm = terms + P->term0 + Q->b0;
     for (int k = 0; k < Q->nb; ++k) {
         const DevTerm& T = tm[k];
-        const double2 f = make_double2(T.re, T.im);
+        const cd2 f = mk2(T.re, T.im);
         if (T.cls == PT_BB) {
             const bool on = ((base >> T.a) & 1ull) && (T.b < 0 || ((base >> T.b) & 1ull));
             if (on) v[0] = cmul(v[0], f);
@@ -322,19 +338,19 @@ __global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
                 if (i == idx) v[i] = cmul(v[i], f);
         }
     }
-    double2* out = factors + ((size_t)(P->slot0 + slot) * nblocks + blk) * PHASE_BLOCK_VALUES;
+    cd2* out = factors + ((size_t)(P->slot0 + slot) * nblocks + blk) * PHASE_BLOCK_VALUES;
 #pragma unroll
     for (int k = 0; k < PHASE_BLOCK_VALUES; ++k) out[k] = v[k];
 }
 
 template <int R>
 __global__ void __launch_bounds__(1 << (SV_TILE_BITS - SV_REG_BITS))
-sv_pass_kernel(double2* __restrict__ state, int nq, const DevPass* __restrict__ P,
+sv_pass_kernel(cd2* __restrict__ state, int nq, const DevPass* __restrict__ P,
                const DevStage* __restrict__ stages, const DevGate* __restrict__ gates,
                const DevPhase* __restrict__ phases, const DevTerm* __restrict__ terms,
-               const double2* __restrict__ factors) {
+               const cd2* __restrict__ factors) {
     constexpr int NR = 1 << R;
-    __shared__ double2 lds[kLdsSlots];
+    __shared__ cd2 lds[kLdsSlots];
 
     const int t = threadIdx.x;
     const int K = P->K;
@@ -365,7 +381,7 @@ sv_pass_kernel(double2* __restrict__ state, int nq, const DevPass* __restrict__ 
     make_layout<R>(Lcur, load_regs, P, base, t);
     Regs cur = load_regs;
 
-    double2 a[NR];
+    cd2 a[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) a[j] = state[Lcur.glob(j)];
 
@@ -395,7 +411,7 @@ sv_pass_kernel(double2* __restrict__ state, int nq, const DevPass* __restrict__ 
                 case DG_U2: apply_u2<R>(a, G, Lcur); break;
                 case DG_PHASE: {
                     const DevPhase& Q = ph[G.qa];
-                    const double2* bf =
+                    const cd2* bf =
                         Q.slot >= 0 ? factors + ((size_t)(P->slot0 + Q.slot) * nblocks + blockIdx.x) *
                                                     PHASE_BLOCK_VALUES
                                     : nullptr;
@@ -422,30 +438,30 @@ sv_pass_kernel(double2* __restrict__ state, int nq, const DevPass* __restrict__ 
     for (int j = 0; j < NR; ++j) state[Lcur.glob(j)] = a[j];
 }
 
-__global__ void sv_fill_zero_kernel(double2* __restrict__ state, uint64_t n) {
+__global__ void sv_fill_zero_kernel(cd2* __restrict__ state, uint64_t n) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        state[i] = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
+        state[i] = mk2(i == 0 ? 1.0 : 0.0, 0.0);
 }
 
-__global__ void sv_scale_kernel(double2* __restrict__ state, uint64_t n, double2 f) {
+__global__ void sv_scale_kernel(cd2* __restrict__ state, uint64_t n, cd2 f) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         state[i] = cmul(f, state[i]);
 }
 
 // Zero the amplitudes whose bit q is set.
-__global__ void sv_project_kernel(double2* __restrict__ state, uint64_t n, int q) {
+__global__ void sv_project_kernel(cd2* __restrict__ state, uint64_t n, int q) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        if ((i >> q) & 1ull) state[i] = make_double2(0.0, 0.0);
+        if ((i >> q) & 1ull) state[i] = mk2(0.0, 0.0);
 }
 
-__global__ void sv_probs_kernel(const double2* __restrict__ state, uint64_t n,
+__global__ void sv_probs_kernel(const cd2* __restrict__ state, uint64_t n,
                                 double* __restrict__ p) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const double2 v = state[i];
+        const cd2 v = state[i];
         p[i] = fma(v.x, v.x, v.y * v.y);
     }
 }
@@ -455,12 +471,12 @@ constexpr int kNormThreads = 256;
 
 // Per-block partial sums of |a|^2 (fixed grid, fixed order: deterministic).
 __global__ void __launch_bounds__(kNormThreads)
-sv_norm_partial_kernel(const double2* __restrict__ state, uint64_t n, double* __restrict__ part) {
+sv_norm_partial_kernel(const cd2* __restrict__ state, uint64_t n, double* __restrict__ part) {
     __shared__ double red[kNormThreads];
     double acc = 0.0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const double2 v = state[i];
+        const cd2 v = state[i];
         acc = fma(v.x, v.x, fma(v.y, v.y, acc));
     }
     red[threadIdx.x] = acc;
@@ -498,18 +514,18 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
         if (P.nslot > 0) {
             const uint64_t work = (uint64_t)P.nslot * blocks;
             hipLaunchKernelGGL(sv_phase_block_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256),
-                               0, stream, nq, dP, dphases, dterms, dfactors, blocks);
+                               0, stream, nq, dP, dphases, dterms, reinterpret_cast<cd2*>(dfactors), blocks);
             QS_HIP(hipGetLastError());
         }
         switch (P.R) {
             case 1: hipLaunchKernelGGL(sv_pass_kernel<1>, dim3(blocks), dim3(threads), 0, stream,
-                                       state, nq, dP, dstages, dgates, dphases, dterms, dfactors); break;
+                                       reinterpret_cast<cd2*>(state), nq, dP, dstages, dgates, dphases, dterms, reinterpret_cast<cd2*>(dfactors)); break;
             case 2: hipLaunchKernelGGL(sv_pass_kernel<2>, dim3(blocks), dim3(threads), 0, stream,
-                                       state, nq, dP, dstages, dgates, dphases, dterms, dfactors); break;
+                                       reinterpret_cast<cd2*>(state), nq, dP, dstages, dgates, dphases, dterms, reinterpret_cast<cd2*>(dfactors)); break;
             case 3: hipLaunchKernelGGL(sv_pass_kernel<3>, dim3(blocks), dim3(threads), 0, stream,
-                                       state, nq, dP, dstages, dgates, dphases, dterms, dfactors); break;
+                                       reinterpret_cast<cd2*>(state), nq, dP, dstages, dgates, dphases, dterms, reinterpret_cast<cd2*>(dfactors)); break;
             default: hipLaunchKernelGGL(sv_pass_kernel<4>, dim3(blocks), dim3(threads), 0, stream,
-                                        state, nq, dP, dstages, dgates, dphases, dterms, dfactors); break;
+                                        reinterpret_cast<cd2*>(state), nq, dP, dstages, dgates, dphases, dterms, reinterpret_cast<cd2*>(dfactors)); break;
         }
         QS_HIP(hipGetLastError());
     }
@@ -518,29 +534,29 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
 
 int sv_fill_zero(double2* state, int nq, hipStream_t stream) {
     const uint64_t n = 1ull << nq;
-    hipLaunchKernelGGL(sv_fill_zero_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, state, n);
+    hipLaunchKernelGGL(sv_fill_zero_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, reinterpret_cast<cd2*>(state), n);
     QS_HIP(hipGetLastError());
     return QS_OK;
 }
 
 int sv_scale(double2* state, int nq, double re, double im, hipStream_t stream) {
     const uint64_t n = 1ull << nq;
-    hipLaunchKernelGGL(sv_scale_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, state, n,
-                       make_double2(re, im));
+    hipLaunchKernelGGL(sv_scale_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream,
+                       reinterpret_cast<cd2*>(state), n, mk2(re, im));
     QS_HIP(hipGetLastError());
     return QS_OK;
 }
 
 int sv_project(double2* state, int nq, int q, hipStream_t stream) {
     const uint64_t n = 1ull << nq;
-    hipLaunchKernelGGL(sv_project_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, state, n, q);
+    hipLaunchKernelGGL(sv_project_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, reinterpret_cast<cd2*>(state), n, q);
     QS_HIP(hipGetLastError());
     return QS_OK;
 }
 
 int sv_probs(const double2* state, int nq, double* p, hipStream_t stream) {
     const uint64_t n = 1ull << nq;
-    hipLaunchKernelGGL(sv_probs_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, state, n, p);
+    hipLaunchKernelGGL(sv_probs_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, reinterpret_cast<const cd2*>(state), n, p);
     QS_HIP(hipGetLastError());
     return QS_OK;
 }
@@ -550,7 +566,7 @@ int sv_norm_sq(const double2* state, int nq, double* out, hipStream_t stream) {
     double* part = nullptr;
     QS_HIP(hipMallocAsync(reinterpret_cast<void**>(&part), kNormBlocks * sizeof(double), stream));
     hipLaunchKernelGGL(sv_norm_partial_kernel, dim3(kNormBlocks), dim3(kNormThreads), 0, stream,
-                       state, n, part);
+                       reinterpret_cast<const cd2*>(state), n, part);
     QS_HIP(hipGetLastError());
     std::vector<double> h(kNormBlocks);
     QS_HIP(hipMemcpyAsync(h.data(), part, kNormBlocks * sizeof(double), hipMemcpyDeviceToHost,

@@ -55,6 +55,8 @@ class MPSChain:
         self._ws = None
         self._ws_tasks = 0
         self._status = None
+        self._err = None
+        self._last_batch = 0
         self.updates = 0
 
     # -- plumbing ------------------------------------------------------------
@@ -102,6 +104,10 @@ class MPSChain:
         _lib.call('qs_mps_update_batch', tasks, k, prec, ctypes.c_void_p(ws.data_ptr()),
                   ws.numel(), ctypes.c_void_p(status.data_ptr()), self._stream())
         self._dlogs.append(dl)
+        self._last_batch = k
+        # running minimum of the statuses (negative = failure), kept on the device
+        m = status[:k].min().reshape(1)
+        self._err = m if self._err is None else torch.minimum(self._err, m)
         self.updates += k
 
     def apply_2q(self, i: int, g: np.ndarray) -> None:
@@ -117,12 +123,16 @@ class MPSChain:
                   self._stream())
 
     def check_status(self) -> None:
-        """Raise if any update of the last batches failed to converge."""
-        if self._status is not None:
-            bad = self._status.cpu().numpy()
-            if (bad != 0).any():
-                raise RuntimeError(f'MPS update status {int(bad[bad != 0][0])} '
+        """Raise if any two-site update so far failed (status < 0)."""
+        if self._err is not None:
+            worst = int(self._err.cpu().item())
+            if worst < 0:
+                raise RuntimeError(f'MPS update status {worst} '
                                    f'(QS_E_NOCONV={_lib.QS_E_NOCONV}, QS_E_CHI={_lib.QS_E_CHI})')
+
+    def last_sweeps(self) -> np.ndarray:
+        """Jacobi sweeps of each update of the last batch."""
+        return self._status.cpu().numpy()[:self._last_batch].copy()
 
     # -- host views -----------------------------------------------------------
     @property

@@ -1,0 +1,134 @@
+"""Many independent MPS circuits of uneven length and bond dimension, batched.
+
+Each circuit is a brick-wall of two-site gates on its own chain (own n, chi_max,
+depth, seed).  Round r submits layer r of every circuit whose depth exceeds r as
+one qs_mps_update_batch call (split into chunks of at most `max_tasks` tasks so
+the theta workspace stays bounded); within a chunk the library groups tasks by
+SVD size class.  Task tables are built as NumPy structured arrays with the
+qs_mps_task layout, and gates are generated per (circuit, layer) in vectorised
+form, so the host keeps up with the device at millions of updates.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib
+from .mps import MPSChain, PRECISIONS
+
+TASK_DTYPE = np.dtype([('sites', '<u8'), ('bond', '<u8'), ('dlog', '<u8'),
+                       ('site_stride', '<i8'), ('site', '<i4'), ('chi_max', '<i4'),
+                       ('cutoff', '<f8'), ('gate', '<f8', (32,))])
+assert TASK_DTYPE.itemsize == ctypes.sizeof(_lib.QsMpsTask)
+
+
+def near_identity_batch(rng: np.random.Generator, k: int, eps: float = 0.1) -> np.ndarray:
+    """k gates exp(-i*eps*H), H = (M + M^H)/2, M complex Gaussian: (k, 4, 4)."""
+    m = rng.standard_normal((k, 4, 4)) + 1j * rng.standard_normal((k, 4, 4))
+    h = 0.5 * (m + np.conj(np.swapaxes(m, 1, 2)))
+    w, v = np.linalg.eigh(h)
+    return (v * np.exp(-1j * eps * w)[:, None, :]) @ np.conj(np.swapaxes(v, 1, 2))
+
+
+@dataclass
+class CircuitSpec:
+    n: int
+    chi_max: int
+    depth: int
+    seed: int
+
+
+def random_specs(count: int = 256, seed: int = 7, n_range=(50, 2000),
+                 chis=(16, 32, 64), depth_range=(10, 60)) -> List[CircuitSpec]:
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(count):
+        out.append(CircuitSpec(int(rng.integers(n_range[0], n_range[1] + 1)),
+                               int(rng.choice(chis)),
+                               int(rng.integers(depth_range[0], depth_range[1] + 1)),
+                               int(seed * 100003 + c)))
+    return out
+
+
+class MultiRun:
+    """Runs a list of circuit specs on one GPU, round by round."""
+
+    def __init__(self, specs: List[CircuitSpec], cutoff: float = 1e-10,
+                 precision: str = 'fp32', device: int = 0, max_tasks: int = 16384):
+        import torch
+        self.specs = specs
+        self.cutoff = cutoff
+        self.prec = PRECISIONS[precision]
+        self.device = device
+        self.max_tasks = max_tasks
+        self.chains = [MPSChain(s.n, s.chi_max, cutoff, device, precision) for s in specs]
+        self.rngs = [np.random.default_rng(s.seed) for s in specs]
+        total = sum((s.n // 2) * ((s.depth + 1) // 2) + ((s.n - 1) // 2) * (s.depth // 2)
+                    for s in specs)
+        self.total_updates = total
+        dev = f'cuda:{device}'
+        self.dlog = torch.zeros(total, dtype=torch.float64, device=dev)
+        self.owner = np.empty(total, dtype=np.int32)      # circuit of each update
+        self._next = 0
+        chi = max(s.chi_max for s in specs)
+        nbytes = _lib.lib().qs_mps_workspace_bytes(max_tasks, chi)
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.status = torch.zeros(max_tasks, dtype=torch.int32, device=dev)
+        self.err = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.rounds = max(s.depth for s in specs)
+
+    def _round_tasks(self, r: int) -> np.ndarray:
+        parts = []
+        dbase = self.dlog.data_ptr()
+        for c, (s, ch) in enumerate(zip(self.specs, self.chains)):
+            if r >= s.depth:
+                continue
+            sites = np.arange(r % 2, s.n - 1, 2, dtype=np.int32)
+            k = sites.size
+            if k == 0:
+                continue
+            g = near_identity_batch(self.rngs[c], k)
+            t = np.zeros(k, dtype=TASK_DTYPE)
+            t['sites'] = ch.sites.data_ptr()
+            t['bond'] = ch.bond.data_ptr()
+            t['dlog'] = dbase + 8 * (self._next + np.arange(k, dtype=np.uint64))
+            t['site_stride'] = ch.stride
+            t['site'] = sites
+            t['chi_max'] = s.chi_max
+            t['cutoff'] = self.cutoff
+            gf = g.reshape(k, 16)
+            t['gate'][:, 0::2] = gf.real
+            t['gate'][:, 1::2] = gf.imag
+            self.owner[self._next:self._next + k] = c
+            self._next += k
+            parts.append(t)
+        return np.concatenate(parts) if parts else np.zeros(0, dtype=TASK_DTYPE)
+
+    def _submit(self, tasks: np.ndarray) -> None:
+        import torch
+        stream = _lib.current_stream(self.device)
+        for b in range(0, tasks.size, self.max_tasks):
+            chunk = np.ascontiguousarray(tasks[b:b + self.max_tasks])
+            ptr = chunk.ctypes.data_as(ctypes.POINTER(_lib.QsMpsTask))
+            _lib.call('qs_mps_update_batch', ptr, chunk.size, self.prec,
+                      ctypes.c_void_p(self.ws.data_ptr()), self.ws.numel(),
+                      ctypes.c_void_p(self.status.data_ptr()), stream)
+            self.err = torch.minimum(self.err, self.status[:chunk.size].min().reshape(1))
+
+    def run(self, rounds: Optional[int] = None) -> None:
+        for r in range(self.rounds if rounds is None else rounds):
+            self._submit(self._round_tasks(r))
+
+    def check(self) -> None:
+        worst = int(self.err.cpu().item())
+        if worst < 0:
+            raise RuntimeError(f'a two-site update failed with status {worst}')
+
+    def norm_logs(self) -> np.ndarray:
+        """norm_log of every circuit (increments summed in update order)."""
+        d = self.dlog.cpu().numpy()[:self._next]
+        return np.bincount(self.owner[:self._next], weights=d, minlength=len(self.specs))

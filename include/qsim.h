@@ -176,7 +176,7 @@ size_t qs_mps_workspace_bytes(int ntasks, int chi_max);
  * device contraction contract_and_apply_2q (fpga_mps_simulator.cpp:287-371).
  * `workspace` is a device buffer of at least qs_mps_workspace_bytes(ntasks, max chi_max).
  * When `status_dev` is non-null it receives per-task statuses (device int32[ntasks]):
- * 0 or QS_E_NOCONV. */
+ * the number of Jacobi sweeps used (>= 1) on success, QS_E_NOCONV or QS_E_CHI. */
 int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision,
                         void* workspace, size_t ws_bytes, int32_t* status_dev,
                         void* stream);
