@@ -249,7 +249,9 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
     os << "{\"passes\":[";
     for (size_t p = 0; p < plan.passes.size(); ++p) {
         const DevPass& P = plan.passes[p];
-        os << (p ? "," : "") << "{\"K\":" << P.K << ",\"R\":" << P.R << ",\"tq\":[";
+        os << (p ? "," : "") << "{\"K\":" << P.K << ",\"R\":" << P.R << ",\"load_regs\":["
+           << P.load.regs[0] << "," << P.load.regs[1] << "," << P.load.regs[2] << ","
+           << P.load.regs[3] << "],\"tq\":[";
         for (int i = 0; i < P.K; ++i) os << (i ? "," : "") << P.tq[i];
         os << "],\"stages\":[";
         for (int s = 0; s < P.nstage; ++s) {
@@ -264,7 +266,7 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
                    << ",\"cmask\":" << G.cmask << ",\"m\":[";
                 for (int k = 0; k < 32; ++k) os << (k ? "," : "") << G.m[k];
                 os << "]";
-                if (G.kind == DG_PHASE) {
+                if (G.kind == DG_PHASE || G.kind == DG_PHASE_SIGN) {
                     const DevPhase& Q = plan.phases[P.phase0 + G.qa];
                     os << ",\"phase\":{\"sign\":" << Q.sign << ",\"slot\":" << Q.slot
                        << ",\"c\":[" << Q.c[0] << "," << Q.c[1] << "],\"kr\":[";

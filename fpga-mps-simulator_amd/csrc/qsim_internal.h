@@ -52,22 +52,44 @@ constexpr int SV_MAX_QUBITS = 34;      // 2^34 * 16 B = 256 GiB, inside 288 GB o
 constexpr int SV_TILE_BITS = 12;       // qubits per fused pass tile (4096 amplitudes)
 constexpr int SV_REG_BITS = 4;         // amplitudes per thread = 16
 constexpr int SV_LOW_BITS = 4;         // qubits 0..3 are in every tile (256 B runs)
-constexpr int SV_MAX_STAGES = 24;      // register layouts per pass
 
-// Device gate of a fused pass.  U1/U2 name register bits (positions within the
-// thread's 2^R register block); DIAG1/DIAG2 name global qubits (general diagonal,
-// evaluated per amplitude); PHASE is a group of unitary diagonal gates described by
-// a DevPhase record (index in qa).
-enum : int32_t { DG_U1 = 0, DG_U2 = 1, DG_DIAG1 = 2, DG_DIAG2 = 3, DG_PHASE = 5 };
+// Device op of a fused pass (one record per op, executed in order by every thread).
+//   DG_U1X    one-qubit gates on every register bit; the 2x2 matrix of register bit b
+//             is m[8b .. 8b+7] (identity where r0, the mask of bits carrying a gate,
+//             has no bit)
+//   DG_U1S    uncontrolled one-qubit gate on register bit r0, matrix m[0..7]
+//   DG_U1     controlled one-qubit gate on register bit r0, control mask cmask
+//   DG_U2     two-qubit gate on register bits r0 (row-index MSB) and r1, controls cmask
+//   DG_DIAG1/2 general diagonal on global qubits qa (MSB for DIAG2) and qb, evaluated
+//             per amplitude (non-unit entries or several controls)
+//   DG_PHASE / DG_PHASE_SIGN  unitary diagonal group, DevPhase record qa; the SIGN form
+//             has only +-1 factors
+//   DG_LAYOUT transpose through LDS to the DevLayout stored at the start of m
+enum : int32_t {
+    DG_U1 = 0, DG_U2 = 1, DG_DIAG1 = 2, DG_DIAG2 = 3, DG_U1X = 4, DG_PHASE = 5,
+    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8
+};
 
 struct alignas(16) DevGate {
     int32_t kind;
-    int32_t r0, r1;        // register bits (U1: r0; U2: r0 = MSB of row index)
-    int32_t qa, qb;        // DIAG: global qubits (qa = MSB for DIAG2); PHASE: qa = record
+    int32_t r0, r1;
+    int32_t qa, qb;
     int32_t pad;
     uint64_t cmask;        // global control mask
     double m[32];
 };
+
+// A register layout of a pass, precomputed on the host: which tile positions are the
+// register bits, and where each thread-index bit lands (local position and global
+// qubit).  Thread bits fill the non-register positions in ascending order; entries
+// beyond the thread count are zero (those thread bits are always 0).
+struct DevLayout {
+    int32_t regs[4];       // local positions of register bits 0..R-1, ascending
+    int32_t rq[4];         // global qubits of register bits
+    int32_t tpos[8];       // local position of thread bit b
+    int32_t tq[8];         // global qubit of thread bit b
+};
+static_assert(sizeof(DevLayout) <= sizeof(double) * 32, "DevLayout must fit in DevGate::m");
 
 // A unitary diagonal group is a product of phase factors f^(bit products).  Factors are
 // classified by where their qubits live in the pass: block bits (not in the tile),
@@ -100,9 +122,12 @@ struct alignas(16) DevPhase {
     double crr[6][2];      // register-pair factors: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
 };
 
+// Planner-side stage: register positions and the op range run under them (the op
+// range starts with the DG_LAYOUT op that establishes the layout, except for a first
+// stage loaded directly from HBM).  The device reads only the op stream.
 struct DevStage {
     int32_t regs[4];       // local tile positions of the register bits, ascending
-    int32_t g0, g1;        // gate range in the pass's gate array
+    int32_t g0, g1;        // op range in the pass's op array
 };
 
 struct DevPass {
@@ -110,13 +135,16 @@ struct DevPass {
     int32_t tq[SV_TILE_BITS];  // tile qubits ascending; local bit i <-> global qubit tq[i]
     uint64_t tile_mask;
     int32_t stage0;            // first stage in the stage array
-    int32_t gate0;             // first gate in the gate array
+    int32_t gate0;             // first op in the op array
     int32_t phase0;            // first phase record
     int32_t term0;             // first term
     int32_t nphase;            // phase records of this pass
     int32_t nslot;             // block-factor slots of this pass
     int32_t slot0;             // first slot in the block-factor table
-    int32_t pad;
+    int32_t nop;               // ops of this pass
+    int32_t nbq;               // block qubits (not in the tile): nq - K, or 0
+    int32_t bq[SV_MAX_QUBITS]; // block qubits ascending: bit i of the tile index -> bq[i]
+    DevLayout load;            // HBM load layout (the store layout is the last one set)
 };
 
 // Host-side logical gate, after name -> matrix conversion.
@@ -152,6 +180,9 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan);
 // Launches every pass of `plan` (already uploaded to `dev_blob`) on `stream`.
 int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_blob,
                    hipStream_t stream);
+
+// Fills a DevLayout for register positions regs[0..R-1] (ascending) of pass P.
+void sv_fill_layout(const DevPass& P, const int* regs, DevLayout* out);
 
 int sv_fill_zero(double2* state, int nq, hipStream_t stream);
 int sv_scale(double2* state, int nq, double re, double im, hipStream_t stream);

@@ -31,11 +31,12 @@ __device__ __forceinline__ int lds_slot(int l) { return l + (l >> 4); }
 
 constexpr int kLdsSlots = (1 << SV_TILE_BITS) + ((1 << SV_TILE_BITS) >> 4);
 
-// Per-stage addressing: local index and global index of register j of this thread.
+// Addressing under one register layout: local (LDS) index and the thread's part of
+// the global index of register j; the tile's block bits are OR-ed in separately.
 template <int R>
 struct Layout {
     int loc_t;                 // local index bits contributed by the thread id
-    uint64_t glob_t;           // global bits of base | thread part
+    uint64_t glob_t;           // global index bits contributed by the thread id
     int lreg[R];               // local bit of register bit i
     uint64_t greg[R];          // global bit of register bit i
 
@@ -55,158 +56,179 @@ struct Layout {
     }
 };
 
-struct Regs {
-    int v[4];
-};
-
 template <int R>
-__device__ __forceinline__ void make_layout(Layout<R>& L, const Regs regs, const DevPass* P,
-                                            uint64_t base, int t) {
-    int regmask = 0;
+__device__ __forceinline__ void make_layout(Layout<R>& L, const DevLayout& D, int t) {
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        L.lreg[i] = 1 << regs.v[i];
-        L.greg[i] = 1ull << P->tq[regs.v[i]];
-        regmask |= 1 << regs.v[i];
+        L.lreg[i] = 1 << D.regs[i];
+        L.greg[i] = 1ull << D.rq[i];
     }
-    // Thread bits fill the non-register local positions in ascending order.
     int loc = 0;
-    uint64_t g = base;
-    int tb = 0;
-    const int K = P->K;
-    for (int p = 0; p < K; ++p) {
-        if ((regmask >> p) & 1) continue;
-        const int bit = (t >> tb) & 1;
-        loc |= bit << p;
-        g |= (uint64_t)bit << P->tq[p];
-        ++tb;
+    uint64_t g = 0;
+#pragma unroll
+    for (int b = 0; b < SV_TILE_BITS - R; ++b) {
+        const int bit = (t >> b) & 1;
+        loc |= bit << D.tpos[b];
+        g |= (uint64_t)bit << D.tq[b];
     }
     L.loc_t = loc;
     L.glob_t = g;
 }
 
+// Global index bits of tile `tile`: bit i of the tile index lands on block qubit bq[i].
+__device__ __forceinline__ uint64_t tile_base(const DevPass* P, uint32_t tile) {
+    uint64_t base = 0;
+    const int nb = P->nbq;
+    for (int i = 0; i < nb; ++i) base |= (uint64_t)((tile >> i) & 1u) << P->bq[i];
+    return base;
+}
+
+// Uncontrolled 2x2 matrix m (row-major complex) on register bit B, in place.
 template <int R, int B>
-__device__ __forceinline__ void apply_u1_bit(cd2 (&a)[1 << R], const DevGate& G,
-                                             const Layout<R>& L) {
+__device__ __forceinline__ void u1_bit(cd2 (&a)[1 << R], const double* __restrict__ m) {
+    const cd2 m00 = mk2(m[0], m[1]), m01 = mk2(m[2], m[3]);
+    const cd2 m10 = mk2(m[4], m[5]), m11 = mk2(m[6], m[7]);
+    int npair = 0;
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        if ((j >> B) & 1) continue;           // compile-time after unrolling
+        const cd2 x0 = a[j], x1 = a[j | (1 << B)];
+        a[j] = cmadd(cmul(m00, x0), m01, x1);
+        a[j | (1 << B)] = cmadd(cmul(m10, x0), m11, x1);
+        // Two pairs (eight independent FMA chains) per scheduling group keep the FP64
+        // pipe fed without holding every pair's results in temporaries at once.
+        if (++npair % 2 == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// One-qubit gates on the register bits set in G.r0 (DG_U1X).  The bits commute, so
+// they run in register-bit order.
+template <int R>
+__device__ __forceinline__ void apply_u1x(cd2 (&a)[1 << R], const DevGate& G) {
+    const int mask = G.r0;
+    if (mask & 1) u1_bit<R, 0>(a, G.m);
+    if constexpr (R > 1) if (mask & 2) u1_bit<R, 1>(a, G.m + 8);
+    if constexpr (R > 2) if (mask & 4) u1_bit<R, 2>(a, G.m + 16);
+    if constexpr (R > 3) if (mask & 8) u1_bit<R, 3>(a, G.m + 24);
+}
+
+// Controlled 2x2 matrix on register bit B (DG_U1): amplitudes whose global index
+// lacks a control bit keep their value (branch-free select).
+template <int R, int B>
+__device__ __forceinline__ void apply_cu1_bit(cd2 (&a)[1 << R], const DevGate& G,
+                                              const Layout<R>& L, uint64_t base) {
     const cd2 m00 = mk2(G.m[0], G.m[1]), m01 = mk2(G.m[2], G.m[3]);
     const cd2 m10 = mk2(G.m[4], G.m[5]), m11 = mk2(G.m[6], G.m[7]);
     const uint64_t cm = G.cmask;
-    if (cm == 0) {
-#pragma unroll
-        for (int j = 0; j < (1 << R); ++j) {
-            if ((j >> B) & 1) continue;           // compile-time after unrolling
-            const cd2 x0 = a[j], x1 = a[j | (1 << B)];
-            a[j] = cmadd(cmul(m00, x0), m01, x1);
-            a[j | (1 << B)] = cmadd(cmul(m10, x0), m11, x1);
-        }
-        return;
-    }
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
         if ((j >> B) & 1) continue;
-        const bool ok = (L.glob(j) & cm) == cm;
+        const bool ok = ((base | L.glob(j)) & cm) == cm;
         const cd2 x0 = a[j], x1 = a[j | (1 << B)];
         const cd2 y0 = cmadd(cmul(m00, x0), m01, x1);
         const cd2 y1 = cmadd(cmul(m10, x0), m11, x1);
         a[j] = ok ? y0 : x0;
         a[j | (1 << B)] = ok ? y1 : x1;
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
+// Two-qubit matrix on register bits B0 (row-index MSB) and B1; amplitudes whose global
+// index lacks a control bit keep their value (branch-free select; cmask 0 selects all).
 template <int R, int B0, int B1>
 __device__ __forceinline__ void apply_u2_bits(cd2 (&a)[1 << R], const DevGate& G,
-                                              const Layout<R>& L) {
+                                              const Layout<R>& L, uint64_t base) {
     const uint64_t cm = G.cmask;
     const cd2* m = reinterpret_cast<const cd2*>(G.m);
+    constexpr int b0 = 1 << B0, b1 = 1 << B1;
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
-        if (j & ((1 << B0) | (1 << B1))) continue;
-        if (cm != 0 && (L.glob(j) & cm) != cm) continue;
-        constexpr int b0 = 1 << B0, b1 = 1 << B1;
+        if (j & (b0 | b1)) continue;          // compile-time after unrolling
+        const bool ok = ((base | L.glob(j)) & cm) == cm;
         const cd2 x0 = a[j], x1 = a[j | b1], x2 = a[j | b0], x3 = a[j | b0 | b1];
+        cd2 y[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const cd2 y = cmadd(cmadd(cmadd(cmul(m[4 * r], x0), m[4 * r + 1], x1),
-                                          m[4 * r + 2], x2), m[4 * r + 3], x3);
-            if (r == 0) a[j] = y;
-            else if (r == 1) a[j | b1] = y;
-            else if (r == 2) a[j | b0] = y;
-            else a[j | b0 | b1] = y;
-        }
+        for (int r = 0; r < 4; ++r)
+            y[r] = cmadd(cmadd(cmadd(cmul(m[4 * r], x0), m[4 * r + 1], x1), m[4 * r + 2], x2),
+                         m[4 * r + 3], x3);
+        a[j] = ok ? y[0] : x0;
+        a[j | b1] = ok ? y[1] : x1;
+        a[j | b0] = ok ? y[2] : x2;
+        a[j | b0 | b1] = ok ? y[3] : x3;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Ops that differ only in which registers they touch are dispatched by a switch whose
+// case bodies are bracketed by distinct empty asm statements: without them the
+// optimizer merges the isomorphic bodies behind pointer selects, which moves the
+// register block to scratch memory.  Every case body is straight-line code.
+#define QS_CASE(tag) asm volatile("; " tag)
+
+template <int R>
+__device__ __forceinline__ void apply_u1s(cd2 (&a)[1 << R], const DevGate& G) {
+    constexpr int S1 = R > 1 ? 1 : 0, S2 = R > 2 ? 2 : 0, S3 = R > 3 ? 3 : 0;
+    switch (G.r0) {
+        case 0: QS_CASE("u1s 0 <"); u1_bit<R, 0>(a, G.m); QS_CASE("u1s 0 >"); break;
+        case 1: if constexpr (R > 1) { QS_CASE("u1s 1 <"); u1_bit<R, S1>(a, G.m); QS_CASE("u1s 1 >"); } break;
+        case 2: if constexpr (R > 2) { QS_CASE("u1s 2 <"); u1_bit<R, S2>(a, G.m); QS_CASE("u1s 2 >"); } break;
+        default: if constexpr (R > 3) { QS_CASE("u1s 3 <"); u1_bit<R, S3>(a, G.m); QS_CASE("u1s 3 >"); } break;
     }
 }
 
 template <int R>
-__device__ __forceinline__ void apply_u1(cd2 (&a)[1 << R], const DevGate& G,
-                                         const Layout<R>& L) {
+__device__ __forceinline__ void apply_cu1(cd2 (&a)[1 << R], const DevGate& G,
+                                          const Layout<R>& L, uint64_t base) {
+    constexpr int S1 = R > 1 ? 1 : 0, S2 = R > 2 ? 2 : 0, S3 = R > 3 ? 3 : 0;
     switch (G.r0) {
-        case 0: apply_u1_bit<R, 0>(a, G, L); break;
-        case 1: if constexpr (R > 1) apply_u1_bit<R, (R > 1 ? 1 : 0)>(a, G, L); break;
-        case 2: if constexpr (R > 2) apply_u1_bit<R, (R > 2 ? 2 : 0)>(a, G, L); break;
-        case 3: if constexpr (R > 3) apply_u1_bit<R, (R > 3 ? 3 : 0)>(a, G, L); break;
-        default: break;
+        case 0: QS_CASE("cu1 0 <"); apply_cu1_bit<R, 0>(a, G, L, base); QS_CASE("cu1 0 >"); break;
+        case 1: if constexpr (R > 1) { QS_CASE("cu1 1 <"); apply_cu1_bit<R, S1>(a, G, L, base); QS_CASE("cu1 1 >"); } break;
+        case 2: if constexpr (R > 2) { QS_CASE("cu1 2 <"); apply_cu1_bit<R, S2>(a, G, L, base); QS_CASE("cu1 2 >"); } break;
+        default: if constexpr (R > 3) { QS_CASE("cu1 3 <"); apply_cu1_bit<R, S3>(a, G, L, base); QS_CASE("cu1 3 >"); } break;
     }
 }
 
 template <int R>
 __device__ __forceinline__ void apply_u2(cd2 (&a)[1 << R], const DevGate& G,
-                                         const Layout<R>& L) {
+                                         const Layout<R>& L, uint64_t base) {
     constexpr int S1 = R > 1 ? 1 : 0, S2 = R > 2 ? 2 : 0, S3 = R > 3 ? 3 : 0;
-    const int code = G.r0 * 4 + G.r1;
-    if constexpr (R >= 2) {
-        switch (code) {
-            case 0 * 4 + 1: apply_u2_bits<R, 0, S1>(a, G, L); return;
-            case 1 * 4 + 0: apply_u2_bits<R, S1, 0>(a, G, L); return;
-            default: break;
-        }
-    }
-    if constexpr (R >= 3) {
-        switch (code) {
-            case 0 * 4 + 2: apply_u2_bits<R, 0, S2>(a, G, L); return;
-            case 2 * 4 + 0: apply_u2_bits<R, S2, 0>(a, G, L); return;
-            case 1 * 4 + 2: apply_u2_bits<R, S1, S2>(a, G, L); return;
-            case 2 * 4 + 1: apply_u2_bits<R, S2, S1>(a, G, L); return;
-            default: break;
-        }
-    }
-    if constexpr (R >= 4) {
-        switch (code) {
-            case 0 * 4 + 3: apply_u2_bits<R, 0, S3>(a, G, L); return;
-            case 3 * 4 + 0: apply_u2_bits<R, S3, 0>(a, G, L); return;
-            case 1 * 4 + 3: apply_u2_bits<R, S1, S3>(a, G, L); return;
-            case 3 * 4 + 1: apply_u2_bits<R, S3, S1>(a, G, L); return;
-            case 2 * 4 + 3: apply_u2_bits<R, S2, S3>(a, G, L); return;
-            case 3 * 4 + 2: apply_u2_bits<R, S3, S2>(a, G, L); return;
-            default: break;
-        }
+    switch (G.r0 * 4 + G.r1) {
+        case 0 * 4 + 1: if constexpr (R >= 2) { QS_CASE("u2 01 <"); apply_u2_bits<R, 0, S1>(a, G, L, base); QS_CASE("u2 01 >"); } break;
+        case 1 * 4 + 0: if constexpr (R >= 2) { QS_CASE("u2 10 <"); apply_u2_bits<R, S1, 0>(a, G, L, base); QS_CASE("u2 10 >"); } break;
+        case 0 * 4 + 2: if constexpr (R >= 3) { QS_CASE("u2 02 <"); apply_u2_bits<R, 0, S2>(a, G, L, base); QS_CASE("u2 02 >"); } break;
+        case 2 * 4 + 0: if constexpr (R >= 3) { QS_CASE("u2 20 <"); apply_u2_bits<R, S2, 0>(a, G, L, base); QS_CASE("u2 20 >"); } break;
+        case 1 * 4 + 2: if constexpr (R >= 3) { QS_CASE("u2 12 <"); apply_u2_bits<R, S1, S2>(a, G, L, base); QS_CASE("u2 12 >"); } break;
+        case 2 * 4 + 1: if constexpr (R >= 3) { QS_CASE("u2 21 <"); apply_u2_bits<R, S2, S1>(a, G, L, base); QS_CASE("u2 21 >"); } break;
+        case 0 * 4 + 3: if constexpr (R >= 4) { QS_CASE("u2 03 <"); apply_u2_bits<R, 0, S3>(a, G, L, base); QS_CASE("u2 03 >"); } break;
+        case 3 * 4 + 0: if constexpr (R >= 4) { QS_CASE("u2 30 <"); apply_u2_bits<R, S3, 0>(a, G, L, base); QS_CASE("u2 30 >"); } break;
+        case 1 * 4 + 3: if constexpr (R >= 4) { QS_CASE("u2 13 <"); apply_u2_bits<R, S1, S3>(a, G, L, base); QS_CASE("u2 13 >"); } break;
+        case 3 * 4 + 1: if constexpr (R >= 4) { QS_CASE("u2 31 <"); apply_u2_bits<R, S3, S1>(a, G, L, base); QS_CASE("u2 31 >"); } break;
+        case 2 * 4 + 3: if constexpr (R >= 4) { QS_CASE("u2 23 <"); apply_u2_bits<R, S2, S3>(a, G, L, base); QS_CASE("u2 23 >"); } break;
+        case 3 * 4 + 2: if constexpr (R >= 4) { QS_CASE("u2 32 <"); apply_u2_bits<R, S3, S2>(a, G, L, base); QS_CASE("u2 32 >"); } break;
+        default: break;
     }
 }
 
+// General diagonal (non-unit entries or several controls), per amplitude; amplitudes
+// without every control bit are multiplied by exactly 1.
 template <int R>
 __device__ __forceinline__ void apply_diag(cd2 (&a)[1 << R], const DevGate& G,
-                                           const Layout<R>& L) {
+                                           const Layout<R>& L, uint64_t base) {
     const uint64_t cm = G.cmask;
     const int qa = G.qa, qb = G.qb;
-    if (G.kind == DG_DIAG1) {
-        const cd2 d0 = mk2(G.m[0], G.m[1]), d1 = mk2(G.m[2], G.m[3]);
-#pragma unroll
-        for (int j = 0; j < (1 << R); ++j) {
-            const uint64_t g = L.glob(j);
-            if (cm != 0 && (g & cm) != cm) continue;
-            a[j] = cmul(((g >> qa) & 1ull) ? d1 : d0, a[j]);
-        }
-        return;
-    }
-    // DG_DIAG2: index = 2*bit(qa) + bit(qb)
-    const cd2 d00 = mk2(G.m[0], G.m[1]), d01 = mk2(G.m[2], G.m[3]);
-    const cd2 d10 = mk2(G.m[4], G.m[5]), d11 = mk2(G.m[6], G.m[7]);
+    const cd2 one = mk2(1.0, 0.0);
+    const bool two = G.kind == DG_DIAG2;
+    // DG_DIAG1: entries d0, d1 by bit(qa); DG_DIAG2: d[2*bit(qa) + bit(qb)]
+    const cd2 d0 = mk2(G.m[0], G.m[1]), d1 = mk2(G.m[2], G.m[3]);
+    const cd2 d2 = mk2(G.m[4], G.m[5]), d3 = mk2(G.m[6], G.m[7]);
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
-        const uint64_t g = L.glob(j);
-        if (cm != 0 && (g & cm) != cm) continue;
-        const bool ba = (g >> qa) & 1ull, bb = (g >> qb) & 1ull;
-        const cd2 d = ba ? (bb ? d11 : d10) : (bb ? d01 : d00);
-        a[j] = cmul(d, a[j]);
+        const uint64_t g = base | L.glob(j);
+        const bool ok = (g & cm) == cm;
+        const bool ba = (g >> qa) & 1ull;
+        const bool bb = two && ((g >> qb) & 1ull);
+        const cd2 d = two ? (ba ? (bb ? d3 : d2) : (bb ? d1 : d0)) : (ba ? d1 : d0);
+        a[j] = cmul(ok ? d : one, a[j]);
     }
 }
 
@@ -215,86 +237,124 @@ __device__ __forceinline__ constexpr unsigned reg_col_mask(int r) {
     return r == 0 ? 0xAAAAu : r == 1 ? 0xCCCCu : r == 2 ? 0xF0F0u : 0xFF00u;
 }
 
-__device__ __forceinline__ bool is_unit(cd2 z) { return z.x == 1.0 && z.y == 0.0; }
+// Per-thread factors of a unitary diagonal group (DevPhase): the constant A of this
+// thread (constant x block factor x thread-bit terms) and kappa_r of each register
+// bit r.  Block-class factors come from the prologue's table.
+struct PhaseFactors {
+    cd2 A, k0, k1, k2, k3;     // scalars: an indexed array would live in scratch
+};
 
-// Applies a unitary diagonal group (DevPhase).  Block-class factors come from the
-// prologue's table, thread-class factors are folded per thread, register factors are
-// applied to the halves/quarters of the register block they select.
 template <int R>
-__device__ __forceinline__ void apply_phase(cd2 (&a)[1 << R], const DevPhase& Q,
-                                            const DevTerm* __restrict__ terms,
-                                            const cd2* __restrict__ bf, int t) {
-    constexpr int NR = 1 << R;
-    cd2 A = mk2(Q.c[0], Q.c[1]);
-    // per-register-bit factors as scalars (an indexed array would live in scratch)
-    cd2 k0 = mk2(1.0, 0.0), k1 = k0, k2 = k0, k3 = k0;
+__device__ __forceinline__ PhaseFactors phase_factors(const DevPhase& Q,
+                                                      const DevTerm* __restrict__ terms,
+                                                      const cd2* __restrict__ bf, int t) {
+    PhaseFactors F;
+    F.A = mk2(Q.c[0], Q.c[1]);
+    F.k0 = mk2(Q.kr[0][0], Q.kr[0][1]);
+    F.k1 = mk2(Q.kr[1][0], Q.kr[1][1]);
+    F.k2 = mk2(Q.kr[2][0], Q.kr[2][1]);
+    F.k3 = mk2(Q.kr[3][0], Q.kr[3][1]);
+    // Control flow depends only on uniform data; thread-dependent factors are selected.
     if (bf) {
-        A = cmul(A, bf[0]);
-        for (int q = 0; q < 8; ++q)
-            if ((t >> q) & 1) A = cmul(A, bf[1 + q]);
-        k0 = bf[9];
-        if (R > 1) k1 = bf[10];
-        if (R > 2) k2 = bf[11];
-        if (R > 3) k3 = bf[12];
+        F.A = cmul(F.A, bf[0]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const cd2 Af = cmul(F.A, bf[1 + q]);
+            F.A = ((t >> q) & 1) ? Af : F.A;
+        }
+        F.k0 = cmul(F.k0, bf[9]);
+        if (R > 1) F.k1 = cmul(F.k1, bf[10]);
+        if (R > 2) F.k2 = cmul(F.k2, bf[11]);
+        if (R > 3) F.k3 = cmul(F.k3, bf[12]);
     }
-    k0 = cmul(k0, mk2(Q.kr[0][0], Q.kr[0][1]));
-    if (R > 1) k1 = cmul(k1, mk2(Q.kr[1][0], Q.kr[1][1]));
-    if (R > 2) k2 = cmul(k2, mk2(Q.kr[2][0], Q.kr[2][1]));
-    if (R > 3) k3 = cmul(k3, mk2(Q.kr[3][0], Q.kr[3][1]));
     for (int k = 0; k < Q.nt; ++k) {
         const DevTerm& T = terms[Q.t0 + k];
         const cd2 f = mk2(T.re, T.im);
-        if (T.cls == PT_T1) {
-            if ((t >> T.a) & 1) A = cmul(A, f);
-        } else if (T.cls == PT_TT) {
-            if (((t >> T.a) & (t >> T.b)) & 1) A = cmul(A, f);
-        } else if ((t >> T.b) & 1) {  // PT_TR
-            if (T.a == 0) k0 = cmul(k0, f);
-            else if (T.a == 1) k1 = cmul(k1, f);
-            else if (T.a == 2) k2 = cmul(k2, f);
-            else k3 = cmul(k3, f);
+        if (T.cls == PT_TR) {
+            const bool on = (t >> T.b) & 1;
+            if (T.a == 0) F.k0 = on ? cmul(F.k0, f) : F.k0;
+            else if (T.a == 1) F.k1 = on ? cmul(F.k1, f) : F.k1;
+            else if (T.a == 2) F.k2 = on ? cmul(F.k2, f) : F.k2;
+            else F.k3 = on ? cmul(F.k3, f) : F.k3;
+        } else {
+            const bool on = T.cls == PT_T1 ? ((t >> T.a) & 1) : (((t >> T.a) & (t >> T.b)) & 1);
+            F.A = on ? cmul(F.A, f) : F.A;
         }
     }
-    if (Q.sign) {
-        // every factor is +-1: the 16-bit parity mask of the register block
-        unsigned M = (A.x < 0.0) ? 0xFFFFu : 0u;
-        if (k0.x < 0.0) M ^= reg_col_mask(0);
-        if (R > 1 && k1.x < 0.0) M ^= reg_col_mask(1);
-        if (R > 2 && k2.x < 0.0) M ^= reg_col_mask(2);
-        if (R > 3 && k3.x < 0.0) M ^= reg_col_mask(3);
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-#pragma unroll
-            for (int s2 = r + 1; s2 < R; ++s2) {
-                const int pi = r == 0 ? s2 - 1 : (r == 1 ? s2 + 1 : 5);
-                if (Q.crr[pi][0] < 0.0) M ^= reg_col_mask(r) & reg_col_mask(s2);
-            }
-#pragma unroll
-        for (int j = 0; j < NR; ++j)
-            if ((M >> j) & 1u) a[j] = -a[j];
-        return;
-    }
-#pragma unroll
-    for (int j = 0; j < NR; ++j) {
-        cd2 f = A;
-        if (j & 1) f = cmul(f, k0);
-        if (R > 1 && (j & 2)) f = cmul(f, k1);
-        if (R > 2 && (j & 4)) f = cmul(f, k2);
-        if (R > 3 && (j & 8)) f = cmul(f, k3);
-        a[j] = cmul(f, a[j]);
-    }
+    return F;
+}
+
+// DG_PHASE_SIGN: every factor is +-1, so the group is a 16-bit parity mask of the
+// register block; straight-line sign flips.
+template <int R>
+__device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], const DevPhase& Q,
+                                                 const DevTerm* __restrict__ terms,
+                                                 const cd2* __restrict__ bf, int t) {
+    const PhaseFactors F = phase_factors<R>(Q, terms, bf, t);
+    unsigned M = (F.A.x < 0.0) ? 0xFFFFu : 0u;
+    if (F.k0.x < 0.0) M ^= reg_col_mask(0);
+    if (R > 1 && F.k1.x < 0.0) M ^= reg_col_mask(1);
+    if (R > 2 && F.k2.x < 0.0) M ^= reg_col_mask(2);
+    if (R > 3 && F.k3.x < 0.0) M ^= reg_col_mask(3);
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int s2 = r + 1; s2 < R; ++s2) {
             const int pi = r == 0 ? s2 - 1 : (r == 1 ? s2 + 1 : 5);
-            const cd2 f = mk2(Q.crr[pi][0], Q.crr[pi][1]);
-            if (!is_unit(f)) {
+            if (Q.crr[pi][0] < 0.0) M ^= reg_col_mask(r) & reg_col_mask(s2);
+        }
 #pragma unroll
-                for (int j = 0; j < NR; ++j)
-                    if ((j >> r) & (j >> s2) & 1) a[j] = cmul(f, a[j]);
+    for (int j = 0; j < (1 << R); ++j) {
+        const double sg = ((M >> j) & 1u) ? -1.0 : 1.0;
+        a[j] = a[j] * sg;
+    }
+}
+
+// DG_PHASE: general unitary diagonal group.  The factor of register index j is
+// A * prod_r kappa_r^{j_r} * prod_{r<s} c_rs^{j_r j_s}; it is built top bit by top bit
+// (g[j + 2^t] = g[j] * kappa_t * prod_{r in j} c_rt) and applied once per amplitude.
+template <int R>
+__device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const DevPhase& Q,
+                                                 const DevTerm* __restrict__ terms,
+                                                 const cd2* __restrict__ bf, int t) {
+    const PhaseFactors F = phase_factors<R>(Q, terms, bf, t);
+    const cd2 c01 = mk2(Q.crr[0][0], Q.crr[0][1]), c02 = mk2(Q.crr[1][0], Q.crr[1][1]);
+    const cd2 c03 = mk2(Q.crr[2][0], Q.crr[2][1]), c12 = mk2(Q.crr[3][0], Q.crr[3][1]);
+    const cd2 c13 = mk2(Q.crr[4][0], Q.crr[4][1]), c23 = mk2(Q.crr[5][0], Q.crr[5][1]);
+    // Factors of indices 0..7 are kept for the top bit; each is applied once known.
+    const cd2 g0 = F.A, g1 = cmul(F.A, F.k0);
+    a[0] = cmul(g0, a[0]);
+    a[1] = cmul(g1, a[1]);
+    if constexpr (R > 1) {
+        const cd2 g2 = cmul(F.A, F.k1), g3 = cmul(g1, cmul(F.k1, c01));
+        a[2] = cmul(g2, a[2]);
+        a[3] = cmul(g3, a[3]);
+        if constexpr (R > 2) {
+            __builtin_amdgcn_sched_barrier(0);
+            const cd2 w1 = cmul(F.k2, c02), w2 = cmul(F.k2, c12), w3 = cmul(w1, c12);
+            const cd2 g4 = cmul(F.A, F.k2), g5 = cmul(g1, w1);
+            const cd2 g6 = cmul(g2, w2), g7 = cmul(g3, w3);
+            a[4] = cmul(g4, a[4]);
+            a[5] = cmul(g5, a[5]);
+            a[6] = cmul(g6, a[6]);
+            a[7] = cmul(g7, a[7]);
+            if constexpr (R > 3) {
+                __builtin_amdgcn_sched_barrier(0);
+                const cd2 v1 = cmul(F.k3, c03), v2 = cmul(F.k3, c13), v4 = cmul(F.k3, c23);
+                const cd2 v3 = cmul(v1, c13), v5 = cmul(v1, c23), v6 = cmul(v2, c23);
+                const cd2 v7 = cmul(v3, c23);
+                a[8] = cmul(cmul(F.A, F.k3), a[8]);
+                a[9] = cmul(cmul(g1, v1), a[9]);
+                a[10] = cmul(cmul(g2, v2), a[10]);
+                a[11] = cmul(cmul(g3, v3), a[11]);
+                __builtin_amdgcn_sched_barrier(0);
+                a[12] = cmul(cmul(g4, v4), a[12]);
+                a[13] = cmul(cmul(g5, v5), a[13]);
+                a[14] = cmul(cmul(g6, v6), a[14]);
+                a[15] = cmul(cmul(g7, v7), a[15]);
             }
         }
+    }
 }
 
 // Block-class phase factors of every phase group of one pass, one thread per
@@ -311,16 +371,7 @@ __global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
     const DevPhase* Q = nullptr;
     for (int k = 0; k < P->nphase; ++k)
         if (phases[P->phase0 + k].slot == slot) Q = &phases[P->phase0 + k];
-    uint64_t base = 0;
-    {
-        const uint64_t tmask = P->tile_mask;
-        uint64_t b = blk;
-        for (int q = 0; q < nq && b; ++q) {
-            if ((tmask >> q) & 1ull) continue;
-            base |= (b & 1ull) << q;
-            b >>= 1;
-        }
-    }
+    const uint64_t base = tile_base(P, blk);
     cd2 v[PHASE_BLOCK_VALUES];
 #pragma unroll
     for (int k = 0; k < PHASE_BLOCK_VALUES; ++k) v[k] = mk2(1.0, 0.0);
@@ -343,99 +394,79 @@ __global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
     for (int k = 0; k < PHASE_BLOCK_VALUES; ++k) out[k] = v[k];
 }
 
+// One fused pass: workgroup `tile` loads its 2^K amplitudes, runs the pass's op stream
+// on them in registers (LDS only for layout changes) and stores them back.
 template <int R>
 __global__ void __launch_bounds__(1 << (SV_TILE_BITS - SV_REG_BITS))
-sv_pass_kernel(cd2* __restrict__ state, int nq, const DevPass* __restrict__ P,
-               const DevStage* __restrict__ stages, const DevGate* __restrict__ gates,
-               const DevPhase* __restrict__ phases, const DevTerm* __restrict__ terms,
-               const cd2* __restrict__ factors) {
+sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
+               const DevGate* __restrict__ gates, const DevPhase* __restrict__ phases,
+               const DevTerm* __restrict__ terms, const cd2* __restrict__ factors,
+               uint32_t ntiles) {
     constexpr int NR = 1 << R;
     __shared__ cd2 lds[kLdsSlots];
 
     const int t = threadIdx.x;
-    const int K = P->K;
-    const int nstage = P->nstage;
-    const DevStage* st = stages + P->stage0;
+    const uint32_t tile = blockIdx.x;
+    const int nop = P->nop;
     const DevGate* gt = gates + P->gate0;
     const DevPhase* ph = phases + P->phase0;
     const DevTerm* tm = terms + P->term0;
-    const uint32_t nblocks = gridDim.x;
-    // Block base: bits of blockIdx.x deposited into the non-tile qubits, ascending.
-    uint64_t base = 0;
-    if (!P->tile_is_all) {
-        const uint64_t tmask = P->tile_mask;
-        uint64_t b = blockIdx.x;
-        for (int q = 0; q < nq && b; ++q) {
-            if ((tmask >> q) & 1ull) continue;
-            base |= (b & 1ull) << q;
-            b >>= 1;
-        }
-    }
+    const cd2* tile_factors = factors + ((size_t)P->slot0 * ntiles + tile) * PHASE_BLOCK_VALUES;
+    const uint64_t base = tile_base(P, tile);
 
-    // Load layout: register bits are the top R local positions; thread bits are the
-    // low ones, so lanes walk consecutive addresses on qubits 0..3.
-    Regs load_regs{{0, 0, 0, 0}};
-#pragma unroll
-    for (int i = 0; i < R; ++i) load_regs.v[i] = K - R + i;
+    // The planner picks an HBM load layout whose thread bits cover qubits 0..3 (16
+    // lanes walk 256 consecutive bytes) and ends the op stream in such a layout.
     Layout<R> Lcur;
-    make_layout<R>(Lcur, load_regs, P, base, t);
-    Regs cur = load_regs;
-
+    make_layout<R>(Lcur, P->load, t);
     cd2 a[NR];
 #pragma unroll
-    for (int j = 0; j < NR; ++j) a[j] = state[Lcur.glob(j)];
+    for (int j = 0; j < NR; ++j) a[j] = state[base | Lcur.glob(j)];
 
-    for (int s = 0; s < nstage; ++s) {
-        Regs want{{0, 0, 0, 0}};
-        bool same = true;
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            want.v[i] = st[s].regs[i];
-            same = same && (want.v[i] == cur.v[i]);
-        }
-        if (!same) {
+    // Every op is straight-line code on the register block.  The dispatch is unrolled
+    // twice: an op writes its results to fresh registers while its inputs are live, so
+    // two ops per trip let the results ping-pong between two register sets instead of
+    // being copied back at every loop latch.
+    auto run_op = [&](const DevGate& G) __attribute__((always_inline)) {
+        const int kind = G.kind;
+        if (kind == DG_U1X) {
+            u1_bit<R, 0>(a, G.m);
+            if constexpr (R > 1) u1_bit<R, (R > 1 ? 1 : 0)>(a, G.m + 8);
+            if constexpr (R > 2) u1_bit<R, (R > 2 ? 2 : 0)>(a, G.m + 16);
+            if constexpr (R > 3) u1_bit<R, (R > 3 ? 3 : 0)>(a, G.m + 24);
+        } else if (kind == DG_U1S) {
+            apply_u1s<R>(a, G);
+        } else if (kind == DG_PHASE_SIGN || kind == DG_PHASE) {
+            const DevPhase& Q = ph[G.qa];
+            const cd2* bf = Q.slot >= 0
+                                ? tile_factors + (size_t)Q.slot * ntiles * PHASE_BLOCK_VALUES
+                                : nullptr;
+            if (kind == DG_PHASE_SIGN)
+                apply_phase_sign<R>(a, Q, tm, bf, t);
+            else
+                apply_phase_cplx<R>(a, Q, tm, bf, t);
+        } else if (kind == DG_LAYOUT) {
 #pragma unroll
             for (int j = 0; j < NR; ++j) lds[lds_slot(Lcur.loc(j))] = a[j];
             __syncthreads();
-            make_layout<R>(Lcur, want, P, base, t);
-            cur = want;
+            make_layout<R>(Lcur, *reinterpret_cast<const DevLayout*>(G.m), t);
 #pragma unroll
             for (int j = 0; j < NR; ++j) a[j] = lds[lds_slot(Lcur.loc(j))];
             __syncthreads();
+        } else if (kind == DG_U1) {
+            apply_cu1<R>(a, G, Lcur, base);
+        } else if (kind == DG_U2) {
+            apply_u2<R>(a, G, Lcur, base);
+        } else {
+            apply_diag<R>(a, G, Lcur, base);
         }
-        const int g1 = st[s].g1;
-        for (int gi = st[s].g0; gi < g1; ++gi) {
-            const DevGate& G = gt[gi];
-            switch (G.kind) {
-                case DG_U1: apply_u1<R>(a, G, Lcur); break;
-                case DG_U2: apply_u2<R>(a, G, Lcur); break;
-                case DG_PHASE: {
-                    const DevPhase& Q = ph[G.qa];
-                    const cd2* bf =
-                        Q.slot >= 0 ? factors + ((size_t)(P->slot0 + Q.slot) * nblocks + blockIdx.x) *
-                                                    PHASE_BLOCK_VALUES
-                                    : nullptr;
-                    apply_phase<R>(a, Q, tm, bf, t);
-                    break;
-                }
-                default: apply_diag<R>(a, G, Lcur); break;
-            }
-        }
+    };
+    for (int gi = 0; gi < nop; gi += 2) {
+        run_op(gt[gi]);
+        if (gi + 1 < nop) run_op(gt[gi + 1]);
     }
 
-    bool back = false;
 #pragma unroll
-    for (int i = 0; i < R; ++i) back = back || (cur.v[i] != load_regs.v[i]);
-    if (back) {
-#pragma unroll
-        for (int j = 0; j < NR; ++j) lds[lds_slot(Lcur.loc(j))] = a[j];
-        __syncthreads();
-        make_layout<R>(Lcur, load_regs, P, base, t);
-#pragma unroll
-        for (int j = 0; j < NR; ++j) a[j] = lds[lds_slot(Lcur.loc(j))];
-    }
-#pragma unroll
-    for (int j = 0; j < NR; ++j) state[Lcur.glob(j)] = a[j];
+    for (int j = 0; j < NR; ++j) state[base | Lcur.glob(j)] = a[j];
 }
 
 __global__ void sv_fill_zero_kernel(cd2* __restrict__ state, uint64_t n) {
@@ -501,31 +532,32 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
     const char* blob = reinterpret_cast<const char*>(dev_blob);
     const SvBlobLayout L = sv_blob_layout(plan, nq);
     const DevPass* dpasses = reinterpret_cast<const DevPass*>(blob + L.passes);
-    const DevStage* dstages = reinterpret_cast<const DevStage*>(blob + L.stages);
     const DevGate* dgates = reinterpret_cast<const DevGate*>(blob + L.gates);
     const DevPhase* dphases = reinterpret_cast<const DevPhase*>(blob + L.phases);
     const DevTerm* dterms = reinterpret_cast<const DevTerm*>(blob + L.terms);
-    double2* dfactors = reinterpret_cast<double2*>(const_cast<char*>(blob) + L.factors);
+    cd2* dfactors = reinterpret_cast<cd2*>(const_cast<char*>(blob) + L.factors);
+    cd2* st = reinterpret_cast<cd2*>(state);
     for (size_t p = 0; p < plan.passes.size(); ++p) {
         const DevPass& P = plan.passes[p];
         const DevPass* dP = dpasses + p;
-        const unsigned blocks = P.tile_is_all ? 1u : (1u << (nq - P.K));
+        const unsigned tiles = P.tile_is_all ? 1u : (1u << (nq - P.K));
         const unsigned threads = 1u << (P.K - P.R);
+        const unsigned grid = tiles;
         if (P.nslot > 0) {
-            const uint64_t work = (uint64_t)P.nslot * blocks;
+            const uint64_t work = (uint64_t)P.nslot * tiles;
             hipLaunchKernelGGL(sv_phase_block_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256),
-                               0, stream, nq, dP, dphases, dterms, reinterpret_cast<cd2*>(dfactors), blocks);
+                               0, stream, nq, dP, dphases, dterms, dfactors, tiles);
             QS_HIP(hipGetLastError());
         }
         switch (P.R) {
-            case 1: hipLaunchKernelGGL(sv_pass_kernel<1>, dim3(blocks), dim3(threads), 0, stream,
-                                       reinterpret_cast<cd2*>(state), nq, dP, dstages, dgates, dphases, dterms, reinterpret_cast<cd2*>(dfactors)); break;
-            case 2: hipLaunchKernelGGL(sv_pass_kernel<2>, dim3(blocks), dim3(threads), 0, stream,
-                                       reinterpret_cast<cd2*>(state), nq, dP, dstages, dgates, dphases, dterms, reinterpret_cast<cd2*>(dfactors)); break;
-            case 3: hipLaunchKernelGGL(sv_pass_kernel<3>, dim3(blocks), dim3(threads), 0, stream,
-                                       reinterpret_cast<cd2*>(state), nq, dP, dstages, dgates, dphases, dterms, reinterpret_cast<cd2*>(dfactors)); break;
-            default: hipLaunchKernelGGL(sv_pass_kernel<4>, dim3(blocks), dim3(threads), 0, stream,
-                                        reinterpret_cast<cd2*>(state), nq, dP, dstages, dgates, dphases, dterms, reinterpret_cast<cd2*>(dfactors)); break;
+            case 1: hipLaunchKernelGGL(sv_pass_kernel<1>, dim3(grid), dim3(threads), 0, stream,
+                                       st, dP, dgates, dphases, dterms, dfactors, tiles); break;
+            case 2: hipLaunchKernelGGL(sv_pass_kernel<2>, dim3(grid), dim3(threads), 0, stream,
+                                       st, dP, dgates, dphases, dterms, dfactors, tiles); break;
+            case 3: hipLaunchKernelGGL(sv_pass_kernel<3>, dim3(grid), dim3(threads), 0, stream,
+                                       st, dP, dgates, dphases, dterms, dfactors, tiles); break;
+            default: hipLaunchKernelGGL(sv_pass_kernel<4>, dim3(grid), dim3(threads), 0, stream,
+                                        st, dP, dgates, dphases, dterms, dfactors, tiles); break;
         }
         QS_HIP(hipGetLastError());
     }

@@ -220,136 +220,383 @@ void emit_phase_group(const std::vector<const HostGate*>& diag, const int* regs,
     plan->terms.insert(plan->terms.end(), bt.begin(), bt.end());
     ph.slot = bt.empty() ? -1 : P.nslot++;
     DevGate d{};
-    d.kind = DG_PHASE;
+    d.kind = sign ? DG_PHASE_SIGN : DG_PHASE;
     d.qa = (int)plan->phases.size() - P.phase0;
     d.qb = -1;
     plan->phases.push_back(ph);
     plan->gates.push_back(d);
 }
 
+// ------------------------------------------------------------------ stage scheduler
+//
+// Within a pass every gate's targets are tile qubits, so the only constraint left is
+// dependence order: gate j follows an earlier gate i when they share a qubit and are
+// not both diagonal.  A stage fixes which R tile positions are register bits; the
+// scheduler picks, stage by stage, the register set under which the most gates can
+// run (following chains: a gate becomes ready once its predecessors ran), applies them
+// in dependence order, and folds ready diagonal gates into one phase group just before
+// the first gate that needs them.  Uncontrolled one-qubit gates that run back to back
+// are merged into a DG_U1X record (one 2x2 matrix per register bit).
+struct StageScheduler {
+    const std::vector<const Item*>& inc;
+    const DevPass& P;
+    int K, R, L;
+    int n;
+    std::vector<int> need_mask;               // local-position mask of the targets
+    std::vector<std::vector<int>> preds, succs;
+
+    StageScheduler(const std::vector<const Item*>& items, const DevPass& pass)
+        : inc(items), P(pass), K(pass.K), R(pass.R), L(std::min(SV_LOW_BITS, pass.K)) {
+        n = (int)inc.size();
+        need_mask.assign(n, 0);
+        preds.assign(n, {});
+        succs.assign(n, {});
+        // Sparse dependences: per qubit, the last non-diagonal gate and the diagonal
+        // gates seen since it.
+        std::vector<int> last_nd(64, -1);
+        std::vector<std::vector<int>> diag_since(64);
+        for (int j = 0; j < n; ++j) {
+            const Item& it = *inc[j];
+            if (!it.diag) {
+                need_mask[j] = 1 << local_pos(P.tq, K, it.g.q0);
+                if (it.g.kind == QS_GATE_U2) need_mask[j] |= 1 << local_pos(P.tq, K, it.g.q1);
+            }
+            std::vector<int> pr;
+            for (int q = 0; q < 64; ++q) {
+                if (!((it.touch >> q) & 1ull)) continue;
+                if (last_nd[q] >= 0) pr.push_back(last_nd[q]);
+                if (!it.diag)
+                    for (int d : diag_since[q]) pr.push_back(d);
+            }
+            std::sort(pr.begin(), pr.end());
+            pr.erase(std::unique(pr.begin(), pr.end()), pr.end());
+            preds[j] = pr;
+            for (int p : pr) succs[p].push_back(j);
+            for (int q = 0; q < 64; ++q) {
+                if (!((it.touch >> q) & 1ull)) continue;
+                if (it.diag) {
+                    diag_since[q].push_back(j);
+                } else {
+                    last_nd[q] = j;
+                    diag_since[q].clear();
+                }
+            }
+        }
+    }
+
+    struct State {
+        std::vector<char> done;
+        std::vector<int> rem;                 // unfinished predecessors
+        int left_nd = 0;                      // non-diagonal gates not yet run
+    };
+
+    State initial() const {
+        State s;
+        s.done.assign(n, 0);
+        s.rem.assign(n, 0);
+        for (int j = 0; j < n; ++j) {
+            s.rem[j] = (int)preds[j].size();
+            if (!inc[j]->diag) ++s.left_nd;
+        }
+        return s;
+    }
+
+    void finish(State& s, int j) const {
+        s.done[j] = 1;
+        for (int k : succs[j]) --s.rem[k];
+        if (!inc[j]->diag) --s.left_nd;
+    }
+
+    // Non-diagonal gate j can run now (its diagonal predecessors may still be pending
+    // as long as they are themselves ready).
+    bool runnable(const State& s, int j, bool* need_flush) const {
+        bool flush = false;
+        for (int p : preds[j]) {
+            if (s.done[p]) continue;
+            if (inc[p]->diag && s.rem[p] == 0) {
+                flush = true;
+                continue;
+            }
+            return false;
+        }
+        if (need_flush) *need_flush = flush;
+        return true;
+    }
+
+    // Marks every ready diagonal gate done; returns their indices.
+    std::vector<int> flush_ready(State& s) const {
+        std::vector<int> out;
+        for (int j = 0; j < n; ++j)
+            if (!s.done[j] && inc[j]->diag && s.rem[j] == 0) out.push_back(j);
+        for (int j : out) finish(s, j);
+        return out;
+    }
+
+    // Runs every gate whose targets lie in `mask` until none is left; returns the
+    // weighted count (two-qubit gates count twice).  `emit` receives the gate order.
+    // Gates that need no pending diagonal run first, so one-qubit gates on distinct
+    // register bits gather into one op and every flush serves as many gates as possible.
+    template <class Emit>
+    int run_stage(State& s, int mask, Emit&& emit) const {
+        int count = 0;
+        for (;;) {
+            bool progress = false, blocked = false;
+            for (int j = 0; j < n; ++j) {
+                if (s.done[j] || inc[j]->diag || (need_mask[j] & ~mask)) continue;
+                bool flush = false;
+                if (!runnable(s, j, &flush)) continue;
+                if (flush) {
+                    blocked = true;
+                    continue;
+                }
+                finish(s, j);
+                emit(std::vector<int>(), j);
+                count += (inc[j]->g.kind == QS_GATE_U2) ? 2 : 1;
+                progress = true;
+            }
+            if (progress) continue;
+            if (!blocked) break;
+            emit(flush_ready(s), -1);
+        }
+        return count;
+    }
+
+    int score(const State& s, int mask) const {
+        State t = s;
+        const int full = run_stage(t, mask, [](const std::vector<int>&, int) {});
+        // ready gates half inside the set make a partial set worth extending
+        int partial = 0;
+        for (int j = 0; j < n; ++j) {
+            if (t.done[j] || inc[j]->diag) continue;
+            if ((need_mask[j] & mask) && (need_mask[j] & ~mask) && runnable(t, j, nullptr))
+                ++partial;
+        }
+        return 4 * full + partial;
+    }
+
+    // Register set of the next stage.  prefer_low: break ties toward the always-in-tile
+    // low positions (keeps the high ones for the last stage, whose layout can then be
+    // stored without a transpose).
+    int choose(const State& s, bool prefer_low) const {
+        int mask = 0;
+        for (int slot = 0; slot < R; ++slot) {
+            int best = -1, best_score = -1, best_pref = -1;
+            for (int p = 0; p < K; ++p) {
+                if ((mask >> p) & 1) continue;
+                const int sc = score(s, mask | (1 << p));
+                const bool low = p < L;
+                const int pref = (prefer_low == low ? 2 * K : 0) + (low ? K - p : p);
+                if (sc > best_score || (sc == best_score && pref > best_pref)) {
+                    best = p;
+                    best_score = sc;
+                    best_pref = pref;
+                }
+            }
+            mask |= 1 << best;
+        }
+        // Guarantee progress: the set must hold the targets of some runnable gate.
+        State t = s;
+        if (run_stage(t, mask, [](const std::vector<int>&, int) {}) == 0) {
+            for (int j = 0; j < n; ++j) {
+                if (s.done[j] || inc[j]->diag || !runnable(s, j, nullptr)) continue;
+                int m2 = need_mask[j];
+                for (int p = K - 1; p >= 0 && popcount64((uint64_t)m2) < R; --p)
+                    if ((mask >> p) & 1) m2 |= 1 << p;
+                for (int p = K - 1; p >= 0 && popcount64((uint64_t)m2) < R; --p) m2 |= 1 << p;
+                return m2;
+            }
+        }
+        return mask;
+    }
+
+    // Stage register masks of a whole schedule under one tie-break policy.
+    std::vector<int> schedule(bool low_after_first) const {
+        State s = initial();
+        std::vector<int> masks;
+        while (s.left_nd > 0) {
+            const int m = choose(s, low_after_first && !masks.empty());
+            run_stage(s, m, [](const std::vector<int>&, int) {});
+            masks.push_back(m);
+        }
+        return masks;
+    }
+
+    int cost(const std::vector<int>& masks) const {
+        if (masks.empty()) return 0;
+        const int lowmask = (1 << L) - 1;
+        int c = (int)masks.size() - 1;
+        if (masks.front() & lowmask) ++c;
+        if (masks.back() & lowmask) ++c;
+        return c;
+    }
+};
+
 void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan) {
     const int K = P.K, R = P.R;
-    int load[4] = {0, 0, 0, 0};
-    for (int i = 0; i < R; ++i) load[i] = K - R + i;
-    int cur[4] = {load[0], load[1], load[2], load[3]};
     P.stage0 = (int)plan->stages.size();
     P.gate0 = (int)plan->gates.size();
     P.phase0 = (int)plan->phases.size();
     P.term0 = (int)plan->terms.size();
     P.nslot = 0;
-    int nstage = 0;
-    bool open = false;
-    DevStage st{};
-    std::vector<const HostGate*> diag;   // pending unitary diagonal gates of the stage
 
-    auto regmask_of = [&](const int* r) {
+    StageScheduler S(inc, P);
+    std::vector<int> masks = S.schedule(false);
+    {
+        std::vector<int> alt = S.schedule(true);
+        if (S.cost(alt) < S.cost(masks)) masks.swap(alt);
+    }
+    if (masks.empty()) {   // only diagonal gates: one stage in the load layout
         int m = 0;
-        for (int i = 0; i < R; ++i) m |= 1 << r[i];
-        return m;
-    };
-    auto flush_diag = [&]() {
-        if (!diag.empty()) emit_phase_group(diag, cur, P, plan);
-        diag.clear();
-    };
-    auto open_stage = [&](const int* regs) {
-        flush_diag();
-        if (open) {
-            st.g1 = (int)plan->gates.size() - P.gate0;
-            plan->stages.push_back(st);
-            ++nstage;
+        for (int i = 0; i < R; ++i) m |= 1 << (K - R + i);
+        masks.push_back(m);
+    }
+
+    StageScheduler::State st = S.initial();
+    int cur[4] = {0, 0, 0, 0};
+    // Uncontrolled one-qubit gates gathered per register bit until another op is
+    // emitted; three or four bits become one DG_U1X, fewer become DG_U1S ops.
+    int u1_mask = 0;
+    double u1_m[4][8];
+    auto close_u1 = [&]() {
+        if (!u1_mask) return;
+        if (__builtin_popcount(u1_mask) >= 3 && R == 4) {
+            DevGate d{};
+            d.kind = DG_U1X;
+            d.r0 = u1_mask;
+            d.qa = d.qb = -1;
+            for (int b = 0; b < 4; ++b) {
+                if ((u1_mask >> b) & 1) {
+                    std::memcpy(d.m + 8 * b, u1_m[b], sizeof(double) * 8);
+                } else {
+                    d.m[8 * b] = d.m[8 * b + 6] = 1.0;   // identity
+                }
+            }
+            plan->gates.push_back(d);
+        } else {
+            for (int b = 0; b < 4; ++b) {
+                if (!((u1_mask >> b) & 1)) continue;
+                DevGate d{};
+                d.kind = DG_U1S;
+                d.r0 = b;
+                d.qa = d.qb = -1;
+                std::memcpy(d.m, u1_m[b], sizeof(double) * 8);
+                plan->gates.push_back(d);
+            }
         }
-        std::memset(&st, 0, sizeof(st));
-        for (int i = 0; i < R; ++i) st.regs[i] = regs[i];
-        st.g0 = (int)plan->gates.size() - P.gate0;
-        open = true;
+        u1_mask = 0;
     };
     auto reg_index = [&](int lp) {
         for (int i = 0; i < R; ++i)
             if (cur[i] == lp) return i;
         return -1;
     };
-
-    for (size_t n = 0; n < inc.size(); ++n) {
-        const Item& it = *inc[n];
-        const HostGate& g = it.g;
-        if (it.diag) {
-            if (!open) open_stage(cur);
+    auto emit_diagonals = [&](const std::vector<int>& idx) {
+        if (idx.empty()) return;
+        close_u1();
+        std::vector<const HostGate*> group;
+        std::vector<const HostGate*> general;
+        for (int j : idx) {
             std::vector<PTerm> probe;
-            if (phase_terms(g, probe)) {
-                diag.push_back(&it.g);
-                continue;
-            }
-            // General diagonal (non-unit entries or several controls): per amplitude.
-            flush_diag();
+            if (phase_terms(inc[j]->g, probe))
+                group.push_back(&inc[j]->g);
+            else
+                general.push_back(&inc[j]->g);
+        }
+        if (!group.empty()) emit_phase_group(group, cur, P, plan);
+        for (const HostGate* g : general) {   // non-unit entries or several controls
             DevGate d{};
-            d.cmask = g.cmask;
-            d.qa = g.q0;
-            d.qb = (g.kind == QS_GATE_DIAG2) ? g.q1 : -1;
-            d.kind = (g.kind == QS_GATE_DIAG1) ? DG_DIAG1 : DG_DIAG2;
-            std::memcpy(d.m, g.m, sizeof(double) * (g.kind == QS_GATE_DIAG1 ? 4 : 8));
+            d.cmask = g->cmask;
+            d.qa = g->q0;
+            d.qb = (g->kind == QS_GATE_DIAG2) ? g->q1 : -1;
+            d.kind = (g->kind == QS_GATE_DIAG1) ? DG_DIAG1 : DG_DIAG2;
+            std::memcpy(d.m, g->m, sizeof(double) * (g->kind == QS_GATE_DIAG1 ? 4 : 8));
             plan->gates.push_back(d);
-            continue;
         }
-        int need[2];
-        int nneed = 0;
-        need[nneed++] = local_pos(P.tq, K, g.q0);
-        if (g.kind == QS_GATE_U2) need[nneed++] = local_pos(P.tq, K, g.q1);
-        const int curmask = regmask_of(cur);
-        int needmask = 0;
-        for (int i = 0; i < nneed; ++i) needmask |= 1 << need[i];
-        if (!open || (needmask & ~curmask)) {
-            int nm = needmask;
-            if (needmask & ~curmask) {
-                // New register layout: this gate's targets, then targets of the next
-                // non-diagonal gates that still fit, then previous registers.
-                for (size_t k = n + 1; k < inc.size() && popcount64(nm) < R; ++k) {
-                    const Item& nx = *inc[k];
-                    if (nx.diag) continue;
-                    int m2 = 1 << local_pos(P.tq, K, nx.g.q0);
-                    if (nx.g.kind == QS_GATE_U2) m2 |= 1 << local_pos(P.tq, K, nx.g.q1);
-                    if (popcount64((uint64_t)(nm | m2)) <= R) nm |= m2;
-                }
-                for (int i = 0; i < R && popcount64(nm) < R; ++i) nm |= 1 << cur[i];
-                for (int p = K - 1; p >= 0 && popcount64(nm) < R; --p) nm |= 1 << p;
-                // the pending diagonal group runs under the old layout
-                flush_diag();
-                int k = 0;
-                for (int p = 0; p < K; ++p)
-                    if ((nm >> p) & 1) cur[k++] = p;
-            }
-            open_stage(cur);
+    };
+    auto emit_gate = [&](int j) {
+        const HostGate& g = inc[j]->g;
+        const int r0 = reg_index(local_pos(P.tq, K, g.q0));
+        if (g.kind == QS_GATE_U1 && g.cmask == 0) {
+            if ((u1_mask >> r0) & 1)
+                mat2_mul(g.m, u1_m[r0], u1_m[r0]);
+            else
+                std::memcpy(u1_m[r0], g.m, sizeof(double) * 8);
+            u1_mask |= 1 << r0;
+            return;
         }
-        flush_diag();
+        close_u1();
         DevGate d{};
         d.cmask = g.cmask;
         d.qa = d.qb = -1;
+        d.r0 = r0;
         if (g.kind == QS_GATE_U1) {
             d.kind = DG_U1;
-            d.r0 = reg_index(need[0]);
-            // Merge with a directly preceding U1 on the same register bit and controls.
-            if ((int)plan->gates.size() > P.gate0 + st.g0) {
-                DevGate& prev = plan->gates.back();
-                if (prev.kind == DG_U1 && prev.r0 == d.r0 && prev.cmask == d.cmask) {
-                    mat2_mul(g.m, prev.m, prev.m);
-                    continue;
-                }
-            }
             std::memcpy(d.m, g.m, sizeof(double) * 8);
         } else {
             d.kind = DG_U2;
-            d.r0 = reg_index(need[0]);
-            d.r1 = reg_index(need[1]);
+            d.r1 = reg_index(local_pos(P.tq, K, g.q1));
             std::memcpy(d.m, g.m, sizeof(double) * 32);
         }
         plan->gates.push_back(d);
+    };
+
+    // HBM layouts: a layout whose register bits avoid the low positions keeps qubits
+    // 0..L-1 on the lane index, so the first stage is loaded directly when it allows;
+    // otherwise loads and stores use the default layout (the top R positions).
+    const int L = std::min(SV_LOW_BITS, K);
+    const int lowmask = (1 << L) - 1;
+    int defmask = 0;
+    for (int i = 0; i < R; ++i) defmask |= 1 << (K - R + i);
+    const bool direct = (masks.front() & lowmask) == 0;
+    const int loadmask = direct ? masks.front() : defmask;
+    {
+        int r[4] = {0, 0, 0, 0}, k = 0;
+        for (int p = 0; p < K; ++p)
+            if ((loadmask >> p) & 1) r[k++] = p;
+        sv_fill_layout(P, r, &P.load);
     }
-    flush_diag();
-    if (open) {
-        st.g1 = (int)plan->gates.size() - P.gate0;
-        plan->stages.push_back(st);
-        ++nstage;
+    auto emit_layout = [&](int mask) {
+        close_u1();
+        DevGate d{};
+        d.kind = DG_LAYOUT;
+        int r[4] = {0, 0, 0, 0}, k = 0;
+        for (int p = 0; p < K; ++p)
+            if ((mask >> p) & 1) r[k++] = p;
+        d.r0 = r[0];
+        d.r1 = r[1];
+        d.qa = r[2];
+        d.qb = r[3];
+        DevLayout lay;
+        sv_fill_layout(P, r, &lay);
+        std::memcpy(d.m, &lay, sizeof(lay));
+        plan->gates.push_back(d);
+    };
+    int curmask = loadmask;
+    for (size_t si = 0; si < masks.size(); ++si) {
+        int k = 0;
+        for (int p = 0; p < K; ++p)
+            if ((masks[si] >> p) & 1) cur[k++] = p;
+        DevStage stg{};
+        for (int i = 0; i < R; ++i) stg.regs[i] = cur[i];
+        stg.g0 = (int)plan->gates.size() - P.gate0;
+        if (masks[si] != curmask) emit_layout(masks[si]);
+        curmask = masks[si];
+        S.run_stage(st, masks[si], [&](const std::vector<int>& diags, int j) {
+            if (j < 0)
+                emit_diagonals(diags);
+            else
+                emit_gate(j);
+        });
+        close_u1();
+        if (si + 1 == masks.size()) {
+            emit_diagonals(S.flush_ready(st));
+            if ((curmask & lowmask) && curmask != defmask) emit_layout(defmask);
+        }
+        stg.g1 = (int)plan->gates.size() - P.gate0;
+        plan->stages.push_back(stg);
     }
-    P.nstage = nstage;
+    P.nstage = (int)masks.size();
+    P.nop = (int)plan->gates.size() - P.gate0;
     P.nphase = (int)plan->phases.size() - P.phase0;
     P.slot0 = plan->nslot_total;
     plan->nslot_total += P.nslot;
@@ -429,6 +676,10 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
         int k = 0;
         for (int q = 0; q < nq; ++q)
             if ((tile >> q) & 1) P.tq[k++] = q;
+        P.nbq = 0;
+        if (!P.tile_is_all)
+            for (int q = 0; q < nq; ++q)
+                if (!((tile >> q) & 1)) P.bq[P.nbq++] = q;
         build_stages(inc, P, plan);
         plan->passes.push_back(P);
         for (size_t i : inc_idx) done[i] = 1;
@@ -444,6 +695,23 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
 }  // namespace qs
 
 namespace qs {
+
+void sv_fill_layout(const DevPass& P, const int* regs, DevLayout* out) {
+    std::memset(out, 0, sizeof(*out));
+    int regmask = 0;
+    for (int i = 0; i < P.R; ++i) {
+        out->regs[i] = regs[i];
+        out->rq[i] = P.tq[regs[i]];
+        regmask |= 1 << regs[i];
+    }
+    int b = 0;
+    for (int p = 0; p < P.K; ++p) {
+        if ((regmask >> p) & 1) continue;
+        out->tpos[b] = p;
+        out->tq[b] = P.tq[p];
+        ++b;
+    }
+}
 
 SvBlobLayout sv_blob_layout(const SvPlan& plan, int nq) {
     SvBlobLayout L{};

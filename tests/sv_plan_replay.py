@@ -8,7 +8,7 @@ target outside the tile) shows up as a mismatch against the oracle.
 
 import numpy as np
 
-DG_U1, DG_U2, DG_DIAG1, DG_DIAG2, DG_PHASE = 0, 1, 2, 3, 5
+DG_U1, DG_U2, DG_DIAG1, DG_DIAG2, DG_U1X, DG_PHASE, DG_LAYOUT, DG_PHASE_SIGN, DG_U1S = 0, 1, 2, 3, 4, 5, 6, 7, 8
 PT_BB, PT_BT, PT_BR, PT_T1, PT_TT, PT_TR = 0, 1, 2, 3, 4, 5
 PAIRS = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
 
@@ -22,30 +22,56 @@ def _ctrl_ok(idx, cmask):
     return (idx & cmask) == cmask
 
 
+def _apply_1q(st, idx, sel, t, m):
+    lo = sel & (((idx >> t) & 1) == 0)
+    i0 = idx[lo]
+    i1 = i0 | (1 << t)
+    a0, a1 = st[i0].copy(), st[i1].copy()
+    st[i0] = m[0, 0] * a0 + m[0, 1] * a1
+    st[i1] = m[1, 0] * a0 + m[1, 1] * a1
+
+
 def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
     st = state.copy()
     idx = np.arange(st.size, dtype=np.int64)
     for p in plan['passes']:
         tq = p['tq']
+        R = p['R']
         assert len(tq) == p['K'] and sorted(tq) == tq
         tile = 0
         for q in tq:
             tile |= 1 << q
-        for s in p['stages']:
+        low = min(4, p['K'])
+        layout = list(p['load_regs'][:R])
+        # HBM layouts keep the low positions on the lane index (or are the default)
+        default = list(range(p['K'] - R, p['K']))
+        assert all(r >= low for r in layout) or layout == default
+        for si, s in enumerate(p['stages']):
             regs = s['regs']
-            assert len(regs) == p['R'] and sorted(regs) == regs
-            for g in s['gates']:
+            assert len(regs) == R and sorted(regs) == regs
+            for gi, g in enumerate(s['gates']):
+                if g['kind'] == DG_LAYOUT:
+                    layout = [g['r0'], g['r1'], g['qa'], g['qb']][:R]
+                    assert sorted(layout) == layout
+                    continue
+                # every other op runs under the stage's register layout
+                assert layout == regs, (si, gi, layout, regs)
                 cm = int(g['cmask'])
                 sel = _ctrl_ok(idx, cm)
                 if g['kind'] == DG_U1:
-                    t = tq[regs[g['r0']]]
-                    m = _cm(g['m'], 4).reshape(2, 2)
-                    lo = sel & (((idx >> t) & 1) == 0)
-                    i0 = idx[lo]
-                    i1 = i0 | (1 << t)
-                    a0, a1 = st[i0].copy(), st[i1].copy()
-                    st[i0] = m[0, 0] * a0 + m[0, 1] * a1
-                    st[i1] = m[1, 0] * a0 + m[1, 1] * a1
+                    _apply_1q(st, idx, sel, tq[regs[g['r0']]], _cm(g['m'], 4).reshape(2, 2))
+                elif g['kind'] == DG_U1X:
+                    # the kernel applies all four matrices; bits outside r0 hold identity
+                    assert cm == 0 and R == 4 and bin(g['r0']).count('1') >= 3
+                    allm = _cm(g['m'], 16)
+                    for b in range(len(regs)):
+                        mb = allm[4 * b:4 * b + 4].reshape(2, 2)
+                        if not (g['r0'] >> b) & 1:
+                            assert np.array_equal(mb, np.eye(2))
+                        _apply_1q(st, idx, sel, tq[regs[b]], mb)
+                elif g['kind'] == DG_U1S:
+                    assert cm == 0 and 0 <= g['r0'] < R
+                    _apply_1q(st, idx, sel, tq[regs[g['r0']]], _cm(g['m'], 4).reshape(2, 2))
                 elif g['kind'] == DG_U2:
                     ta = tq[regs[g['r0']]]
                     tb = tq[regs[g['r1']]]
@@ -56,7 +82,8 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                     out = m @ v
                     for k in range(4):
                         st[ii[k]] = out[k]
-                elif g['kind'] == DG_PHASE:
+                elif g['kind'] in (DG_PHASE, DG_PHASE_SIGN):
+                    assert (g['kind'] == DG_PHASE_SIGN) == bool(g['phase']['sign'])
                     st *= phase_factor(g['phase'], tq, regs, idx)
                 elif g['kind'] == DG_DIAG1:
                     d = _cm(g['m'], 2)
@@ -69,8 +96,11 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                 else:
                     raise AssertionError(f"unknown kind {g['kind']}")
                 # every non-diagonal target must be a tile qubit
-                if g['kind'] in (DG_U1, DG_U2):
+                if g['kind'] in (DG_U1, DG_U2, DG_U1S):
                     assert (tile >> tq[regs[g['r0']]]) & 1
+                if g['kind'] == DG_U1:
+                    assert cm != 0, 'uncontrolled one-qubit gates travel as U1X'
+        assert all(r >= low for r in layout) or layout == default, layout
     return st
 
 
