@@ -107,6 +107,8 @@ int run_plan(double2* state, int nq, const SvPlan& plan, hipStream_t stream) {
     put(L.gates, plan.gates.data(), plan.gates.size() * sizeof(DevGate));
     put(L.phases, plan.phases.data(), plan.phases.size() * sizeof(DevPhase));
     put(L.terms, plan.terms.data(), plan.terms.size() * sizeof(DevTerm));
+    put(L.tfactors, plan.thread_factors.data(), plan.thread_factors.size() * sizeof(double));
+    put(L.tmasks, plan.thread_masks.data(), plan.thread_masks.size() * sizeof(uint32_t));
     void* dev = nullptr;
     QS_HIP(hipMallocAsync(&dev, L.total, stream));
     // Pageable source: the runtime stages the bytes before returning.
@@ -275,6 +277,17 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
                     os << "],\"crr\":[";
                     for (int k = 0; k < 6; ++k)
                         os << (k ? "," : "") << Q.crr[k][0] << "," << Q.crr[k][1];
+                    os << "],\"table\":[";
+                    const int nthr = 1 << (P.K - P.R);
+                    if (Q.sign) {
+                        for (int t = 0; t < nthr; ++t)
+                            os << (t ? "," : "") << plan.thread_masks[Q.tab + t];
+                    } else {
+                        const size_t n0 = (size_t)Q.tab * 2 * PHASE_THREAD_VALUES;
+                        const size_t n1 = n0 + (size_t)nthr * 2 * PHASE_THREAD_VALUES;
+                        for (size_t k = n0; k < n1; ++k)
+                            os << (k > n0 ? "," : "") << plan.thread_factors[k];
+                    }
                     os << "],\"terms\":[";
                     const int n = Q.nt + Q.nb;
                     for (int k = 0; k < n; ++k) {

@@ -105,18 +105,24 @@ static_assert(sizeof(DevLayout) <= sizeof(double) * 32, "DevLayout must fit in D
 //     PT_TR: a register bit, b thread bit        -> kappa_a
 enum : int32_t { PT_BB = 0, PT_BT = 1, PT_BR = 2, PT_T1 = 3, PT_TT = 4, PT_TR = 5 };
 constexpr int PHASE_BLOCK_VALUES = 13;
+constexpr int PHASE_THREAD_VALUES = 5;
 
 struct alignas(16) DevTerm {
     int32_t cls, a, b, pad;
     double re, im;
 };
 
+// The thread- and register-class part of a group is folded on the host into a table
+// with one entry per thread: for SIGN groups a 16-bit mask of the register indices to
+// negate (constant, thread, register and register-pair factors together), otherwise
+// the complex factors (A_t, kappa_0..3 of thread t), PHASE_THREAD_VALUES per thread.
 struct alignas(16) DevPhase {
     int32_t t0, nt;        // thread-class terms in the pass's term table
     int32_t b0, nb;        // block-class terms in the pass's term table
     int32_t slot;          // block-factor slot of this group in the pass (-1: none)
     int32_t sign;          // 1: every factor is +1 or -1 (parity arithmetic)
-    int32_t pad[2];
+    int32_t tab;           // first entry of this group's per-thread table
+    int32_t pad;
     double c[2];           // constant factor
     double kr[4][2];       // constant per-register-bit factors
     double crr[6][2];      // register-pair factors: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
@@ -161,6 +167,8 @@ struct SvPlan {
     std::vector<DevGate> gates;
     std::vector<DevPhase> phases;
     std::vector<DevTerm> terms;
+    std::vector<double> thread_factors;   // complex groups: 2 * PHASE_THREAD_VALUES per thread
+    std::vector<uint32_t> thread_masks;   // sign groups: one mask per thread
     int nslot_total = 0;       // block-factor slots over all passes
 };
 
@@ -170,7 +178,7 @@ struct SvPlan {
 inline size_t sv_blob_offset(size_t bytes) { return ((bytes + 255) / 256) * 256; }
 
 struct SvBlobLayout {
-    size_t passes, stages, gates, phases, terms, factors, total;
+    size_t passes, stages, gates, phases, terms, tfactors, tmasks, factors, total;
 };
 SvBlobLayout sv_blob_layout(const SvPlan& plan, int nq);
 

@@ -237,72 +237,28 @@ __device__ __forceinline__ constexpr unsigned reg_col_mask(int r) {
     return r == 0 ? 0xAAAAu : r == 1 ? 0xCCCCu : r == 2 ? 0xF0F0u : 0xFF00u;
 }
 
-// Per-thread factors of a unitary diagonal group (DevPhase): the constant A of this
-// thread (constant x block factor x thread-bit terms) and kappa_r of each register
-// bit r.  Block-class factors come from the prologue's table.
-struct PhaseFactors {
-    cd2 A, k0, k1, k2, k3;     // scalars: an indexed array would live in scratch
-};
+// A unitary diagonal group (DevPhase).  The host folds the constant, thread-class,
+// register and register-pair factors into one table entry per thread; block-class
+// factors come from the prologue's per-tile table bf = [A_B, lambda_0..7, kappa_0..3]
+// (lambda_q applies when thread bit q is set, kappa_r to register bit r).
 
+// DG_PHASE_SIGN: every factor is +-1; the group is a 16-bit mask of the register
+// indices to negate.
+// `M` is this thread's table entry.
 template <int R>
-__device__ __forceinline__ PhaseFactors phase_factors(const DevPhase& Q,
-                                                      const DevTerm* __restrict__ terms,
-                                                      const cd2* __restrict__ bf, int t) {
-    PhaseFactors F;
-    F.A = mk2(Q.c[0], Q.c[1]);
-    F.k0 = mk2(Q.kr[0][0], Q.kr[0][1]);
-    F.k1 = mk2(Q.kr[1][0], Q.kr[1][1]);
-    F.k2 = mk2(Q.kr[2][0], Q.kr[2][1]);
-    F.k3 = mk2(Q.kr[3][0], Q.kr[3][1]);
-    // Control flow depends only on uniform data; thread-dependent factors are selected.
-    if (bf) {
-        F.A = cmul(F.A, bf[0]);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const cd2 Af = cmul(F.A, bf[1 + q]);
-            F.A = ((t >> q) & 1) ? Af : F.A;
-        }
-        F.k0 = cmul(F.k0, bf[9]);
-        if (R > 1) F.k1 = cmul(F.k1, bf[10]);
-        if (R > 2) F.k2 = cmul(F.k2, bf[11]);
-        if (R > 3) F.k3 = cmul(F.k3, bf[12]);
-    }
-    for (int k = 0; k < Q.nt; ++k) {
-        const DevTerm& T = terms[Q.t0 + k];
-        const cd2 f = mk2(T.re, T.im);
-        if (T.cls == PT_TR) {
-            const bool on = (t >> T.b) & 1;
-            if (T.a == 0) F.k0 = on ? cmul(F.k0, f) : F.k0;
-            else if (T.a == 1) F.k1 = on ? cmul(F.k1, f) : F.k1;
-            else if (T.a == 2) F.k2 = on ? cmul(F.k2, f) : F.k2;
-            else F.k3 = on ? cmul(F.k3, f) : F.k3;
-        } else {
-            const bool on = T.cls == PT_T1 ? ((t >> T.a) & 1) : (((t >> T.a) & (t >> T.b)) & 1);
-            F.A = on ? cmul(F.A, f) : F.A;
-        }
-    }
-    return F;
-}
-
-// DG_PHASE_SIGN: every factor is +-1, so the group is a 16-bit parity mask of the
-// register block; straight-line sign flips.
-template <int R>
-__device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], const DevPhase& Q,
-                                                 const DevTerm* __restrict__ terms,
+__device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], unsigned M,
                                                  const cd2* __restrict__ bf, int t) {
-    const PhaseFactors F = phase_factors<R>(Q, terms, bf, t);
-    unsigned M = (F.A.x < 0.0) ? 0xFFFFu : 0u;
-    if (F.k0.x < 0.0) M ^= reg_col_mask(0);
-    if (R > 1 && F.k1.x < 0.0) M ^= reg_col_mask(1);
-    if (R > 2 && F.k2.x < 0.0) M ^= reg_col_mask(2);
-    if (R > 3 && F.k3.x < 0.0) M ^= reg_col_mask(3);
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int s2 = r + 1; s2 < R; ++s2) {
-            const int pi = r == 0 ? s2 - 1 : (r == 1 ? s2 + 1 : 5);
-            if (Q.crr[pi][0] < 0.0) M ^= reg_col_mask(r) & reg_col_mask(s2);
-        }
+    if (bf) {
+        // value 0 of a sign group: A_B and the packed negative lambda / kappa bits
+        const cd2 b0 = bf[0];
+        const unsigned bits = (unsigned)b0.y;
+        const bool neg = (b0.x < 0.0) != (__builtin_popcount((unsigned)t & bits & 0xFFu) & 1);
+        M ^= neg ? 0xFFFFu : 0u;
+        if (bits & 0x100u) M ^= reg_col_mask(0);
+        if (R > 1 && (bits & 0x200u)) M ^= reg_col_mask(1);
+        if (R > 2 && (bits & 0x400u)) M ^= reg_col_mask(2);
+        if (R > 3 && (bits & 0x800u)) M ^= reg_col_mask(3);
+    }
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
         const double sg = ((M >> j) & 1u) ? -1.0 : 1.0;
@@ -313,26 +269,39 @@ __device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], const DevPhas
 // DG_PHASE: general unitary diagonal group.  The factor of register index j is
 // A * prod_r kappa_r^{j_r} * prod_{r<s} c_rs^{j_r j_s}; it is built top bit by top bit
 // (g[j + 2^t] = g[j] * kappa_t * prod_{r in j} c_rt) and applied once per amplitude.
+// `e` is this thread's table entry [A_t, kappa_0..3].
 template <int R>
 __device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const DevPhase& Q,
-                                                 const DevTerm* __restrict__ terms,
+                                                 const cd2 (&e)[PHASE_THREAD_VALUES],
                                                  const cd2* __restrict__ bf, int t) {
-    const PhaseFactors F = phase_factors<R>(Q, terms, bf, t);
+    cd2 A = e[0], k0 = e[1], k1 = e[2], k2 = e[3], k3 = e[4];
+    if (bf) {
+        A = cmul(A, bf[0]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const cd2 Af = cmul(A, bf[1 + q]);
+            A = ((t >> q) & 1) ? Af : A;
+        }
+        k0 = cmul(k0, bf[9]);
+        if (R > 1) k1 = cmul(k1, bf[10]);
+        if (R > 2) k2 = cmul(k2, bf[11]);
+        if (R > 3) k3 = cmul(k3, bf[12]);
+    }
     const cd2 c01 = mk2(Q.crr[0][0], Q.crr[0][1]), c02 = mk2(Q.crr[1][0], Q.crr[1][1]);
     const cd2 c03 = mk2(Q.crr[2][0], Q.crr[2][1]), c12 = mk2(Q.crr[3][0], Q.crr[3][1]);
     const cd2 c13 = mk2(Q.crr[4][0], Q.crr[4][1]), c23 = mk2(Q.crr[5][0], Q.crr[5][1]);
     // Factors of indices 0..7 are kept for the top bit; each is applied once known.
-    const cd2 g0 = F.A, g1 = cmul(F.A, F.k0);
+    const cd2 g0 = A, g1 = cmul(A, k0);
     a[0] = cmul(g0, a[0]);
     a[1] = cmul(g1, a[1]);
     if constexpr (R > 1) {
-        const cd2 g2 = cmul(F.A, F.k1), g3 = cmul(g1, cmul(F.k1, c01));
+        const cd2 g2 = cmul(A, k1), g3 = cmul(g1, cmul(k1, c01));
         a[2] = cmul(g2, a[2]);
         a[3] = cmul(g3, a[3]);
         if constexpr (R > 2) {
             __builtin_amdgcn_sched_barrier(0);
-            const cd2 w1 = cmul(F.k2, c02), w2 = cmul(F.k2, c12), w3 = cmul(w1, c12);
-            const cd2 g4 = cmul(F.A, F.k2), g5 = cmul(g1, w1);
+            const cd2 w1 = cmul(k2, c02), w2 = cmul(k2, c12), w3 = cmul(w1, c12);
+            const cd2 g4 = cmul(A, k2), g5 = cmul(g1, w1);
             const cd2 g6 = cmul(g2, w2), g7 = cmul(g3, w3);
             a[4] = cmul(g4, a[4]);
             a[5] = cmul(g5, a[5]);
@@ -340,10 +309,10 @@ __device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const DevPhas
             a[7] = cmul(g7, a[7]);
             if constexpr (R > 3) {
                 __builtin_amdgcn_sched_barrier(0);
-                const cd2 v1 = cmul(F.k3, c03), v2 = cmul(F.k3, c13), v4 = cmul(F.k3, c23);
+                const cd2 v1 = cmul(k3, c03), v2 = cmul(k3, c13), v4 = cmul(k3, c23);
                 const cd2 v3 = cmul(v1, c13), v5 = cmul(v1, c23), v6 = cmul(v2, c23);
                 const cd2 v7 = cmul(v3, c23);
-                a[8] = cmul(cmul(F.A, F.k3), a[8]);
+                a[8] = cmul(cmul(A, k3), a[8]);
                 a[9] = cmul(cmul(g1, v1), a[9]);
                 a[10] = cmul(cmul(g2, v2), a[10]);
                 a[11] = cmul(cmul(g3, v3), a[11]);
@@ -389,6 +358,15 @@ __global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
                 if (i == idx) v[i] = cmul(v[i], f);
         }
     }
+    if (Q->sign) {
+        // Sign groups read only value 0: its imaginary part (zero for a +-1 factor)
+        // carries the negative lambda_q as bits 0..7 and negative kappa_r as bits 8..11.
+        uint32_t bits = 0;
+#pragma unroll
+        for (int i = 1; i < PHASE_BLOCK_VALUES; ++i)
+            if (v[i].x < 0.0) bits |= 1u << (i - 1);
+        v[0].y = (double)bits;
+    }
     cd2* out = factors + ((size_t)(P->slot0 + slot) * nblocks + blk) * PHASE_BLOCK_VALUES;
 #pragma unroll
     for (int k = 0; k < PHASE_BLOCK_VALUES; ++k) out[k] = v[k];
@@ -400,8 +378,8 @@ template <int R>
 __global__ void __launch_bounds__(1 << (SV_TILE_BITS - SV_REG_BITS))
 sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
                const DevGate* __restrict__ gates, const DevPhase* __restrict__ phases,
-               const DevTerm* __restrict__ terms, const cd2* __restrict__ factors,
-               uint32_t ntiles) {
+               const cd2* __restrict__ tfactors, const uint32_t* __restrict__ tmasks,
+               const cd2* __restrict__ factors, uint32_t ntiles) {
     constexpr int NR = 1 << R;
     __shared__ cd2 lds[kLdsSlots];
 
@@ -410,7 +388,6 @@ sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
     const int nop = P->nop;
     const DevGate* gt = gates + P->gate0;
     const DevPhase* ph = phases + P->phase0;
-    const DevTerm* tm = terms + P->term0;
     const cd2* tile_factors = factors + ((size_t)P->slot0 * ntiles + tile) * PHASE_BLOCK_VALUES;
     const uint64_t base = tile_base(P, tile);
 
@@ -426,7 +403,8 @@ sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
     // twice: an op writes its results to fresh registers while its inputs are live, so
     // two ops per trip let the results ping-pong between two register sets instead of
     // being copied back at every loop latch.
-    auto run_op = [&](const DevGate& G) __attribute__((always_inline)) {
+    auto run_op = [&](int gi) __attribute__((always_inline)) {
+        const DevGate& G = gt[gi];
         const int kind = G.kind;
         if (kind == DG_U1X) {
             u1_bit<R, 0>(a, G.m);
@@ -440,10 +418,13 @@ sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
             const cd2* bf = Q.slot >= 0
                                 ? tile_factors + (size_t)Q.slot * ntiles * PHASE_BLOCK_VALUES
                                 : nullptr;
-            if (kind == DG_PHASE_SIGN)
-                apply_phase_sign<R>(a, Q, tm, bf, t);
-            else
-                apply_phase_cplx<R>(a, Q, tm, bf, t);
+            if (kind == DG_PHASE_SIGN) {
+                apply_phase_sign<R>(a, tmasks[Q.tab + t], bf, t);
+            } else {
+                const cd2* e = tfactors + (size_t)(Q.tab + t) * PHASE_THREAD_VALUES;
+                const cd2 f[PHASE_THREAD_VALUES] = {e[0], e[1], e[2], e[3], e[4]};
+                apply_phase_cplx<R>(a, Q, f, bf, t);
+            }
         } else if (kind == DG_LAYOUT) {
 #pragma unroll
             for (int j = 0; j < NR; ++j) lds[lds_slot(Lcur.loc(j))] = a[j];
@@ -461,8 +442,8 @@ sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
         }
     };
     for (int gi = 0; gi < nop; gi += 2) {
-        run_op(gt[gi]);
-        if (gi + 1 < nop) run_op(gt[gi + 1]);
+        run_op(gi);
+        if (gi + 1 < nop) run_op(gi + 1);
     }
 
 #pragma unroll
@@ -535,6 +516,8 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
     const DevGate* dgates = reinterpret_cast<const DevGate*>(blob + L.gates);
     const DevPhase* dphases = reinterpret_cast<const DevPhase*>(blob + L.phases);
     const DevTerm* dterms = reinterpret_cast<const DevTerm*>(blob + L.terms);
+    const cd2* dtf = reinterpret_cast<const cd2*>(blob + L.tfactors);
+    const uint32_t* dtm = reinterpret_cast<const uint32_t*>(blob + L.tmasks);
     cd2* dfactors = reinterpret_cast<cd2*>(const_cast<char*>(blob) + L.factors);
     cd2* st = reinterpret_cast<cd2*>(state);
     for (size_t p = 0; p < plan.passes.size(); ++p) {
@@ -551,13 +534,13 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
         }
         switch (P.R) {
             case 1: hipLaunchKernelGGL(sv_pass_kernel<1>, dim3(grid), dim3(threads), 0, stream,
-                                       st, dP, dgates, dphases, dterms, dfactors, tiles); break;
+                                       st, dP, dgates, dphases, dtf, dtm, dfactors, tiles); break;
             case 2: hipLaunchKernelGGL(sv_pass_kernel<2>, dim3(grid), dim3(threads), 0, stream,
-                                       st, dP, dgates, dphases, dterms, dfactors, tiles); break;
+                                       st, dP, dgates, dphases, dtf, dtm, dfactors, tiles); break;
             case 3: hipLaunchKernelGGL(sv_pass_kernel<3>, dim3(grid), dim3(threads), 0, stream,
-                                       st, dP, dgates, dphases, dterms, dfactors, tiles); break;
+                                       st, dP, dgates, dphases, dtf, dtm, dfactors, tiles); break;
             default: hipLaunchKernelGGL(sv_pass_kernel<4>, dim3(grid), dim3(threads), 0, stream,
-                                        st, dP, dgates, dphases, dterms, dfactors, tiles); break;
+                                        st, dP, dgates, dphases, dtf, dtm, dfactors, tiles); break;
         }
         QS_HIP(hipGetLastError());
     }

@@ -196,6 +196,19 @@ void emit_phase_group(const std::vector<const HostGate*>& diag, const int* regs,
         }
     }
     DevPhase ph{};
+    // Per-thread table of the thread/register classes.
+    const int nthr = 1 << (K - R);
+    auto thread_factors = [&](int t, cd& A, cd* k) {
+        A = c;
+        for (int i = 0; i < 4; ++i) k[i] = kr[i];
+        for (const DevTerm& x : tt) {
+            const cd f(x.re, x.im);
+            const bool ba = (t >> x.a) & 1, bb = x.b >= 0 && ((t >> x.b) & 1);
+            if (x.cls == PT_T1 && ba) A *= f;
+            else if (x.cls == PT_TT && ba && bb) A *= f;
+            else if (x.cls == PT_TR && bb) k[x.a] *= f;
+        }
+    };
     bool sign = is_sign(c);
     for (int i = 0; i < 4; ++i) sign = sign && is_sign(kr[i]);
     for (int i = 0; i < 6; ++i) sign = sign && is_sign(crr[i]);
@@ -211,6 +224,32 @@ void emit_phase_group(const std::vector<const HostGate*>& diag, const int* regs,
     for (int i = 0; i < 6; ++i) {
         ph.crr[i][0] = crr[i].real();
         ph.crr[i][1] = crr[i].imag();
+    }
+    if (sign) {
+        ph.tab = (int)plan->thread_masks.size();
+        static const uint32_t col[4] = {0xAAAAu, 0xCCCCu, 0xF0F0u, 0xFF00u};
+        static const int pr[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+        for (int t = 0; t < nthr; ++t) {
+            cd A, k[4];
+            thread_factors(t, A, k);
+            uint32_t M = A.real() < 0.0 ? 0xFFFFu : 0u;
+            for (int r = 0; r < R; ++r)
+                if (k[r].real() < 0.0) M ^= col[r];
+            for (int i = 0; i < 6; ++i)
+                if (pr[i][1] < R && crr[i].real() < 0.0) M ^= col[pr[i][0]] & col[pr[i][1]];
+            plan->thread_masks.push_back(M);
+        }
+    } else {
+        ph.tab = (int)(plan->thread_factors.size() / (2 * PHASE_THREAD_VALUES));
+        for (int t = 0; t < nthr; ++t) {
+            cd A, k[4];
+            thread_factors(t, A, k);
+            const cd v[PHASE_THREAD_VALUES] = {A, k[0], k[1], k[2], k[3]};
+            for (const cd& z : v) {
+                plan->thread_factors.push_back(z.real());
+                plan->thread_factors.push_back(z.imag());
+            }
+        }
     }
     ph.t0 = (int)plan->terms.size() - P.term0;
     ph.nt = (int)tt.size();
@@ -610,6 +649,8 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
     plan->gates.clear();
     plan->phases.clear();
     plan->terms.clear();
+    plan->thread_factors.clear();
+    plan->thread_masks.clear();
     plan->nslot_total = 0;
     std::vector<Item> items;
     items.reserve(ng);
@@ -726,6 +767,10 @@ SvBlobLayout sv_blob_layout(const SvPlan& plan, int nq) {
     off += sv_blob_offset(std::max<size_t>(plan.phases.size(), 1) * sizeof(DevPhase));
     L.terms = off;
     off += sv_blob_offset(std::max<size_t>(plan.terms.size(), 1) * sizeof(DevTerm));
+    L.tfactors = off;
+    off += sv_blob_offset(std::max<size_t>(plan.thread_factors.size(), 1) * sizeof(double));
+    L.tmasks = off;
+    off += sv_blob_offset(std::max<size_t>(plan.thread_masks.size(), 1) * sizeof(uint32_t));
     L.factors = off;
     size_t nblocks = 1;
     if (!plan.passes.empty() && nq > plan.passes[0].K) nblocks = (size_t)1 << (nq - plan.passes[0].K);

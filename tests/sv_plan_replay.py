@@ -105,7 +105,63 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
 
 
 def phase_factor(ph, tq, regs, idx):
-    """Per-amplitude factor of a phase group, resolving each term class to qubits."""
+    """Per-amplitude factor of a phase group, resolving each term class to qubits.
+
+    Also rebuilds the factor the way the kernel does (per-thread table for the
+    thread/register classes, block terms on top) and checks that both agree.
+    """
+    f = _phase_factor_terms(ph, tq, regs, idx)
+    g = _phase_factor_table(ph, tq, regs, idx)
+    assert np.allclose(f, g, rtol=0, atol=1e-12), np.max(np.abs(f - g))
+    return f
+
+
+def _thread_and_reg_index(tq, regs, idx):
+    K = len(tq)
+    thread_pos = [p for p in range(K) if p not in regs]
+    t = np.zeros(idx.size, dtype=np.int64)
+    for b, p in enumerate(thread_pos):
+        t |= ((idx >> tq[p]) & 1) << b
+    j = np.zeros(idx.size, dtype=np.int64)
+    for r, p in enumerate(regs):
+        j |= ((idx >> tq[p]) & 1) << r
+    return t, j, thread_pos
+
+
+def _phase_factor_table(ph, tq, regs, idx):
+    t, j, thread_pos = _thread_and_reg_index(tq, regs, idx)
+    bit = lambda q: (idx >> q) & 1
+    tab = np.asarray(ph['table'])
+    R = len(regs)
+    if ph['sign']:
+        M = tab.astype(np.int64)[t]
+        f = np.where((M >> j) & 1 == 1, -1.0 + 0j, 1.0 + 0j)
+    else:
+        v = tab.reshape(-1, 5, 2)
+        e = v[..., 0] + 1j * v[..., 1]           # (threads, 5): A, kappa_0..3
+        f = e[t, 0].copy()
+        for r in range(R):
+            f = np.where((j >> r) & 1 == 1, f * e[t, 1 + r], f)
+        for k, (r, s2) in enumerate(PAIRS):
+            if s2 < R:
+                c = complex(ph['crr'][2 * k], ph['crr'][2 * k + 1])
+                f = np.where(((j >> r) & (j >> s2) & 1) == 1, f * c, f)
+    # block classes, as the prologue folds them
+    for cls, a, b, re, im in ph['terms']:
+        z = complex(re, im)
+        if cls == PT_BB:
+            on = bit(a) & (bit(b) if b >= 0 else 1)
+        elif cls == PT_BT:
+            on = bit(tq[thread_pos[a]]) & bit(b)
+        elif cls == PT_BR:
+            on = bit(tq[regs[a]]) & bit(b)
+        else:
+            continue
+        f = np.where(on == 1, f * z, f)
+    return f
+
+
+def _phase_factor_terms(ph, tq, regs, idx):
     K = len(tq)
     thread_pos = [p for p in range(K) if p not in regs]      # thread bit i -> local position
     bit = lambda q: (idx >> q) & 1
