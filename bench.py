@@ -5,8 +5,11 @@ Headline (`value`): 28-qubit complex128 statevector sweep, gates/s, on the
 workload "20 layers of U3 on every qubit + CZ ladder, then QFT-28" (1520
 logical gates) with the state resident in HBM.  A step applies the whole
 circuit once.  The line also carries:
-  roofline     : HBM bytes of the dominant kernel (the fused pass) per launch
-                 over its average duration, against 8 TB/s,
+  roofline     : algorithmic HBM bytes of the dominant kernel (the fused pass:
+                 read + write of the 4 GiB state) per launch over its average
+                 duration, timed with HIP events around every pass launch of the
+                 timed steps, against 8 TB/s; `traffic` is the measured HBM bytes
+                 per launch from the committed rocprofv3 counter summary,
   cpu_baseline : the NumPy oracle's pair updates on the host, on a bounded
                  sample of the same 28-qubit workload,
   mps          : two-site MPS updates/s at chi=64 (1000 qubits, brick-wall of
@@ -45,13 +48,16 @@ def parse():
     ap.add_argument('--no-mps', action='store_true', help='skip the MPS leg')
     ap.add_argument('--cpu-gates', type=int, default=6,
                     help='logical gates of the workload timed on the CPU')
+    ap.add_argument('--calibration-copy', action='store_true',
+                    help='after the SV leg, copy the 4 GiB state once (a copy kernel of '
+                         'known size for scaling the HBM byte counters of a profile)')
     return ap.parse_args()
 
 
-def sv_leg(steps: int, warmup: int):
+def sv_leg(steps: int, warmup: int, calibration_copy: bool = False):
     import torch
     from qsim_amd import workloads
-    from qsim_amd.sv import StateVector, gates_to_records, plan_passes
+    from qsim_amd.sv import StateVector, gates_to_records, pass_times, plan_passes, time_passes
 
     gates = workloads.sv_benchmark_circuit(SV_QUBITS, SV_DEPTH, SV_SEED)
     recs, _ = gates_to_records(gates, SV_QUBITS)
@@ -62,6 +68,8 @@ def sv_leg(steps: int, warmup: int):
         sv.apply_records(recs)
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # the library brackets every pass-kernel launch with HIP events on this stream
+    time_passes(True)
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(steps):
@@ -69,10 +77,17 @@ def sv_leg(steps: int, warmup: int):
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    kernel_ms = pass_times()
+    time_passes(False)
     gpu_ms = ev0.elapsed_time(ev1)
     norm = sv.norm_sq()
+    if calibration_copy:
+        sv.tensor.clone()
+        torch.cuda.synchronize()
     bytes_per_pass = 2 * 16 * (1 << SV_QUBITS)
-    per_launch_s = gpu_ms / 1e3 / (steps * npass)
+    if len(kernel_ms) != steps * npass:
+        raise RuntimeError(f'expected {steps * npass} pass timings, got {len(kernel_ms)}')
+    per_launch_s = sum(kernel_ms) / len(kernel_ms) / 1e3
     return {
         'wall_s': wall, 'gpu_ms': gpu_ms, 'npass': npass, 'ngates': len(gates),
         'per_launch_s': per_launch_s, 'bytes_per_pass': bytes_per_pass,
@@ -102,16 +117,16 @@ def cpu_leg(ngates: int):
 
 
 def traffic_from_profiles():
-    """HBM bytes per pass launch from the committed rocprofv3 PMC summary, if present."""
-    path = os.path.join(ROOT, 'profiles', 'sv_pass_28q_pmc.json')
+    """HBM bytes per pass launch from the committed rocprofv3 PMC summary, if present
+    (the line `hbm_bytes_per_launch: N` of profiles/sv_pass_28q_traffic.md)."""
+    path = os.path.join(ROOT, 'profiles', 'sv_pass_28q_traffic.md')
     if not os.path.exists(path):
         return None
-    try:
-        with open(path) as f:
-            d = json.load(f)
-        return d.get('hbm_bytes_per_launch')
-    except Exception:
-        return None
+    with open(path) as f:
+        for line in f:
+            if line.strip().startswith('hbm_bytes_per_launch:'):
+                return int(float(line.split(':', 1)[1].strip()))
+    return None
 
 
 def mps_leg():
@@ -133,7 +148,7 @@ def main():
         print(json.dumps({'error': 'no GPU visible'}))
         sys.exit(2)
 
-    sv = sv_leg(args.steps, args.warmup)
+    sv = sv_leg(args.steps, args.warmup, args.calibration_copy)
     value = sv['ngates'] * args.steps / (sv['gpu_ms'] / 1e3)
     achieved = sv['bytes_per_pass'] / sv['per_launch_s'] / 1e9
     out = {

@@ -310,6 +310,18 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
     return QS_OK;
 }
 
+int qs_sv_time_passes(int enable) {
+    clear_error();
+    return sv_time_passes(enable);
+}
+
+int qs_sv_pass_times(double* ms_out, int cap, int* count) {
+    clear_error();
+    QS_REQUIRE(count != nullptr && cap >= 0 && (cap == 0 || ms_out != nullptr), QS_E_ARG,
+               "bad output arguments");
+    return sv_pass_times(ms_out, cap, count);
+}
+
 int qs_sv_norm_sq(const void* state, int nq, double* out_host, void* stream) {
     clear_error();
     int st = check_state(state, nq);

@@ -192,6 +192,10 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
 // Fills a DevLayout for register positions regs[0..R-1] (ascending) of pass P.
 void sv_fill_layout(const DevPass& P, const int* regs, DevLayout* out);
 
+// Pass-kernel timing behind qs_sv_time_passes / qs_sv_pass_times.
+int sv_time_passes(int enable);
+int sv_pass_times(double* ms_out, int cap, int* count);
+
 int sv_fill_zero(double2* state, int nq, hipStream_t stream);
 int sv_scale(double2* state, int nq, double re, double im, hipStream_t stream);
 int sv_project(double2* state, int nq, int q, hipStream_t stream);

@@ -508,6 +508,26 @@ int grid_for(uint64_t n, int threads) {
 
 }  // namespace
 
+// Pass-kernel timing (qs_sv_time_passes): events from a per-thread pool, bracketing
+// each pass launch on its stream.
+struct PassTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;      // created events, reused
+    size_t used = 0;                   // events recorded since the last read-out
+    ~PassTimer() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+};
+thread_local PassTimer g_timer;
+
 int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_blob,
                    hipStream_t stream) {
     const char* blob = reinterpret_cast<const char*>(dev_blob);
@@ -532,6 +552,13 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
                                0, stream, nq, dP, dphases, dterms, dfactors, tiles);
             QS_HIP(hipGetLastError());
         }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (g_timer.on) {
+            e0 = g_timer.next();
+            e1 = g_timer.next();
+            QS_REQUIRE(e0 && e1, QS_E_HIP, "hipEventCreate failed");
+            QS_HIP(hipEventRecord(e0, stream));
+        }
         switch (P.R) {
             case 1: hipLaunchKernelGGL(sv_pass_kernel<1>, dim3(grid), dim3(threads), 0, stream,
                                        st, dP, dgates, dphases, dtf, dtm, dfactors, tiles); break;
@@ -543,7 +570,27 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
                                         st, dP, dgates, dphases, dtf, dtm, dfactors, tiles); break;
         }
         QS_HIP(hipGetLastError());
+        if (e1) QS_HIP(hipEventRecord(e1, stream));
     }
+    return QS_OK;
+}
+
+int sv_time_passes(int enable) {
+    g_timer.on = enable != 0;
+    g_timer.used = 0;
+    return QS_OK;
+}
+
+int sv_pass_times(double* ms_out, int cap, int* count) {
+    const int n = (int)(g_timer.used / 2);
+    for (int i = 0; i < n && i < cap; ++i) {
+        QS_HIP(hipEventSynchronize(g_timer.pool[2 * i + 1]));
+        float ms = 0.0f;
+        QS_HIP(hipEventElapsedTime(&ms, g_timer.pool[2 * i], g_timer.pool[2 * i + 1]));
+        ms_out[i] = ms;
+    }
+    *count = n;
+    g_timer.used = 0;
     return QS_OK;
 }
 

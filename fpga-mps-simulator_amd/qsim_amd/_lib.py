@@ -64,6 +64,9 @@ _PROTOS = {
     'qs_sv_plan_json': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(QsGate), ctypes.c_int,
                                        ctypes.c_char_p, ctypes.c_size_t,
                                        ctypes.POINTER(ctypes.c_size_t)]),
+    'qs_sv_time_passes': (ctypes.c_int, [ctypes.c_int]),
+    'qs_sv_pass_times': (ctypes.c_int, [_c_double_p, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int)]),
     'qs_sv_norm_sq': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p,
                                      ctypes.c_void_p]),
     'qs_sv_scale': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,

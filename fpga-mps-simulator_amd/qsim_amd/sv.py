@@ -120,6 +120,19 @@ class StateVector:
         return p
 
 
+def time_passes(enable: bool) -> None:
+    """Bracket every fused-pass kernel launched from this thread with HIP events."""
+    _lib.call('qs_sv_time_passes', 1 if enable else 0)
+
+
+def pass_times(cap: int = 1 << 16) -> List[float]:
+    """Durations (ms) of the pass kernels recorded since the last call; waits for them."""
+    buf = (ctypes.c_double * cap)()
+    n = ctypes.c_int(0)
+    _lib.call('qs_sv_pass_times', buf, cap, ctypes.byref(n))
+    return [buf[i] for i in range(min(n.value, cap))]
+
+
 def plan_passes(nq: int, records: Sequence[tuple]) -> int:
     arr = records_to_array(records)
     n = ctypes.c_int(0)

@@ -112,6 +112,14 @@ int qs_sv_plan_passes(int nq, const qs_gate* gates, int ng, int* npasses);
 int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
                     size_t* needed);
 
+/* Instrumentation, no counterpart in the reference.  While enabled (per host
+ * thread), every fused-pass kernel that qs_sv_apply / qs_sv_apply_words launch is
+ * bracketed by HIP events recorded on the caller's stream.  qs_sv_pass_times waits
+ * for the recorded events, writes up to `cap` kernel durations in milliseconds (launch
+ * order) into ms_out and the number recorded into *count, then clears the record. */
+int qs_sv_time_passes(int enable);
+int qs_sv_pass_times(double* ms_out, int cap, int* count);
+
 /* sum |a_i|^2 into *out_host.  Synchronous. */
 int qs_sv_norm_sq(const void* state, int nq, double* out_host, void* stream);
 
