@@ -26,7 +26,8 @@ namespace qs {
 
 namespace {
 
-constexpr int kMaxChi = 64;              // register SVD holds 2*chi <= 128 rows/cols
+constexpr int kMaxChi = 128;             // largest chi_max of a task
+constexpr int kRegChi = 64;              // register SVD / MFMA contraction: 2*chi <= 128
 constexpr int kTeamLanes = 16;
 constexpr int kMaxSweeps = 60;
 
@@ -63,6 +64,12 @@ __device__ __forceinline__ bool load_dims(const qs_mps_task& T, Dims& d) {
     d.cr = T.bond[T.site + 2];
     return d.cl >= 1 && d.cm >= 1 && d.cr >= 1 && d.cl <= T.chi_max && d.cm <= T.chi_max &&
            d.cr <= T.chi_max && T.chi_max <= kMaxChi;
+}
+
+// Workspace elements per task: theta (2chi x 2chi); the chi-128 class adds its working
+// copy of the columns.
+inline size_t theta_slot_elems(int chi) {
+    return (size_t)(chi > kRegChi ? 8 : 4) * chi * chi;
 }
 
 __device__ __forceinline__ cd2* site_ptr(const qs_mps_task& T, int k) {
@@ -121,8 +128,8 @@ mps_contract_fp64_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMfmaThreads = 256;                 // 4 waves, one 64x64 quadrant each
-constexpr int kMaxM = 2 * kMaxChi;                // 128
-constexpr int kMaxK = kMaxChi;                    // 64
+constexpr int kMaxM = 2 * kRegChi;                // 128
+constexpr int kMaxK = kRegChi;                    // 64
 
 __global__ void __launch_bounds__(kMfmaThreads)
 mps_contract_f32_mfma_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws,
@@ -137,7 +144,7 @@ mps_contract_f32_mfma_kernel(const DevTask* __restrict__ tasks, cd2* __restrict_
 
     const qs_mps_task& T = tasks[blockIdx.x].t;
     Dims d;
-    if (!load_dims(T, d)) {
+    if (!load_dims(T, d) || d.cl > kRegChi || d.cm > kRegChi || d.cr > kRegChi) {
         if (threadIdx.x == 0 && status) status[tasks[blockIdx.x].orig] = QS_E_CHI;
         return;
     }
@@ -551,6 +558,227 @@ __global__ void mps_apply_1q_kernel(cd2* __restrict__ sites, const int32_t* __re
 
 using namespace qs;
 
+// ------------------------------------------------- Jacobi SVD, chi_max up to 128
+// theta up to 256 x 256 (1 MiB) does not fit on chip, so the columns live in a working
+// copy W (column-major, next to theta in the task's workspace slot, L2-resident while
+// the task runs).  Columns are paired by index with the round-robin tournament: in round
+// r the player in slot s >= 1 is 1 + (s - 1 + r) mod (2P - 1), slot 0 keeps player 0,
+// and pair i is (slot i, slot 2P - 1 - i); every column belongs to one pair per round,
+// so a round is read-rotate-write in place and one barrier.  32 teams of 32 lanes; team
+// t handles pairs t, t + 32, ... (8 rows per lane per column in registers).
+constexpr int kBigLanes = 32;
+constexpr int kBigTeams = 32;
+constexpr int kBigThreads = kBigTeams * kBigLanes;    // 1024
+constexpr int kBigRows = 2 * kMaxChi / kBigLanes;     // 8 rows per lane
+
+__device__ __forceinline__ double big_sum(double v) {
+    v += __shfl_xor(v, 16, kBigLanes);
+    v += __shfl_xor(v, 8, kBigLanes);
+    v += __shfl_xor(v, 4, kBigLanes);
+    v += __shfl_xor(v, 2, kBigLanes);
+    v += __shfl_xor(v, 1, kBigLanes);
+    return v;
+}
+
+__device__ __forceinline__ int big_player(int slot, int r, int P) {
+    return slot == 0 ? 0 : 1 + (slot - 1 + r) % (2 * P - 1);
+}
+
+struct BigShared {
+    double sval[2 * kMaxChi];
+    int order[2 * kMaxChi];
+    int rank[2 * kMaxChi];
+    double part[kBigTeams];
+    double fro;
+    int rotated;
+    int keep;
+    double scale;
+};
+
+__global__ void __launch_bounds__(kBigThreads)
+mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size_t slot,
+                   int32_t* __restrict__ status) {
+    __shared__ BigShared sh;
+    const DevTask& DT = tasks[blockIdx.x];
+    const qs_mps_task& T = DT.t;
+    Dims d;
+    if (!load_dims(T, d)) {
+        if (threadIdx.x == 0 && status) status[DT.orig] = QS_E_CHI;
+        return;
+    }
+    const int m = 2 * d.cl, n = 2 * d.cr;
+    const int P = n / 2;
+    const int tid = threadIdx.x;
+    const int team = tid / kBigLanes, l16 = tid % kBigLanes;
+    const cd2* th = ws + (size_t)blockIdx.x * slot;
+    cd2* W = ws + (size_t)blockIdx.x * slot + slot / 2;
+
+    double fpart = 0.0;
+    for (int idx = tid; idx < m * n; idx += kBigThreads) {
+        const int r = idx / n, c = idx - r * n;
+        const cd2 v = th[idx];
+        W[(size_t)c * m + r] = v;
+        fpart = fma(v.x, v.x, fma(v.y, v.y, fpart));
+    }
+    fpart = big_sum(fpart);
+    if (l16 == 0) sh.part[team] = fpart;
+    __syncthreads();
+    if (tid == 0) {
+        double f = 0.0;
+        for (int t = 0; t < kBigTeams; ++t) f += sh.part[t];
+        sh.fro = f;
+    }
+    __syncthreads();
+    const double zero_sq = 1e-28 * sh.fro;
+    const double tol = 2.220446049250313e-16 * sqrt((double)m);
+
+    int sweeps = 0;
+    for (; sweeps < kMaxSweeps; ++sweeps) {
+        if (tid == 0) sh.rotated = 0;
+        __syncthreads();
+        const int rounds = P > 1 ? 2 * P - 1 : 1;
+        for (int rd = 0; rd < rounds; ++rd) {
+            for (int pi = team; pi < P; pi += kBigTeams) {
+                const int cp = big_player(pi, rd, P), cq = big_player(2 * P - 1 - pi, rd, P);
+                cd2* ca = W + (size_t)cp * m;
+                cd2* cb = W + (size_t)cq * m;
+                cd2 a[kBigRows], b[kBigRows];
+                double al = 0, be = 0, gr = 0, gi = 0;
+#pragma unroll
+                for (int k = 0; k < kBigRows; ++k) {
+                    const int r = l16 + kBigLanes * k;
+                    a[k] = r < m ? ca[r] : mk2(0, 0);
+                    b[k] = r < m ? cb[r] : mk2(0, 0);
+                    al = fma(a[k].x, a[k].x, fma(a[k].y, a[k].y, al));
+                    be = fma(b[k].x, b[k].x, fma(b[k].y, b[k].y, be));
+                    gr = fma(a[k].x, b[k].x, fma(a[k].y, b[k].y, gr));
+                    gi = fma(a[k].x, b[k].y, fma(-a[k].y, b[k].x, gi));
+                }
+                al = big_sum(al);
+                be = big_sum(be);
+                gr = big_sum(gr);
+                gi = big_sum(gi);
+                const double g = sqrt(gr * gr + gi * gi);
+                if (al > zero_sq && be > zero_sq && g > tol * sqrt(al) * sqrt(be)) {
+                    const double zeta = (be - al) / (2.0 * g);
+                    double t;
+                    if (fabs(zeta) > 1e150) {
+                        t = 0.5 / zeta;
+                    } else {
+                        t = 1.0 / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        if (zeta < 0) t = -t;
+                    }
+                    const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                    const double er = gr / g, ei = gi / g;
+                    const cd2 sem = mk2(s * er, -s * ei);
+                    const cd2 sep = mk2(s * er, s * ei);
+#pragma unroll
+                    for (int k = 0; k < kBigRows; ++k) {
+                        const int r = l16 + kBigLanes * k;
+                        const cd2 x = a[k], y = b[k];
+                        const cd2 sy = cmul(sem, y), sx = cmul(sep, x);
+                        if (r < m) {
+                            ca[r] = mk2(c * x.x - sy.x, c * x.y - sy.y);
+                            cb[r] = mk2(c * y.x + sx.x, c * y.y + sx.y);
+                        }
+                    }
+                    if (l16 == 0) sh.rotated = 1;
+                }
+            }
+            __syncthreads();
+        }
+        const int any = sh.rotated;
+        __syncthreads();
+        if (!any) break;
+    }
+
+    // Column norms (singular values), ranking, keep rule and norm_log as in the
+    // register kernel.
+    for (int c = team; c < n; c += kBigTeams) {
+        double v = 0.0;
+        for (int r = l16; r < m; r += kBigLanes) {
+            const cd2 x = W[(size_t)c * m + r];
+            v = fma(x.x, x.x, fma(x.y, x.y, v));
+        }
+        v = big_sum(v);
+        if (l16 == 0) sh.sval[c] = sqrt(v);
+    }
+    __syncthreads();
+    if (tid < n) {
+        const double v = sh.sval[tid];
+        int r = 0;
+        for (int j = 0; j < n; ++j) {
+            const double w = sh.sval[j];
+            r += (w > v) || (w == v && j < tid);
+        }
+        sh.rank[tid] = r;
+        sh.order[r] = tid;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int kmin = m < n ? m : n;
+        const double s0 = sh.sval[sh.order[0]];
+        const double thr = T.cutoff * s0;
+        int cnt = 0;
+        double total = 0.0;
+        for (int k = 0; k < kmin; ++k) {
+            const double sv = sh.sval[sh.order[k]];
+            cnt += (sv > thr);
+            total += sv * sv;
+        }
+        int keep = cnt < 1 ? 1 : cnt;
+        if (keep > T.chi_max) keep = T.chi_max;
+        double dlog = 0.0;
+        if (keep < kmin) {
+            double dropped = 0.0;
+            for (int k = keep; k < kmin; ++k) {
+                const double sv = sh.sval[sh.order[k]];
+                dropped += sv * sv;
+            }
+            if (dropped > 0.0) dlog += 0.5 * log(1.0 - dropped / fmax(total, 1e-300));
+        }
+        double kept = 0.0;
+        for (int k = 0; k < keep; ++k) {
+            const double sv = sh.sval[sh.order[k]];
+            kept += sv * sv;
+        }
+        const double nrm = sqrt(kept);
+        double scale = 1.0;
+        if (nrm > 1e-15) {
+            dlog += log(nrm);
+            scale = 1.0 / nrm;
+        }
+        sh.keep = keep;
+        sh.scale = scale;
+        *T.dlog = dlog;
+        T.bond[T.site + 1] = keep;
+        if (status) status[DT.orig] = (sweeps >= kMaxSweeps) ? QS_E_NOCONV : sweeps + 1;
+    }
+    __syncthreads();
+    const int keep = sh.keep;
+    const double scale = sh.scale;
+    // A' = U sqrt(S_t): column k of U is W[:, order[k]] / s_k.
+    cd2* Ao = site_ptr(T, T.site);
+    for (int idx = tid; idx < m * keep; idx += kBigThreads) {
+        const int r = idx / keep, k = idx - r * keep;
+        const double sv = sh.sval[sh.order[k]];
+        const double f = sv > 0 ? sqrt(sv * scale) / sv : 0.0;
+        const cd2 u = W[(size_t)sh.order[k] * m + r];
+        Ao[idx] = mk2(u.x * f, u.y * f);
+    }
+    // B' = sqrt(S_t) / S * U^H theta = sqrt(S_t) / S^2 * W_k^H theta.
+    cd2* Bo = site_ptr(T, T.site + 1);
+    for (int idx = tid; idx < keep * n; idx += kBigThreads) {
+        const int k = idx / n, c = idx - k * n;
+        const cd2* wk = W + (size_t)sh.order[k] * m;
+        cd2 acc = mk2(0, 0);
+        for (int r = 0; r < m; ++r) acc = cjmadd(acc, wk[r], th[(size_t)r * n + c]);
+        const double sv = sh.sval[sh.order[k]];
+        const double f = sv > 0 ? sqrt(sv * scale) / (sv * sv) : 0.0;
+        Bo[idx] = mk2(acc.x * f, acc.y * f);
+    }
+}
+
 extern "C" {
 
 int qs_mps_max_chi(void) { return kMaxChi; }
@@ -558,12 +786,13 @@ int qs_mps_max_chi(void) { return kMaxChi; }
 size_t qs_mps_workspace_bytes(int ntasks, int chi_max) {
     if (ntasks <= 0 || chi_max <= 0) return 0;
     const size_t head = sv_blob_offset((size_t)ntasks * sizeof(DevTask));
-    const size_t slot = (size_t)4 * chi_max * chi_max;
-    return head + (size_t)ntasks * slot * sizeof(cd2);
+    return head + (size_t)ntasks * theta_slot_elems(chi_max) * sizeof(cd2);
 }
 
 namespace {
-int chi_class(int chi) { return chi <= 8 ? 8 : chi <= 16 ? 16 : chi <= 32 ? 32 : 64; }
+int chi_class(int chi) {
+    return chi <= 8 ? 8 : chi <= 16 ? 16 : chi <= 32 ? 32 : chi <= 64 ? 64 : 128;
+}
 
 void launch_svd(int cls, const DevTask* t, int n, const cd2* theta, size_t slot,
                 int32_t* status, hipStream_t hs) {
@@ -574,8 +803,10 @@ void launch_svd(int cls, const DevTask* t, int n, const cd2* theta, size_t slot,
                                     slot, status); break;
         case 32: hipLaunchKernelGGL(mps_svd_kernel<32>, dim3(n), dim3(16 * 32), 0, hs, t, theta,
                                     slot, status); break;
-        default: hipLaunchKernelGGL(mps_svd_kernel<64>, dim3(n), dim3(16 * 64), 0, hs, t, theta,
+        case 64: hipLaunchKernelGGL(mps_svd_kernel<64>, dim3(n), dim3(16 * 64), 0, hs, t, theta,
                                     slot, status); break;
+        default: hipLaunchKernelGGL(mps_svd_big_kernel, dim3(n), dim3(kBigThreads), 0, hs, t,
+                                    const_cast<cd2*>(theta), slot, status); break;
     }
 }
 }  // namespace
@@ -618,7 +849,7 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
     std::vector<DevTask> dt(ntasks);
     std::vector<int> cls(ntasks);
     int k = 0;
-    for (int c : {8, 16, 32, 64})
+    for (int c : {8, 16, 32, 64, 128})
         for (int i = 0; i < ntasks; ++i)
             if (chi_class(tasks[i].chi_max) == c) {
                 dt[k].t = tasks[i];
@@ -630,15 +861,21 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
     char* ws = static_cast<char*>(workspace);
     DevTask* dtasks = reinterpret_cast<DevTask*>(ws);
     cd2* theta = reinterpret_cast<cd2*>(ws + sv_blob_offset((size_t)ntasks * sizeof(DevTask)));
-    const size_t slot = (size_t)4 * chi * chi;
+    const size_t slot = theta_slot_elems(chi);
     QS_HIP(hipMemcpyAsync(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask),
                           hipMemcpyHostToDevice, hs));
-    if (precision == QS_PREC_FP32_MFMA)
-        hipLaunchKernelGGL(mps_contract_f32_mfma_kernel, dim3(ntasks), dim3(kMfmaThreads), 0, hs,
+    // The MFMA contraction stages its operands in LDS up to chi 64; tasks of the chi-128
+    // class (grouped last) take the FP64 contraction whatever the requested precision.
+    int nreg = 0;
+    while (nreg < ntasks && cls[nreg] <= kRegChi) ++nreg;
+    if (precision == QS_PREC_FP32_MFMA && nreg > 0)
+        hipLaunchKernelGGL(mps_contract_f32_mfma_kernel, dim3(nreg), dim3(kMfmaThreads), 0, hs,
                            dtasks, theta, slot, status_dev);
-    else
-        hipLaunchKernelGGL(mps_contract_fp64_kernel, dim3(ntasks), dim3(256), 0, hs, dtasks,
-                           theta, slot, status_dev);
+    const int f64_first = precision == QS_PREC_FP32_MFMA ? nreg : 0;
+    if (ntasks > f64_first)
+        hipLaunchKernelGGL(mps_contract_fp64_kernel, dim3(ntasks - f64_first), dim3(256), 0, hs,
+                           dtasks + f64_first, theta + (size_t)f64_first * slot, slot,
+                           status_dev);
     QS_HIP(hipGetLastError());
     for (int b = 0; b < ntasks;) {
         int e = b;

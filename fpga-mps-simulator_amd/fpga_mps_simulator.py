@@ -13,8 +13,8 @@ Deviations (each deliberate):
   * no worker process: circuits run in the calling process, so there is no
     pickling of circuits and no process respawn; get_card_stats() reports this
     process;
-  * chi_max may be up to qs_mps_max_chi() (64 on this build), the same bound
-    the reference enforces (CHI_MAX_SVD_HW = 64);
+  * chi_max may be up to qs_mps_max_chi() (128 on this build; the reference
+    enforces CHI_MAX_SVD_HW = 64);
   * xclbin_path / FPGA_MPS_XCLBIN are accepted and ignored;
   * the keyword `precision` ('fp64' default, or 'fp32' for the complex-FP32
     MFMA contraction) has no counterpart in the reference.

@@ -153,7 +153,9 @@ int qs_sv_apply_tfim(const void* psi, void* out, int nq, double J, double h, voi
 #define QS_PREC_FP64       0   /* contraction FP64, SVD FP64                        */
 #define QS_PREC_FP32_MFMA  1   /* contraction complex FP32 on MFMA, SVD FP64        */
 
-/* Largest chi_max the two-site update supports. */
+/* Largest chi_max the two-site update supports (128).  Up to 64 the SVD keeps theta in
+ * registers; the 65..128 class keeps the columns in a working copy in the workspace, and
+ * its contraction runs in FP64 whatever the requested precision. */
 int qs_mps_max_chi(void);
 
 /* One two-site update of a chain: sites i and i+1 of the chain whose slots start at
@@ -175,7 +177,8 @@ typedef struct qs_mps_task {
     double   gate[32];     /* 4x4, row index = 2*sigma_i + sigma_{i+1}         */
 } qs_mps_task;
 
-/* Workspace bytes qs_mps_update_batch needs for `ntasks` tasks of at most chi_max. */
+/* Workspace bytes qs_mps_update_batch needs for `ntasks` tasks of at most chi_max
+ * (per task 4 chi_max^2 complex elements, 8 chi_max^2 when chi_max > 64). */
 size_t qs_mps_workspace_bytes(int ntasks, int chi_max);
 
 /* Run `ntasks` independent two-site updates in one launch sequence (no two tasks may

@@ -33,11 +33,14 @@ def test_abi_version_and_limits():
     lib = _lib.load_library()
     assert lib.qs_abi_version() == 1
     assert lib.qs_sv_max_qubits() == 34
-    assert lib.qs_mps_max_chi() == 64
+    assert lib.qs_mps_max_chi() == 128
     ntasks, chi = 5, 64
     ws = lib.qs_mps_workspace_bytes(ntasks, chi)
     theta = ntasks * 4 * chi * chi * 16          # one (2chi x 2chi) complex128 slot per task
     assert theta < ws < theta + ntasks * 1024 + 256
+    # chi_max above 64: theta plus the SVD's working copy of its columns
+    ws128 = lib.qs_mps_workspace_bytes(ntasks, 128)
+    assert ntasks * 8 * 128 * 128 * 16 < ws128 < ntasks * 8 * 128 * 128 * 16 + ntasks * 1024 + 256
     assert lib.qs_mps_workspace_bytes(0, chi) == 0
     assert ctypes.sizeof(_lib.QsGate) == 16 + 8 + 256
     assert ctypes.sizeof(_lib.QsMpsTask) == 8 * 3 + 8 + 4 + 4 + 8 + 256
@@ -68,8 +71,8 @@ def test_argument_errors_are_reported_without_device():
     t[0].sites = 1
     t[0].bond = 1
     t[0].dlog = 1
-    t[0].chi_max = 128
-    t[0].site_stride = 2 * 128 * 128
+    t[0].chi_max = 129
+    t[0].site_stride = 2 * 129 * 129
     assert lib.qs_mps_update_batch(t, 1, 0, ctypes.c_void_p(1), 1 << 30, None, None) == _lib.QS_E_CHI
     t[0].chi_max = 8
     assert lib.qs_mps_update_batch(t, 1, 7, ctypes.c_void_p(1), 1 << 30, None, None) == _lib.QS_E_ARG

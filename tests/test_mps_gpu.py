@@ -109,6 +109,24 @@ def test_haar_saturated_chi64_fp64_matches_oracle():
     assert np.max(np.abs(sv - rv)) < 1e-10 * np.max(np.abs(rv))
 
 
+@pytest.mark.parametrize('chi,precision', [(128, 'fp64'), (100, 'fp64'), (128, 'fp32')])
+def test_haar_chi128_class_matches_oracle(chi, precision):
+    # chi_max > 64 selects the SVD class that keeps theta's columns in the workspace;
+    # 16 qubits of Haar gates reach bond 256 untruncated, so chi 100 / 128 truncate.
+    n, depth, cut = 16, 16, 1e-12
+    ups = workloads.brickwall_gates(n, depth, seed=8, kind='haar')
+    ch = chain_run(n, chi, cut, ups, precision=precision)
+    ref = mps_ref.MPS(n, chi, cut)
+    for _, i, g in ups:
+        ref.apply_adjacent_2q(i, g)
+    host = ch.to_host()
+    assert host.bond_dims() == ref.bond_dims()
+    assert max(host.bond_dims()) == chi
+    assert abs(host.norm_log - ref.norm_log) < 1e-10 * max(1.0, abs(ref.norm_log))
+    sv, rv = host.get_statevector(), ref.get_statevector()
+    assert np.max(np.abs(sv - rv)) < 1e-10 * np.max(np.abs(rv))
+
+
 def test_fp32_mfma_fidelity():
     n, depth, chi, cut = 64, 16, 64, 1e-10
     ups = workloads.brickwall_gates(n, depth, seed=23)
@@ -204,6 +222,6 @@ def test_random_named_circuits_match_oracle(sim):
 def test_chi_limit_raises():
     from fpga_mps_simulator import FPGAMPSSimulator
     with pytest.raises(ValueError):
-        FPGAMPSSimulator(chi_max=65)
+        FPGAMPSSimulator(chi_max=129)
     with pytest.raises(ValueError):
         FPGAMPSSimulator(engine='cpu')
