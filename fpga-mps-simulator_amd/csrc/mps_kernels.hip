@@ -252,6 +252,54 @@ mps_contract_f32_mfma_kernel(const DevTask* __restrict__ tasks, cd2* __restrict_
 }
 
 // ------------------------------------------------------------------ Jacobi SVD
+// FP64 reciprocal and reciprocal square root: the hardware estimate refined by two
+// Newton steps (relative error at rounding level).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double h = 0.5 * x * y;
+        const double r = fma(-h, y, 0.5);
+        y = fma(y, r, y);
+    }
+    return y;
+}
+
+// Complex Jacobi rotation zeroing a^H b for ||a||^2 = al, ||b||^2 = be, a^H b = gr + i gi:
+//   a' = c a - s e^{-i phi} b,  b' = s e^{i phi} a + c b,  e^{i phi} = (gr + i gi)/|g|,
+// with t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), zeta = (be - al) / (2|g|),
+// c = 1/sqrt(1 + t^2), s = c t.  Returns false (no rotation) when either column is a
+// numerical zero or |g| <= tol ||a|| ||b||.  c^2 + s^2 and |e^{i phi}| equal 1 to rounding.
+__device__ __forceinline__ bool jacobi_rotation(double al, double be, double gr, double gi,
+                                                double zero_sq, double tol, double& c,
+                                                double& s, double& er, double& ei) {
+    const double g2 = fma(gr, gr, gi * gi);
+    if (!(al > zero_sq && be > zero_sq && g2 > tol * tol * al * be)) return false;
+    const double rg = rsq_nr(g2);                    // 1 / |g|
+    const double zeta = 0.5 * (be - al) * rg;
+    double t;
+    if (fabs(zeta) > 1e150) {
+        t = 0.5 * rcp_nr(zeta);
+    } else {
+        const double q = fma(zeta, zeta, 1.0);
+        const double root = q * rsq_nr(q);
+        t = rcp_nr(fabs(zeta) + root);
+        if (zeta < 0) t = -t;
+    }
+    c = rsq_nr(fma(t, t, 1.0));
+    s = c * t;
+    er = gr * rg;
+    ei = gi * rg;
+    return true;
+}
+
 __device__ __forceinline__ double team_sum(double v) {
     v += __shfl_xor(v, 8, kTeamLanes);
     v += __shfl_xor(v, 4, kTeamLanes);
@@ -346,18 +394,8 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
                 be = team_sum(be);
                 gr = team_sum(gr);
                 gi = team_sum(gi);
-                const double g = sqrt(gr * gr + gi * gi);
-                if (al > zero_sq && be > zero_sq && g > tol * sqrt(al) * sqrt(be)) {
-                    const double zeta = (be - al) / (2.0 * g);
-                    double t;
-                    if (fabs(zeta) > 1e150) {
-                        t = 0.5 / zeta;
-                    } else {
-                        t = 1.0 / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                        if (zeta < 0) t = -t;
-                    }
-                    const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-                    const double er = gr / g, ei = gi / g;          // e^{i phi}
+                double c, s, er, ei;
+                if (jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, s, er, ei)) {
                     // a' = c a - s e^{-i phi} b ;  b' = s e^{i phi} a + c b
                     const cd2 sem = mk2(s * er, -s * ei);
                     const cd2 sep = mk2(s * er, s * ei);
@@ -658,18 +696,8 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
                 be = big_sum(be);
                 gr = big_sum(gr);
                 gi = big_sum(gi);
-                const double g = sqrt(gr * gr + gi * gi);
-                if (al > zero_sq && be > zero_sq && g > tol * sqrt(al) * sqrt(be)) {
-                    const double zeta = (be - al) / (2.0 * g);
-                    double t;
-                    if (fabs(zeta) > 1e150) {
-                        t = 0.5 / zeta;
-                    } else {
-                        t = 1.0 / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                        if (zeta < 0) t = -t;
-                    }
-                    const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-                    const double er = gr / g, ei = gi / g;
+                double c, s, er, ei;
+                if (jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, s, er, ei)) {
                     const cd2 sem = mk2(s * er, -s * ei);
                     const cd2 sep = mk2(s * er, s * ei);
 #pragma unroll
