@@ -108,7 +108,6 @@ int run_plan(double2* state, int nq, const SvPlan& plan, hipStream_t stream) {
     put(L.phases, plan.phases.data(), plan.phases.size() * sizeof(DevPhase));
     put(L.terms, plan.terms.data(), plan.terms.size() * sizeof(DevTerm));
     put(L.tfactors, plan.thread_factors.data(), plan.thread_factors.size() * sizeof(double));
-    put(L.tmasks, plan.thread_masks.data(), plan.thread_masks.size() * sizeof(uint32_t));
     void* dev = nullptr;
     QS_HIP(hipMallocAsync(&dev, L.total, stream));
     // Pageable source: the runtime stages the bytes before returning.
@@ -271,6 +270,10 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
                 if (G.kind == DG_PHASE || G.kind == DG_PHASE_SIGN) {
                     const DevPhase& Q = plan.phases[P.phase0 + G.qa];
                     os << ",\"phase\":{\"sign\":" << Q.sign << ",\"slot\":" << Q.slot
+                       << ",\"m0\":" << Q.m0 << ",\"mb\":[";
+                    for (int k = 0; k < SV_TILE_BITS - SV_REG_BITS; ++k)
+                        os << (k ? "," : "") << Q.mb[k];
+                    os << "],\"pp\":[" << Q.pp[0] << "," << Q.pp[1] << "]"
                        << ",\"c\":[" << Q.c[0] << "," << Q.c[1] << "],\"kr\":[";
                     for (int k = 0; k < 4; ++k)
                         os << (k ? "," : "") << Q.kr[k][0] << "," << Q.kr[k][1];

@@ -65,9 +65,13 @@ constexpr int SV_LOW_BITS = 4;         // qubits 0..3 are in every tile (256 B r
 //   DG_PHASE / DG_PHASE_SIGN  unitary diagonal group, DevPhase record qa; the SIGN form
 //             has only +-1 factors
 //   DG_LAYOUT transpose through LDS to the DevLayout stored at the start of m
+//   DG_ROT    scaled real rotation on register bit r0, t = m[0]: r1 bit 0 clear maps a
+//             pair (x0, x1) to (x0 - t x1, x1 + t x0), set to (t x0 - x1, x0 + t x1);
+//             r1 bit 1 set multiplies the second output by k = (m[2], m[3])
+//   DG_SCALE  every amplitude times (m[0], m[1]); r0 = 1 when the factor is real
 enum : int32_t {
     DG_U1 = 0, DG_U2 = 1, DG_DIAG1 = 2, DG_DIAG2 = 3, DG_U1X = 4, DG_PHASE = 5,
-    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8
+    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8, DG_ROT = 9, DG_SCALE = 10
 };
 
 struct alignas(16) DevGate {
@@ -123,6 +127,13 @@ struct alignas(16) DevPhase {
     int32_t sign;          // 1: every factor is +1 or -1 (parity arithmetic)
     int32_t tab;           // first entry of this group's per-thread table
     int32_t pad;
+    // SIGN groups: the per-thread mask as the kernel rebuilds it, with t the thread
+    // index: m0 ^ (XOR of mb[b] over set bits b of t) ^ (0xFFFF if the parity of
+    // t_a t_b over the thread-pair terms is odd); bit b' of byte a of pp (pp[a / 4],
+    // byte a % 4) marks the pair (a, b'), a < b'.
+    uint32_t m0;
+    uint32_t mb[SV_TILE_BITS - SV_REG_BITS];
+    uint32_t pp[2];
     double c[2];           // constant factor
     double kr[4][2];       // constant per-register-bit factors
     double crr[6][2];      // register-pair factors: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
@@ -150,6 +161,12 @@ struct DevPass {
     int32_t nop;               // ops of this pass
     int32_t nbq;               // block qubits (not in the tile): nq - K, or 0
     int32_t bq[SV_MAX_QUBITS]; // block qubits ascending: bit i of the tile index -> bq[i]
+    // The same map as runs of consecutive block qubits: the tile's global index bits are
+    // the OR over k of ((tile >> run_src[k]) & run_mask[k]) << run_dst[k]; unused runs
+    // have mask 0.
+    int32_t run_src[SV_TILE_BITS + 1];
+    int32_t run_dst[SV_TILE_BITS + 1];
+    uint32_t run_mask[SV_TILE_BITS + 1];
     DevLayout load;            // HBM load layout (the store layout is the last one set)
 };
 
@@ -168,7 +185,8 @@ struct SvPlan {
     std::vector<DevPhase> phases;
     std::vector<DevTerm> terms;
     std::vector<double> thread_factors;   // complex groups: 2 * PHASE_THREAD_VALUES per thread
-    std::vector<uint32_t> thread_masks;   // sign groups: one mask per thread
+    std::vector<uint32_t> thread_masks;   // sign groups: one mask per thread (host copy
+                                          // for plan inspection; the kernel rebuilds it)
     int nslot_total = 0;       // block-factor slots over all passes
 };
 
@@ -178,7 +196,7 @@ struct SvPlan {
 inline size_t sv_blob_offset(size_t bytes) { return ((bytes + 255) / 256) * 256; }
 
 struct SvBlobLayout {
-    size_t passes, stages, gates, phases, terms, tfactors, tmasks, factors, total;
+    size_t passes, stages, gates, phases, terms, tfactors, factors, total;
 };
 SvBlobLayout sv_blob_layout(const SvPlan& plan, int nq);
 

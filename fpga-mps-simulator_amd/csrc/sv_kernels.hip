@@ -29,6 +29,11 @@ __device__ __forceinline__ cd2 cmadd(cd2 acc, cd2 a, cd2 b) {
 
 __device__ __forceinline__ int lds_slot(int l) { return l + (l >> 4); }
 
+// Pointer into memory that no thread writes during the launch (address space 4): uniform
+// loads through it become scalar loads.  Through a plain pointer the compiler may keep
+// them on the vector path, where they queue behind the tile's HBM loads and stores.
+typedef const cd2 __attribute__((address_space(4)))* const_cd2_ptr;
+
 constexpr int kLdsSlots = (1 << SV_TILE_BITS) + ((1 << SV_TILE_BITS) >> 4);
 
 // Addressing under one register layout: local (LDS) index and the thread's part of
@@ -75,11 +80,23 @@ __device__ __forceinline__ void make_layout(Layout<R>& L, const DevLayout& D, in
     L.glob_t = g;
 }
 
+// Leading part of a DevGate record (kind through m[3]).
+struct alignas(16) OpHead {
+    int32_t kind, r0, r1, qa, qb, pad;
+    uint64_t cmask;
+    double m[4];
+};
+static_assert(sizeof(OpHead) == 64 && offsetof(DevGate, m) == offsetof(OpHead, m),
+              "OpHead must be the leading part of DevGate");
+
 // Global index bits of tile `tile`: bit i of the tile index lands on block qubit bq[i].
+// Unrolled over the pass's runs of consecutive block qubits so that the run table
+// arrives in one batch of scalar loads.
 __device__ __forceinline__ uint64_t tile_base(const DevPass* P, uint32_t tile) {
     uint64_t base = 0;
-    const int nb = P->nbq;
-    for (int i = 0; i < nb; ++i) base |= (uint64_t)((tile >> i) & 1u) << P->bq[i];
+#pragma unroll
+    for (int k = 0; k < SV_TILE_BITS + 1; ++k)
+        base |= (uint64_t)((tile >> P->run_src[k]) & P->run_mask[k]) << P->run_dst[k];
     return base;
 }
 
@@ -159,6 +176,40 @@ __device__ __forceinline__ void apply_u2_bits(cd2 (&a)[1 << R], const DevGate& G
     }
 }
 
+// a * b + c into a register of its own (three-address v_fma_f64).  Written as plain
+// fma, the compiler picks the tied two-address v_fmac_f64 and then copies c when c is
+// still needed, which costs a 64-bit move per result.  a is uniform (an SGPR).
+__device__ __forceinline__ double fma3(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "s"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Scaled rotation on register bit B (DG_ROT).  Form A (FB false) maps a pair to
+// (x0 - t x1, x1 + t x0), form B to (t x0 - x1, x0 + t x1); with DIAG the second
+// output is multiplied by k.  Four FMAs per pair, plus one complex product with DIAG.
+template <int R, int B, bool FB, bool DIAG>
+__device__ __forceinline__ void rot_bit(cd2 (&a)[1 << R], double t, cd2 k) {
+    int npair = 0;
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        if ((j >> B) & 1) continue;           // compile-time after unrolling
+        const cd2 x0 = a[j], x1 = a[j | (1 << B)];
+        cd2 y0, y1;
+        if constexpr (!FB) {
+            y0 = mk2(fma(-t, x1.x, x0.x), fma(-t, x1.y, x0.y));
+            y1 = mk2(fma3(t, x0.x, x1.x), fma3(t, x0.y, x1.y));
+        } else {
+            y0 = mk2(fma(t, x0.x, -x1.x), fma(t, x0.y, -x1.y));
+            y1 = mk2(fma3(t, x1.x, x0.x), fma3(t, x1.y, x0.y));
+        }
+        if constexpr (DIAG) y1 = cmul(k, y1);
+        a[j] = y0;
+        a[j | (1 << B)] = y1;
+        if (++npair % 4 == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // Ops that differ only in which registers they touch are dispatched by a switch whose
 // case bodies are bracketed by distinct empty asm statements: without them the
 // optimizer merges the isomorphic bodies behind pointer selects, which moves the
@@ -175,6 +226,37 @@ __device__ __forceinline__ void apply_u1s(cd2 (&a)[1 << R], const DevGate& G) {
         default: if constexpr (R > 3) { QS_CASE("u1s 3 <"); u1_bit<R, S3>(a, G.m); QS_CASE("u1s 3 >"); } break;
     }
 }
+
+#define QS_ROT_CASES(B, TAG)                                                              \
+    case 4 * B + 0: QS_CASE("rot " TAG "a <"); rot_bit<R, B, false, false>(a, t, k); QS_CASE("rot " TAG "a >"); break; \
+    case 4 * B + 1: QS_CASE("rot " TAG "b <"); rot_bit<R, B, true, false>(a, t, k); QS_CASE("rot " TAG "b >"); break;  \
+    case 4 * B + 2: QS_CASE("rot " TAG "c <"); rot_bit<R, B, false, true>(a, t, k); QS_CASE("rot " TAG "c >"); break;  \
+    case 4 * B + 3: QS_CASE("rot " TAG "d <"); rot_bit<R, B, true, true>(a, t, k); QS_CASE("rot " TAG "d >"); break;
+
+template <int R>
+__device__ __forceinline__ void apply_rot(cd2 (&a)[1 << R], const DevGate& G, int r0, int r1,
+                                          double t, cd2 k) {
+    if constexpr (R == 4) {
+        switch (r0 * 4 + r1) {
+            QS_ROT_CASES(0, "0") QS_ROT_CASES(1, "1") QS_ROT_CASES(2, "2") QS_ROT_CASES(3, "3")
+            default: break;
+        }
+    } else {
+        // narrow tiles (fewer than four register bits): general 2x2 on the bit
+        double m[8] = {1.0, 0.0, -t, 0.0, t, 0.0, 1.0, 0.0};
+        if (r1 & 1) {
+            m[0] = t; m[2] = -1.0; m[4] = 1.0; m[6] = t;
+        }
+        if (r1 & 2) {
+            const cd2 r0 = cmul(k, mk2(m[4], m[5])), r1 = cmul(k, mk2(m[6], m[7]));
+            m[4] = r0.x; m[5] = r0.y; m[6] = r1.x; m[7] = r1.y;
+        }
+        DevGate H = G;
+        for (int i = 0; i < 8; ++i) H.m[i] = m[i];
+        apply_u1s<R>(a, H);
+    }
+}
+#undef QS_ROT_CASES
 
 template <int R>
 __device__ __forceinline__ void apply_cu1(cd2 (&a)[1 << R], const DevGate& G,
@@ -243,11 +325,26 @@ __device__ __forceinline__ constexpr unsigned reg_col_mask(int r) {
 // (lambda_q applies when thread bit q is set, kappa_r to register bit r).
 
 // DG_PHASE_SIGN: every factor is +-1; the group is a 16-bit mask of the register
-// indices to negate.
-// `M` is this thread's table entry.
+// indices to negate.  Thread t's mask is rebuilt from the group's descriptor (uniform,
+// scalar loads) rather than read from a per-thread table: a vector load here would
+// queue behind the tile's HBM traffic.
+__device__ __forceinline__ unsigned sign_mask(const DevPhase& Q, int t) {
+    unsigned M = Q.m0;
+#pragma unroll
+    for (int b = 0; b < SV_TILE_BITS - SV_REG_BITS; ++b)
+        M ^= ((t >> b) & 1) ? Q.mb[b] : 0u;
+    unsigned par = 0;
+#pragma unroll
+    for (int a = 0; a < SV_TILE_BITS - SV_REG_BITS - 1; ++a) {
+        const unsigned row = (Q.pp[a >> 2] >> (8 * (a & 3))) & 0xFFu;
+        par ^= (unsigned)(t >> a) & (unsigned)__builtin_popcount(row & (unsigned)t);
+    }
+    return M ^ ((par & 1u) ? 0xFFFFu : 0u);
+}
+
 template <int R>
 __device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], unsigned M,
-                                                 const cd2* __restrict__ bf, int t) {
+                                                 const_cd2_ptr bf, int t) {
     if (bf) {
         // value 0 of a sign group: A_B and the packed negative lambda / kappa bits
         const cd2 b0 = bf[0];
@@ -273,7 +370,7 @@ __device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], unsigned M,
 template <int R>
 __device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const DevPhase& Q,
                                                  const cd2 (&e)[PHASE_THREAD_VALUES],
-                                                 const cd2* __restrict__ bf, int t) {
+                                                 const_cd2_ptr bf, int t) {
     cd2 A = e[0], k0 = e[1], k1 = e[2], k2 = e[3], k3 = e[4];
     if (bf) {
         A = cmul(A, bf[0]);
@@ -372,41 +469,42 @@ __global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
     for (int k = 0; k < PHASE_BLOCK_VALUES; ++k) out[k] = v[k];
 }
 
-// One fused pass: workgroup `tile` loads its 2^K amplitudes, runs the pass's op stream
-// on them in registers (LDS only for layout changes) and stores them back.
+// LDS barrier for the tile transposes: waits for this wave's LDS accesses, then syncs
+// the workgroup.  Unlike __syncthreads() it carries no memory fence, so it never waits
+// for global memory operations in flight; the memory clobber keeps the compiler from
+// moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Runs a pass's op stream on one tile's register block.  `lds` is the tile's transpose
+// buffer (kLdsSlots slots); `base` the tile's global index bits.
 template <int R>
-__global__ void __launch_bounds__(1 << (SV_TILE_BITS - SV_REG_BITS))
-sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
-               const DevGate* __restrict__ gates, const DevPhase* __restrict__ phases,
-               const cd2* __restrict__ tfactors, const uint32_t* __restrict__ tmasks,
-               const cd2* __restrict__ factors, uint32_t ntiles) {
+__device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
+                                        const DevGate* __restrict__ gt, int nop,
+                                        const DevPhase* __restrict__ ph,
+                                        const cd2* __restrict__ tfactors,
+                                        const cd2* __restrict__ tile_factors, uint32_t ntiles,
+                                        uint64_t base, cd2* __restrict__ lds, int t) {
     constexpr int NR = 1 << R;
-    __shared__ cd2 lds[kLdsSlots];
-
-    const int t = threadIdx.x;
-    const uint32_t tile = blockIdx.x;
-    const int nop = P->nop;
-    const DevGate* gt = gates + P->gate0;
-    const DevPhase* ph = phases + P->phase0;
-    const cd2* tile_factors = factors + ((size_t)P->slot0 * ntiles + tile) * PHASE_BLOCK_VALUES;
-    const uint64_t base = tile_base(P, tile);
-
-    // The planner picks an HBM load layout whose thread bits cover qubits 0..3 (16
-    // lanes walk 256 consecutive bytes) and ends the op stream in such a layout.
-    Layout<R> Lcur;
-    make_layout<R>(Lcur, P->load, t);
-    cd2 a[NR];
-#pragma unroll
-    for (int j = 0; j < NR; ++j) a[j] = state[base | Lcur.glob(j)];
-
     // Every op is straight-line code on the register block.  The dispatch is unrolled
     // twice: an op writes its results to fresh registers while its inputs are live, so
     // two ops per trip let the results ping-pong between two register sets instead of
     // being copied back at every loop latch.
     auto run_op = [&](int gi) __attribute__((always_inline)) {
         const DevGate& G = gt[gi];
-        const int kind = G.kind;
-        if (kind == DG_U1X) {
+        // The first 64 bytes of the record in one batch of scalar loads: the frequent
+        // ops (DG_ROT, the phase groups, DG_SCALE) need nothing else, so they cost one
+        // scalar-load round trip instead of one for the kind and one for the operands.
+        const OpHead H = *reinterpret_cast<const OpHead*>(&G);
+        const int kind = H.kind, r0 = H.r0, r1 = H.r1, qa = H.qa;
+        const double m0 = H.m[0], m1 = H.m[1], m2 = H.m[2], m3 = H.m[3];
+        // consumed here, so all of them are loaded (and waited for) before the branch
+        asm volatile("" ::"s"(kind), "s"(r0), "s"(r1), "s"(qa), "s"(m0), "s"(m1), "s"(m2),
+                     "s"(m3));
+        if (kind == DG_ROT) {
+            apply_rot<R>(a, G, r0, r1, m0, mk2(m2, m3));
+        } else if (kind == DG_U1X) {
             u1_bit<R, 0>(a, G.m);
             if constexpr (R > 1) u1_bit<R, (R > 1 ? 1 : 0)>(a, G.m + 8);
             if constexpr (R > 2) u1_bit<R, (R > 2 ? 2 : 0)>(a, G.m + 16);
@@ -414,12 +512,13 @@ sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
         } else if (kind == DG_U1S) {
             apply_u1s<R>(a, G);
         } else if (kind == DG_PHASE_SIGN || kind == DG_PHASE) {
-            const DevPhase& Q = ph[G.qa];
-            const cd2* bf = Q.slot >= 0
-                                ? tile_factors + (size_t)Q.slot * ntiles * PHASE_BLOCK_VALUES
-                                : nullptr;
+            const DevPhase& Q = ph[qa];
+            const const_cd2_ptr bf =
+                Q.slot >= 0 ? (const_cd2_ptr)(tile_factors +
+                                              (size_t)Q.slot * ntiles * PHASE_BLOCK_VALUES)
+                            : nullptr;
             if (kind == DG_PHASE_SIGN) {
-                apply_phase_sign<R>(a, tmasks[Q.tab + t], bf, t);
+                apply_phase_sign<R>(a, sign_mask(Q, t), bf, t);
             } else {
                 const cd2* e = tfactors + (size_t)(Q.tab + t) * PHASE_THREAD_VALUES;
                 const cd2 f[PHASE_THREAD_VALUES] = {e[0], e[1], e[2], e[3], e[4]};
@@ -428,15 +527,24 @@ sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
         } else if (kind == DG_LAYOUT) {
 #pragma unroll
             for (int j = 0; j < NR; ++j) lds[lds_slot(Lcur.loc(j))] = a[j];
-            __syncthreads();
+            lds_barrier();
             make_layout<R>(Lcur, *reinterpret_cast<const DevLayout*>(G.m), t);
 #pragma unroll
             for (int j = 0; j < NR; ++j) a[j] = lds[lds_slot(Lcur.loc(j))];
-            __syncthreads();
+            lds_barrier();
         } else if (kind == DG_U1) {
             apply_cu1<R>(a, G, Lcur, base);
         } else if (kind == DG_U2) {
             apply_u2<R>(a, G, Lcur, base);
+        } else if (kind == DG_SCALE) {
+            const cd2 f = mk2(m0, m1);
+            if (r0) {
+#pragma unroll
+                for (int j = 0; j < NR; ++j) a[j] = a[j] * f.x;
+            } else {
+#pragma unroll
+                for (int j = 0; j < NR; ++j) a[j] = cmul(f, a[j]);
+            }
         } else {
             apply_diag<R>(a, G, Lcur, base);
         }
@@ -445,9 +553,36 @@ sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
         run_op(gi);
         if (gi + 1 < nop) run_op(gi + 1);
     }
+}
 
+// One fused pass, one tile per workgroup: workgroup `tile` loads its 2^K amplitudes,
+// runs the pass's op stream on them in registers (LDS only for layout changes) and
+// stores them back.
+template <int R>
+__global__ void __launch_bounds__(1 << (SV_TILE_BITS - SV_REG_BITS))
+sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
+               const DevGate* __restrict__ gates, const DevPhase* __restrict__ phases,
+               const cd2* __restrict__ tfactors, const cd2* __restrict__ factors,
+               uint32_t ntiles) {
+    constexpr int NR = 1 << R;
+    __shared__ cd2 lds[kLdsSlots];
+    const int t = threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    const cd2* tile_factors = factors + ((size_t)P->slot0 * ntiles + tile) * PHASE_BLOCK_VALUES;
+    const uint64_t base = tile_base(P, tile);
+    // The planner picks an HBM load layout whose thread bits cover qubits 0..3 (16
+    // lanes walk 256 consecutive bytes) and ends the op stream in such a layout.
+    Layout<R> Lcur;
+    make_layout<R>(Lcur, P->load, t);
+    // The state streams through once per pass: nontemporal loads and stores keep it
+    // from displacing the op records and phase tables in the caches.
+    cd2 a[NR];
 #pragma unroll
-    for (int j = 0; j < NR; ++j) state[base | Lcur.glob(j)] = a[j];
+    for (int j = 0; j < NR; ++j) a[j] = __builtin_nontemporal_load(state + (base | Lcur.glob(j)));
+    run_ops<R>(a, Lcur, gates + P->gate0, P->nop, phases + P->phase0, tfactors, tile_factors,
+               ntiles, base, lds, t);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) __builtin_nontemporal_store(a[j], state + (base | Lcur.glob(j)));
 }
 
 __global__ void sv_fill_zero_kernel(cd2* __restrict__ state, uint64_t n) {
@@ -537,7 +672,6 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
     const DevPhase* dphases = reinterpret_cast<const DevPhase*>(blob + L.phases);
     const DevTerm* dterms = reinterpret_cast<const DevTerm*>(blob + L.terms);
     const cd2* dtf = reinterpret_cast<const cd2*>(blob + L.tfactors);
-    const uint32_t* dtm = reinterpret_cast<const uint32_t*>(blob + L.tmasks);
     cd2* dfactors = reinterpret_cast<cd2*>(const_cast<char*>(blob) + L.factors);
     cd2* st = reinterpret_cast<cd2*>(state);
     for (size_t p = 0; p < plan.passes.size(); ++p) {
@@ -561,13 +695,13 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
         }
         switch (P.R) {
             case 1: hipLaunchKernelGGL(sv_pass_kernel<1>, dim3(grid), dim3(threads), 0, stream,
-                                       st, dP, dgates, dphases, dtf, dtm, dfactors, tiles); break;
+                                       st, dP, dgates, dphases, dtf, dfactors, tiles); break;
             case 2: hipLaunchKernelGGL(sv_pass_kernel<2>, dim3(grid), dim3(threads), 0, stream,
-                                       st, dP, dgates, dphases, dtf, dtm, dfactors, tiles); break;
+                                       st, dP, dgates, dphases, dtf, dfactors, tiles); break;
             case 3: hipLaunchKernelGGL(sv_pass_kernel<3>, dim3(grid), dim3(threads), 0, stream,
-                                       st, dP, dgates, dphases, dtf, dtm, dfactors, tiles); break;
+                                       st, dP, dgates, dphases, dtf, dfactors, tiles); break;
             default: hipLaunchKernelGGL(sv_pass_kernel<4>, dim3(grid), dim3(threads), 0, stream,
-                                        st, dP, dgates, dphases, dtf, dtm, dfactors, tiles); break;
+                                        st, dP, dgates, dphases, dtf, dfactors, tiles); break;
         }
         QS_HIP(hipGetLastError());
         if (e1) QS_HIP(hipEventRecord(e1, stream));

@@ -21,11 +21,19 @@ namespace {
 
 inline int popcount64(uint64_t x) { return __builtin_popcountll(x); }
 
+typedef std::complex<double> cd;
+
 struct Item {
     HostGate g;
     bool diag;
     uint64_t touch;     // every qubit the gate reads: targets, diag qubits, controls
     uint64_t targets;   // qubits that must be tile qubits (non-diagonal targets)
+    // Uncontrolled unitary one-qubit gates run as scaled rotations (split_rotation):
+    // rot 1 or 2 is the form, t its parameter, post the factor on the |1> output and
+    // scale the constant left over (collected per pass into one DG_SCALE op).
+    int rot;
+    double t;
+    cd post, scale;
 };
 
 void mat2_mul(const double* b, const double* a, double* out) {  // out = b @ a (2x2)
@@ -74,8 +82,39 @@ int local_pos(const int* tq, int K, int q) {
     return -1;
 }
 
+// One-qubit unitary U (row-major complex m[0..7]) written as
+//   U = scale * diag(1, post) * Rn * diag(1, pre),
+//   form 1 (|U00| >= |U10|): Rn = [[1, -t], [t, 1]], t = |U10|/|U00|, scale = g |U00|
+//   form 2:                   Rn = [[t, -1], [1, t]], t = |U00|/|U10|, scale = g |U10|
+// with g = U00/|U00| (U10/|U10| when U00 = 0), post = U10/(g |U10|) and
+// pre = -U01/(g |U10|).  U00, U01 and U10 are reproduced to rounding by construction;
+// U11 is compared, so a matrix that is not unitary to 1e-14 is refused (false), as is a
+// diagonal one (U10 = 0), which has no rotation.  Rn costs two FMAs per output pair
+// component instead of the eight of a complex 2x2 product, and 0 <= t <= 1.
+bool split_rotation(const double* m, int& form, double& t, cd& pre, cd& post, cd& scale) {
+    const cd u00(m[0], m[1]), u01(m[2], m[3]), u10(m[4], m[5]), u11(m[6], m[7]);
+    const double c = std::abs(u00), s = std::abs(u10);
+    if (!(s > 0.0) || !std::isfinite(c) || !std::isfinite(s)) return false;
+    const cd g = c > 0.0 ? u00 / c : u10 / s;
+    post = u10 / (g * s);
+    pre = -u01 / (g * s);
+    const cd r11 = g * post * pre * c;
+    if (std::abs(r11 - u11) > 1e-14 || std::fabs(c * c + s * s - 1.0) > 1e-14 ||
+        std::fabs(std::abs(pre) - 1.0) > 1e-14 || std::fabs(std::abs(post) - 1.0) > 1e-14)
+        return false;
+    if (c >= s) {
+        form = 1;
+        t = s / c;
+        scale = g * c;
+    } else {
+        form = 2;
+        t = c / s;
+        scale = g * s;
+    }
+    return true;
+}
+
 // ------------------------------------------------------------------ phase groups
-typedef std::complex<double> cd;
 
 struct PTerm {
     int qa, qb;            // global qubits; qb < 0: one-body; qa < 0: constant
@@ -229,15 +268,32 @@ void emit_phase_group(const std::vector<const HostGate*>& diag, const int* regs,
         ph.tab = (int)plan->thread_masks.size();
         static const uint32_t col[4] = {0xAAAAu, 0xCCCCu, 0xF0F0u, 0xFF00u};
         static const int pr[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
-        for (int t = 0; t < nthr; ++t) {
-            cd A, k[4];
-            thread_factors(t, A, k);
+        auto mask_of = [&](const cd& A, const cd* k) {
             uint32_t M = A.real() < 0.0 ? 0xFFFFu : 0u;
             for (int r = 0; r < R; ++r)
                 if (k[r].real() < 0.0) M ^= col[r];
             for (int i = 0; i < 6; ++i)
                 if (pr[i][1] < R && crr[i].real() < 0.0) M ^= col[pr[i][0]] & col[pr[i][1]];
-            plan->thread_masks.push_back(M);
+            return M;
+        };
+        for (int t = 0; t < nthr; ++t) {
+            cd A, k[4];
+            thread_factors(t, A, k);
+            plan->thread_masks.push_back(mask_of(A, k));
+        }
+        // The same masks as a descriptor: thread 0's mask, one XOR mask per thread bit
+        // (one-body terms negate every register index, register x thread terms one
+        // column) and the thread-pair terms.
+        ph.m0 = plan->thread_masks[ph.tab];
+        for (const DevTerm& x : tt) {
+            if (x.cls == PT_T1) {
+                ph.mb[x.a] ^= 0xFFFFu;
+            } else if (x.cls == PT_TR) {
+                ph.mb[x.b] ^= col[x.a];
+            } else if (x.cls == PT_TT) {
+                const int a = std::min(x.a, x.b), b = std::max(x.a, x.b);
+                ph.pp[a / 4] ^= 1u << (8 * (a % 4) + b);
+            }
         }
     } else {
         ph.tab = (int)(plan->thread_factors.size() / (2 * PHASE_THREAD_VALUES));
@@ -492,6 +548,9 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
 
     StageScheduler::State st = S.initial();
     int cur[4] = {0, 0, 0, 0};
+    cd scale = 1.0;
+    for (const Item* it : inc)
+        if (it->rot) scale *= it->scale;
     // Uncontrolled one-qubit gates gathered per register bit until another op is
     // emitted; three or four bits become one DG_U1X, fewer become DG_U1S ops.
     int u1_mask = 0;
@@ -555,6 +614,20 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
     auto emit_gate = [&](int j) {
         const HostGate& g = inc[j]->g;
         const int r0 = reg_index(local_pos(P.tq, K, g.q0));
+        if (inc[j]->rot) {
+            close_u1();
+            DevGate d{};
+            d.kind = DG_ROT;
+            d.r0 = r0;
+            const bool post = inc[j]->post != cd(1.0, 0.0);
+            d.r1 = (inc[j]->rot - 1) | (post ? 2 : 0);
+            d.qa = d.qb = -1;
+            d.m[0] = inc[j]->t;
+            d.m[2] = inc[j]->post.real();
+            d.m[3] = inc[j]->post.imag();
+            plan->gates.push_back(d);
+            return;
+        }
         if (g.kind == QS_GATE_U1 && g.cmask == 0) {
             if ((u1_mask >> r0) & 1)
                 mat2_mul(g.m, u1_m[r0], u1_m[r0]);
@@ -629,6 +702,15 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
         close_u1();
         if (si + 1 == masks.size()) {
             emit_diagonals(S.flush_ready(st));
+            if (scale != cd(1.0, 0.0)) {   // constant factors of this pass's rotations
+                DevGate d{};
+                d.kind = DG_SCALE;
+                d.r0 = scale.imag() == 0.0 ? 1 : 0;
+                d.qa = d.qb = -1;
+                d.m[0] = scale.real();
+                d.m[1] = scale.imag();
+                plan->gates.push_back(d);
+            }
             if ((curmask & lowmask) && curmask != defmask) emit_layout(defmask);
         }
         stg.g1 = (int)plan->gates.size() - P.gate0;
@@ -653,11 +735,16 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
     plan->thread_masks.clear();
     plan->nslot_total = 0;
     std::vector<Item> items;
-    items.reserve(ng);
+    items.reserve(2 * (size_t)ng);
+    // Last non-diagonal item on each qubit: a rotation's input factor diag(1, pre)
+    // commutes with the diagonal gates since then, so it joins that item's output
+    // factor when the item is a rotation on the same qubit.
+    std::vector<int> last_nd(SV_MAX_QUBITS, -1);
     for (int i = 0; i < ng; ++i) {
         int st = check_gate(nq, gates[i], i);
         if (st != QS_OK) return st;
         Item it{};
+        it.post = it.scale = 1.0;
         it.g.kind = gates[i].kind;
         it.g.q0 = gates[i].q[0];
         it.g.q1 = gates[i].q[1];
@@ -669,6 +756,31 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
         if (two) q |= 1ull << it.g.q1;
         it.touch = q | it.g.cmask;
         it.targets = it.diag ? 0 : q;
+        cd pre = 1.0;
+        if (it.g.kind == QS_GATE_U1 && it.g.cmask == 0 &&
+            split_rotation(it.g.m, it.rot, it.t, pre, it.post, it.scale) && pre != cd(1.0, 0.0)) {
+            const int j = last_nd[it.g.q0];
+            if (j >= 0 && items[j].rot) {
+                items[j].post *= pre;
+            } else {
+                Item d{};
+                d.g.kind = QS_GATE_DIAG1;
+                d.g.q0 = it.g.q0;
+                d.g.q1 = 0;
+                d.g.cmask = 0;
+                d.g.m[0] = 1.0;
+                d.g.m[2] = pre.real();
+                d.g.m[3] = pre.imag();
+                d.diag = true;
+                d.touch = q;
+                d.targets = 0;
+                d.post = d.scale = 1.0;
+                items.push_back(d);
+            }
+        }
+        if (!it.diag)
+            for (int b = 0; b < nq; ++b)
+                if ((it.touch >> b) & 1ull) last_nd[b] = (int)items.size();
         items.push_back(it);
     }
 
@@ -721,6 +833,18 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
         if (!P.tile_is_all)
             for (int q = 0; q < nq; ++q)
                 if (!((tile >> q) & 1)) P.bq[P.nbq++] = q;
+        {
+            int nrun = 0;
+            for (int i = 0; i < P.nbq;) {
+                int j = i;
+                while (j + 1 < P.nbq && P.bq[j + 1] == P.bq[j] + 1) ++j;
+                P.run_src[nrun] = i;
+                P.run_dst[nrun] = P.bq[i];
+                P.run_mask[nrun] = (uint32_t)((1ull << (j - i + 1)) - 1);
+                ++nrun;
+                i = j + 1;
+            }
+        }
         build_stages(inc, P, plan);
         plan->passes.push_back(P);
         for (size_t i : inc_idx) done[i] = 1;
@@ -769,8 +893,6 @@ SvBlobLayout sv_blob_layout(const SvPlan& plan, int nq) {
     off += sv_blob_offset(std::max<size_t>(plan.terms.size(), 1) * sizeof(DevTerm));
     L.tfactors = off;
     off += sv_blob_offset(std::max<size_t>(plan.thread_factors.size(), 1) * sizeof(double));
-    L.tmasks = off;
-    off += sv_blob_offset(std::max<size_t>(plan.thread_masks.size(), 1) * sizeof(uint32_t));
     L.factors = off;
     size_t nblocks = 1;
     if (!plan.passes.empty() && nq > plan.passes[0].K) nblocks = (size_t)1 << (nq - plan.passes[0].K);

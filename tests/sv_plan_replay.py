@@ -8,7 +8,8 @@ target outside the tile) shows up as a mismatch against the oracle.
 
 import numpy as np
 
-DG_U1, DG_U2, DG_DIAG1, DG_DIAG2, DG_U1X, DG_PHASE, DG_LAYOUT, DG_PHASE_SIGN, DG_U1S = 0, 1, 2, 3, 4, 5, 6, 7, 8
+(DG_U1, DG_U2, DG_DIAG1, DG_DIAG2, DG_U1X, DG_PHASE, DG_LAYOUT, DG_PHASE_SIGN, DG_U1S, DG_ROT,
+ DG_SCALE) = range(11)
 PT_BB, PT_BT, PT_BR, PT_T1, PT_TT, PT_TR = 0, 1, 2, 3, 4, 5
 PAIRS = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
 
@@ -69,6 +70,21 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                         if not (g['r0'] >> b) & 1:
                             assert np.array_equal(mb, np.eye(2))
                         _apply_1q(st, idx, sel, tq[regs[b]], mb)
+                elif g['kind'] == DG_ROT:
+                    # scaled rotation: form A [[1, -t], [t, 1]], form B [[t, -1], [1, t]];
+                    # r1 bit 1: second row times k
+                    assert cm == 0 and 0 <= g['r0'] < R
+                    t = g['m'][0]
+                    m = (np.array([[t, -1], [1, t]], dtype=complex) if g['r1'] & 1
+                         else np.array([[1, -t], [t, 1]], dtype=complex))
+                    assert 0.0 <= t <= 1.0
+                    if g['r1'] & 2:
+                        m[1] *= g['m'][2] + 1j * g['m'][3]
+                    _apply_1q(st, idx, sel, tq[regs[g['r0']]], m)
+                elif g['kind'] == DG_SCALE:
+                    f = g['m'][0] + 1j * g['m'][1]
+                    assert (g['r0'] == 1) == (g['m'][1] == 0.0)
+                    st *= f
                 elif g['kind'] == DG_U1S:
                     assert cm == 0 and 0 <= g['r0'] < R
                     _apply_1q(st, idx, sel, tq[regs[g['r0']]], _cm(g['m'], 4).reshape(2, 2))
@@ -96,10 +112,10 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                 else:
                     raise AssertionError(f"unknown kind {g['kind']}")
                 # every non-diagonal target must be a tile qubit
-                if g['kind'] in (DG_U1, DG_U2, DG_U1S):
+                if g['kind'] in (DG_U1, DG_U2, DG_U1S, DG_ROT):
                     assert (tile >> tq[regs[g['r0']]]) & 1
                 if g['kind'] == DG_U1:
-                    assert cm != 0, 'uncontrolled one-qubit gates travel as U1X'
+                    assert cm != 0, 'uncontrolled one-qubit gates travel as U1X, U1S or ROT'
         assert all(r >= low for r in layout) or layout == default, layout
     return st
 
@@ -128,12 +144,28 @@ def _thread_and_reg_index(tq, regs, idx):
     return t, j, thread_pos
 
 
+def _sign_mask_from_descriptor(ph, t):
+    """Thread t's negation mask as sv_pass_kernel's sign_mask() computes it."""
+    M = int(ph['m0'])
+    for b, mb in enumerate(ph['mb']):
+        if (t >> b) & 1:
+            M ^= int(mb)
+    par = 0
+    for a in range(len(ph['mb']) - 1):
+        row = (int(ph['pp'][a // 4]) >> (8 * (a % 4))) & 0xFF
+        par ^= ((t >> a) & 1) & bin(row & t).count('1')
+    return M ^ (0xFFFF if par & 1 else 0)
+
+
 def _phase_factor_table(ph, tq, regs, idx):
     t, j, thread_pos = _thread_and_reg_index(tq, regs, idx)
     bit = lambda q: (idx >> q) & 1
     tab = np.asarray(ph['table'])
     R = len(regs)
     if ph['sign']:
+        # the kernel rebuilds the table from the group's descriptor
+        for tt in range(tab.size):
+            assert _sign_mask_from_descriptor(ph, tt) == int(tab[tt]), (tt, ph['m0'])
         M = tab.astype(np.int64)[t]
         f = np.where((M >> j) & 1 == 1, -1.0 + 0j, 1.0 + 0j)
     else:
