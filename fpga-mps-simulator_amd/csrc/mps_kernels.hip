@@ -17,6 +17,7 @@
 //                Vh = S^-1 U^H theta is formed from theta in the workspace.
 #include <algorithm>
 #include <cmath>
+#include <atomic>
 #include <utility>
 #include <vector>
 
@@ -28,7 +29,6 @@ namespace {
 
 constexpr int kMaxChi = 128;             // largest chi_max of a task
 constexpr int kRegChi = 64;              // register SVD / MFMA contraction: 2*chi <= 128
-constexpr int kTeamLanes = 16;
 constexpr int kMaxSweeps = 60;
 
 __device__ __forceinline__ cd2 cmul(cd2 a, cd2 b) {
@@ -300,16 +300,19 @@ __device__ __forceinline__ bool jacobi_rotation(double al, double be, double gr,
     return true;
 }
 
+// Sum over the TL lanes of a team (TL a power of two, at most a wavefront).
+template <int TL>
 __device__ __forceinline__ double team_sum(double v) {
-    v += __shfl_xor(v, 8, kTeamLanes);
-    v += __shfl_xor(v, 4, kTeamLanes);
-    v += __shfl_xor(v, 2, kTeamLanes);
-    v += __shfl_xor(v, 1, kTeamLanes);
+#pragma unroll
+    for (int o = TL / 2; o >= 1; o /= 2) v += __shfl_xor(v, o, TL);
     return v;
 }
 
-// Size class CHI (8, 16, 32, 64): theta is at most 2CHI x 2CHI; CHI teams of 16 lanes,
-// CHI/8 rows per lane, LDS exchange buffer of CHI columns of 2CHI rows plus one spare.
+// Size class CHI (8, 16, 32, 64): theta is at most 2CHI x 2CHI; CHI teams of TL lanes
+// (one team per column pair), 2CHI/TL rows per lane, LDS exchange buffer of CHI columns
+// of 2CHI rows plus one spare.  The team width TL is the kernel's tile shape: narrower
+// teams mean fewer threads per task and cheaper reductions, wider ones less work per
+// lane; the launcher takes it from a per-class table (default 16, set by the autotuner).
 template <int CHI>
 struct SvdShared {
     static constexpr int LD = 2 * CHI;
@@ -324,12 +327,13 @@ struct SvdShared {
     double scale;
 };
 
-template <int CHI>
-__global__ void __launch_bounds__(16 * CHI)
+template <int CHI, int TL>
+__global__ void __launch_bounds__(TL * CHI)
 mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
                size_t theta_slot, int32_t* __restrict__ status) {
-    constexpr int NT = 16 * CHI;              // threads
-    constexpr int RPL = CHI / 8;              // rows per lane (16 lanes x RPL = 2 CHI rows)
+    constexpr int NT = TL * CHI;              // threads
+    constexpr int RPL = 2 * CHI / TL;         // rows per lane (TL lanes x RPL = 2 CHI rows)
+    static_assert(RPL >= 1 && RPL * TL == 2 * CHI && TL >= 2 && TL <= 64, "bad team width");
     constexpr int LD = 2 * CHI;               // LDS column stride
     __shared__ SvdShared<CHI> sh;
     const DevTask& DT = tasks[blockIdx.x];
@@ -342,7 +346,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
     const int m = 2 * d.cl, n = 2 * d.cr;
     const int P = n / 2;                         // column pairs = active teams
     const int tid = threadIdx.x;
-    const int team = tid / kTeamLanes, l16 = tid % kTeamLanes;
+    const int team = tid / TL, l16 = tid % TL;
     const bool act = team < P;
     const cd2* th = ws + (size_t)blockIdx.x * theta_slot;
     cd2* slot0 = sh.buf;                     // P columns of 2CHI rows
@@ -351,7 +355,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
     cd2 ca[RPL], cb[RPL];
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
-        const int r = l16 + kTeamLanes * k;
+        const int r = l16 + TL * k;
         ca[k] = (act && r < m) ? th[(size_t)r * n + 2 * team] : mk2(0, 0);
         cb[k] = (act && r < m) ? th[(size_t)r * n + 2 * team + 1] : mk2(0, 0);
     }
@@ -362,7 +366,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
 #pragma unroll
         for (int k = 0; k < RPL; ++k)
             s += ca[k].x * ca[k].x + ca[k].y * ca[k].y + cb[k].x * cb[k].x + cb[k].y * cb[k].y;
-        s = team_sum(s);
+        s = team_sum<TL>(s);
         if (l16 == 0) sh.fro_part[team] = s;
         __syncthreads();
         if (tid == 0) {
@@ -390,10 +394,10 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
                     gr = fma(ca[k].x, cb[k].x, fma(ca[k].y, cb[k].y, gr));
                     gi = fma(ca[k].x, cb[k].y, fma(-ca[k].y, cb[k].x, gi));
                 }
-                al = team_sum(al);
-                be = team_sum(be);
-                gr = team_sum(gr);
-                gi = team_sum(gi);
+                al = team_sum<TL>(al);
+                be = team_sum<TL>(be);
+                gr = team_sum<TL>(gr);
+                gi = team_sum<TL>(gi);
                 double c, s, er, ei;
                 if (jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, s, er, ei)) {
                     // a' = c a - s e^{-i phi} b ;  b' = s e^{i phi} a + c b
@@ -415,7 +419,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
                 if (act) {
 #pragma unroll
                     for (int k = 0; k < RPL; ++k) {
-                        const int r = l16 + kTeamLanes * k;
+                        const int r = l16 + TL * k;
                         slot0[team * LD + r] = cb[k];
                         if (team == 0) spare[r] = cb[k];
                     }
@@ -425,7 +429,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
                     if (team < P - 1) {
 #pragma unroll
                         for (int k = 0; k < RPL; ++k)
-                            cb[k] = slot0[(team + 1) * LD + l16 + kTeamLanes * k];
+                            cb[k] = slot0[(team + 1) * LD + l16 + TL * k];
                     } else {
 #pragma unroll
                         for (int k = 0; k < RPL; ++k) cb[k] = ca[k];
@@ -435,17 +439,17 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
                 if (act) {
 #pragma unroll
                     for (int k = 0; k < RPL; ++k)
-                        slot0[team * LD + l16 + kTeamLanes * k] = ca[k];
+                        slot0[team * LD + l16 + TL * k] = ca[k];
                 }
                 __syncthreads();
                 if (act) {
                     if (team >= 2) {
 #pragma unroll
                         for (int k = 0; k < RPL; ++k)
-                            ca[k] = slot0[(team - 1) * LD + l16 + kTeamLanes * k];
+                            ca[k] = slot0[(team - 1) * LD + l16 + TL * k];
                     } else if (team == 1) {
 #pragma unroll
-                        for (int k = 0; k < RPL; ++k) ca[k] = spare[l16 + kTeamLanes * k];
+                        for (int k = 0; k < RPL; ++k) ca[k] = spare[l16 + TL * k];
                     }
                 }
                 __syncthreads();
@@ -465,8 +469,8 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
         na = fma(ca[k].x, ca[k].x, fma(ca[k].y, ca[k].y, na));
         nb = fma(cb[k].x, cb[k].x, fma(cb[k].y, cb[k].y, nb));
     }
-    na = sqrt(team_sum(na));
-    nb = sqrt(team_sum(nb));
+    na = sqrt(team_sum<TL>(na));
+    nb = sqrt(team_sum<TL>(nb));
     if (act && l16 == 0) {
         sh.sval[2 * team] = na;
         sh.sval[2 * team + 1] = nb;
@@ -531,7 +535,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
         const double ia = na > 0 ? 1.0 / na : 0.0, ib = nb > 0 ? 1.0 / nb : 0.0;
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
-            const int r = l16 + kTeamLanes * k;
+            const int r = l16 + TL * k;
             if (ra < keep) sh.buf[ra * LD + r] = mk2(ca[k].x * ia, ca[k].y * ia);
             if (rb < keep) sh.buf[rb * LD + r] = mk2(cb[k].x * ib, cb[k].y * ib);
         }
@@ -548,22 +552,24 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
     // B' = sqrt(S_t) Vh = sqrt(S_t)/S * U^H theta, (keep, 2, chi_r) = keep x n.
     cd2* Bo = site_ptr(T, T.site + 1);
     {
-        const int c = tid % LD, kg = tid / LD;      // 8 groups of rows of B'
+        constexpr int G = NT / LD;                  // groups of rows of B' (TL / 2)
+        constexpr int QN = CHI / G > 0 ? CHI / G : 1;   // rows per group (keep <= CHI)
+        const int c = tid % LD, kg = tid / LD;
         if (c < n) {
-            cd2 acc[CHI / 8];
+            cd2 acc[QN];
 #pragma unroll
-            for (int q = 0; q < CHI / 8; ++q) acc[q] = mk2(0, 0);
+            for (int q = 0; q < QN; ++q) acc[q] = mk2(0, 0);
             for (int r = 0; r < m; ++r) {
                 const cd2 t = th[(size_t)r * n + c];
 #pragma unroll
-                for (int q = 0; q < CHI / 8; ++q) {
-                    const int k = kg + 8 * q;
+                for (int q = 0; q < QN; ++q) {
+                    const int k = kg + G * q;
                     if (k < keep) acc[q] = cjmadd(acc[q], sh.buf[k * LD + r], t);
                 }
             }
 #pragma unroll
-            for (int q = 0; q < CHI / 8; ++q) {
-                const int k = kg + 8 * q;
+            for (int q = 0; q < QN; ++q) {
+                const int k = kg + G * q;
                 if (k < keep) {
                     const double s = sh.sval[sh.order[k]];
                     const double f = s > 0 ? sqrt(s * scale) / s : 0.0;
@@ -822,22 +828,84 @@ int chi_class(int chi) {
     return chi <= 8 ? 8 : chi <= 16 ? 16 : chi <= 32 ? 32 : chi <= 64 ? 64 : 128;
 }
 
+// Team widths instantiated per register size class (index 0..3 for chi 8, 16, 32, 64):
+// at most 1024 threads (TL * CHI) and at most 16 rows per lane (2 CHI / TL).
+constexpr int kLaneChoices[4][4] = {{4, 8, 16, 0}, {4, 8, 16, 32}, {4, 8, 16, 32}, {8, 16, 0, 0}};
+std::atomic<int> g_svd_lanes[4] = {{16}, {16}, {16}, {16}};
+
+int class_index(int cls) { return cls == 8 ? 0 : cls == 16 ? 1 : cls == 32 ? 2 : cls == 64 ? 3 : -1; }
+
+bool lanes_allowed(int ci, int tl) {
+    for (int c : kLaneChoices[ci])
+        if (c != 0 && c == tl) return true;
+    return false;
+}
+
+#define QS_SVD(C, L)                                                                  \
+    hipLaunchKernelGGL((mps_svd_kernel<C, L>), dim3(n), dim3((L) * (C)), 0, hs, t, theta, \
+                       slot, status)
+
 void launch_svd(int cls, const DevTask* t, int n, const cd2* theta, size_t slot,
                 int32_t* status, hipStream_t hs) {
+    const int ci = class_index(cls);
+    const int tl = ci >= 0 ? g_svd_lanes[ci].load(std::memory_order_relaxed) : 0;
     switch (cls) {
-        case 8: hipLaunchKernelGGL(mps_svd_kernel<8>, dim3(n), dim3(16 * 8), 0, hs, t, theta, slot,
-                                   status); break;
-        case 16: hipLaunchKernelGGL(mps_svd_kernel<16>, dim3(n), dim3(16 * 16), 0, hs, t, theta,
-                                    slot, status); break;
-        case 32: hipLaunchKernelGGL(mps_svd_kernel<32>, dim3(n), dim3(16 * 32), 0, hs, t, theta,
-                                    slot, status); break;
-        case 64: hipLaunchKernelGGL(mps_svd_kernel<64>, dim3(n), dim3(16 * 64), 0, hs, t, theta,
-                                    slot, status); break;
-        default: hipLaunchKernelGGL(mps_svd_big_kernel, dim3(n), dim3(kBigThreads), 0, hs, t,
-                                    const_cast<cd2*>(theta), slot, status); break;
+        case 8:
+            if (tl == 4) QS_SVD(8, 4); else if (tl == 8) QS_SVD(8, 8); else QS_SVD(8, 16);
+            break;
+        case 16:
+            if (tl == 4) QS_SVD(16, 4); else if (tl == 8) QS_SVD(16, 8);
+            else if (tl == 32) QS_SVD(16, 32); else QS_SVD(16, 16);
+            break;
+        case 32:
+            if (tl == 4) QS_SVD(32, 4); else if (tl == 8) QS_SVD(32, 8);
+            else if (tl == 32) QS_SVD(32, 32); else QS_SVD(32, 16);
+            break;
+        case 64:
+            if (tl == 8) QS_SVD(64, 8); else QS_SVD(64, 16);
+            break;
+        default:
+            hipLaunchKernelGGL(mps_svd_big_kernel, dim3(n), dim3(kBigThreads), 0, hs, t,
+                               const_cast<cd2*>(theta), slot, status);
+            break;
     }
 }
+#undef QS_SVD
 }  // namespace
+
+int qs_mps_svd_lane_choices(int chi_class, int* out, int cap, int* count) {
+    clear_error();
+    const int ci = class_index(chi_class);
+    QS_REQUIRE(ci >= 0, QS_E_ARG, "chi class %d has no team-width choices (8, 16, 32 or 64)",
+               chi_class);
+    QS_REQUIRE(count != nullptr && cap >= 0 && (cap == 0 || out != nullptr), QS_E_ARG,
+               "bad output arguments");
+    int k = 0;
+    for (int c : kLaneChoices[ci]) {
+        if (c == 0) continue;
+        if (k < cap) out[k] = c;
+        ++k;
+    }
+    *count = k;
+    return QS_OK;
+}
+
+int qs_mps_svd_lanes(int chi_class) {
+    clear_error();
+    const int ci = class_index(chi_class);
+    QS_REQUIRE(ci >= 0, QS_E_ARG, "chi class %d has no team width", chi_class);
+    return g_svd_lanes[ci].load(std::memory_order_relaxed);
+}
+
+int qs_mps_set_svd_lanes(int chi_class, int lanes) {
+    clear_error();
+    const int ci = class_index(chi_class);
+    QS_REQUIRE(ci >= 0, QS_E_ARG, "chi class %d has no team width", chi_class);
+    QS_REQUIRE(lanes_allowed(ci, lanes), QS_E_ARG, "team width %d not built for chi class %d",
+               lanes, chi_class);
+    g_svd_lanes[ci].store(lanes, std::memory_order_relaxed);
+    return QS_OK;
+}
 
 int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, void* workspace,
                         size_t ws_bytes, int32_t* status_dev, void* stream) {

@@ -83,6 +83,10 @@ _PROTOS = {
     'qs_sv_apply_tfim': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_void_p]),
     'qs_mps_max_chi': (ctypes.c_int, []),
+    'qs_mps_svd_lane_choices': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                               ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    'qs_mps_svd_lanes': (ctypes.c_int, [ctypes.c_int]),
+    'qs_mps_set_svd_lanes': (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     'qs_mps_workspace_bytes': (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
     'qs_mps_update_batch': (ctypes.c_int, [ctypes.POINTER(QsMpsTask), ctypes.c_int,
                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,

@@ -192,6 +192,18 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision,
                         void* workspace, size_t ws_bytes, int32_t* status_dev,
                         void* stream);
 
+/* SVD tile shape, the autotuner's knob (no counterpart in the reference, whose SVD is
+ * LAPACK gesdd on the host).  The register SVD of size class 8, 16, 32 or 64 runs one
+ * team of `lanes` lanes per column pair (lanes * class threads per task, 2 class / lanes
+ * rows per lane).  qs_mps_svd_lane_choices lists the widths built for a class (up to
+ * `cap` into out[], the number in *count); qs_mps_svd_lanes returns the width in use
+ * (default 16), qs_mps_set_svd_lanes selects one (QS_E_ARG if not built).  Every width
+ * computes the same decomposition; the summation order of the column inner products,
+ * and so the last bits of the result, depends on it.  The setting is process-wide. */
+int qs_mps_svd_lane_choices(int chi_class, int* out, int cap, int* count);
+int qs_mps_svd_lanes(int chi_class);
+int qs_mps_set_svd_lanes(int chi_class, int lanes);
+
 /* A'[a, s', b] = sum_s m[s', s] A[a, s, b] on one site, dims read from bond[site],
  * bond[site+1] on the device (fpga_mps_simulator.cpp:232-276, mps_core.py:267-273).
  * m: 2x2 complex row-major, host memory (copied into the launch). */

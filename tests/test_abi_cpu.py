@@ -46,6 +46,29 @@ def test_abi_version_and_limits():
     assert ctypes.sizeof(_lib.QsMpsTask) == 8 * 3 + 8 + 4 + 4 + 8 + 256
 
 
+def test_svd_team_width_table():
+    """The SVD tile-shape table the autotuner drives: choices, set/get, rejection."""
+    from qsim_amd import mps_tune
+    lib = _lib.load_library()
+    expected = {8: [4, 8, 16], 16: [4, 8, 16, 32], 32: [4, 8, 16, 32], 64: [8, 16]}
+    saved = {c: mps_tune.current_lanes(c) for c in expected}
+    try:
+        for c, choices in expected.items():
+            assert mps_tune.lane_choices(c) == choices
+            for w in choices:                       # threads per task = w * c <= 1024
+                assert w * c <= 1024 and 2 * c // w <= 16
+                mps_tune.set_lanes(c, w)
+                assert mps_tune.current_lanes(c) == w
+        assert lib.qs_mps_set_svd_lanes(64, 4) == _lib.QS_E_ARG      # not built
+        assert lib.qs_mps_set_svd_lanes(128, 16) == _lib.QS_E_ARG    # workspace class
+        assert lib.qs_mps_svd_lanes(128) == _lib.QS_E_ARG
+        cnt = ctypes.c_int(-1)
+        assert lib.qs_mps_svd_lane_choices(12, None, 0, ctypes.byref(cnt)) == _lib.QS_E_ARG
+    finally:
+        for c, w in saved.items():
+            mps_tune.set_lanes(c, w)
+
+
 def test_argument_errors_are_reported_without_device():
     lib = _lib.load_library()
     g = (_lib.QsGate * 1)()

@@ -127,6 +127,50 @@ def test_haar_chi128_class_matches_oracle(chi, precision):
     assert np.max(np.abs(sv - rv)) < 1e-10 * np.max(np.abs(rv))
 
 
+@pytest.mark.parametrize('chi', [8, 16, 32, 64])
+def test_every_svd_team_width_matches_oracle(chi):
+    # each built tile shape of the register SVD against the oracle: Haar gates on 14
+    # qubits grow the middle bonds to 128, so every class truncates
+    from qsim_amd import mps_tune
+    n, depth, cut = 14, 14, 1e-12
+    ups = workloads.brickwall_gates(n, depth, seed=31 + chi, kind='haar')
+    ref = mps_ref.MPS(n, chi, cut)
+    for _, i, g in ups:
+        ref.apply_adjacent_2q(i, g)
+    rv = ref.get_statevector()
+    saved = mps_tune.current_lanes(chi)
+    try:
+        for lanes in mps_tune.lane_choices(chi):
+            mps_tune.set_lanes(chi, lanes)
+            host = chain_run(n, chi, cut, ups).to_host()
+            assert host.bond_dims() == ref.bond_dims(), lanes
+            assert abs(host.norm_log - ref.norm_log) < 1e-10 * max(1.0, abs(ref.norm_log))
+            sv = host.get_statevector()
+            assert np.max(np.abs(sv - rv)) < 1e-10 * np.max(np.abs(rv)), lanes
+    finally:
+        mps_tune.set_lanes(chi, saved)
+
+
+def test_autotuner_times_every_width_and_installs_the_fastest(tmp_path):
+    from qsim_amd import mps_tune
+    saved = {c: mps_tune.current_lanes(c) for c in mps_tune.CLASSES}
+    try:
+        rep = mps_tune.autotune(classes=(16, 64), ntasks={16: 256, 64: 64}, reps=1)
+        for c in (16, 64):
+            times = rep[c]['ms_per_update']
+            assert sorted(times) == mps_tune.lane_choices(c)
+            assert all(t > 0 for t in times.values())
+            assert rep[c]['lanes'] == min(times, key=times.get) == mps_tune.current_lanes(c)
+        path = str(tmp_path / 'tune.json')
+        mps_tune.save(path)
+        mps_tune.set_lanes(16, 4 if rep[16]['lanes'] != 4 else 8)
+        assert mps_tune.load(path)
+        assert mps_tune.current_lanes(16) == rep[16]['lanes']
+    finally:
+        for c, w in saved.items():
+            mps_tune.set_lanes(c, w)
+
+
 def test_fp32_mfma_fidelity():
     n, depth, chi, cut = 64, 16, 64, 1e-10
     ups = workloads.brickwall_gates(n, depth, seed=23)
