@@ -17,6 +17,7 @@
 //                Vh = S^-1 U^H theta is formed from theta in the workspace.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <atomic>
 #include <utility>
 #include <vector>
@@ -58,18 +59,22 @@ struct DevTask {
     int32_t pad[3];
 };
 
-__device__ __forceinline__ bool load_dims(const qs_mps_task& T, Dims& d) {
+// Workspace elements per task: theta (2chi x 2chi); the chi-128 class adds its working
+// copy of the columns.
+__host__ __device__ inline size_t theta_slot_elems(int chi) {
+    return (size_t)(chi > kRegChi ? 8 : 4) * chi * chi;
+}
+
+// Bond dimensions of the task's two sites, checked against its chi_max and against the
+// workspace slot the launch gives each task (a task whose chi_max needs a larger slot
+// is refused rather than written past its slot).
+__device__ __forceinline__ bool load_dims(const qs_mps_task& T, Dims& d, size_t slot) {
     d.cl = T.bond[T.site];
     d.cm = T.bond[T.site + 1];
     d.cr = T.bond[T.site + 2];
     return d.cl >= 1 && d.cm >= 1 && d.cr >= 1 && d.cl <= T.chi_max && d.cm <= T.chi_max &&
-           d.cr <= T.chi_max && T.chi_max <= kMaxChi;
-}
-
-// Workspace elements per task: theta (2chi x 2chi); the chi-128 class adds its working
-// copy of the columns.
-inline size_t theta_slot_elems(int chi) {
-    return (size_t)(chi > kRegChi ? 8 : 4) * chi * chi;
+           d.cr <= T.chi_max && T.chi_max >= 1 && T.chi_max <= kMaxChi &&
+           theta_slot_elems(T.chi_max) <= slot;
 }
 
 __device__ __forceinline__ cd2* site_ptr(const qs_mps_task& T, int k) {
@@ -82,7 +87,7 @@ mps_contract_fp64_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws
                          size_t theta_slot, int32_t* __restrict__ status) {
     const qs_mps_task& T = tasks[blockIdx.x].t;
     Dims d;
-    if (!load_dims(T, d)) {
+    if (!load_dims(T, d, theta_slot)) {
         if (threadIdx.x == 0 && status) status[tasks[blockIdx.x].orig] = QS_E_CHI;
         return;
     }
@@ -144,7 +149,7 @@ mps_contract_f32_mfma_kernel(const DevTask* __restrict__ tasks, cd2* __restrict_
 
     const qs_mps_task& T = tasks[blockIdx.x].t;
     Dims d;
-    if (!load_dims(T, d) || d.cl > kRegChi || d.cm > kRegChi || d.cr > kRegChi) {
+    if (!load_dims(T, d, theta_slot) || d.cl > kRegChi || d.cm > kRegChi || d.cr > kRegChi) {
         if (threadIdx.x == 0 && status) status[tasks[blockIdx.x].orig] = QS_E_CHI;
         return;
     }
@@ -339,7 +344,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
     const DevTask& DT = tasks[blockIdx.x];
     const qs_mps_task& T = DT.t;
     Dims d;
-    if (!load_dims(T, d) || d.cl > CHI || d.cr > CHI) {
+    if (!load_dims(T, d, theta_slot) || d.cl > CHI || d.cr > CHI) {
         if (threadIdx.x == 0 && status) status[DT.orig] = QS_E_CHI;
         return;
     }
@@ -646,7 +651,7 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
     const DevTask& DT = tasks[blockIdx.x];
     const qs_mps_task& T = DT.t;
     Dims d;
-    if (!load_dims(T, d)) {
+    if (!load_dims(T, d, slot)) {
         if (threadIdx.x == 0 && status) status[DT.orig] = QS_E_CHI;
         return;
     }
@@ -907,14 +912,14 @@ int qs_mps_set_svd_lanes(int chi_class, int lanes) {
     return QS_OK;
 }
 
-int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, void* workspace,
-                        size_t ws_bytes, int32_t* status_dev, void* stream) {
-    clear_error();
-    QS_REQUIRE(ntasks >= 0, QS_E_ARG, "negative task count");
-    if (ntasks == 0) return QS_OK;
-    QS_REQUIRE(tasks != nullptr && workspace != nullptr, QS_E_ARG, "null tasks or workspace");
-    QS_REQUIRE(precision == QS_PREC_FP64 || precision == QS_PREC_FP32_MFMA, QS_E_ARG,
-               "unknown precision %d", precision);
+namespace {
+constexpr int kClasses[5] = {8, 16, 32, 64, 128};
+
+// Validates `tasks` and writes them into `out` grouped by SVD size class (stable within
+// a class), with each task's original index; counts[c] = tasks of class kClasses[c].
+int plan_batch(const qs_mps_task* tasks, int ntasks, DevTask* out, int32_t* counts,
+               int* chi_out) {
+    QS_REQUIRE(tasks != nullptr, QS_E_ARG, "null tasks");
     int chi = 1;
     for (int i = 0; i < ntasks; ++i) {
         const qs_mps_task& t = tasks[i];
@@ -939,31 +944,30 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
                 QS_REQUIRE(false, QS_E_ARG, "tasks %d and %d share a site", keys[i - 1].second,
                            keys[i].second);
     }
-    const size_t need = qs_mps_workspace_bytes(ntasks, chi);
-    QS_REQUIRE(ws_bytes >= need, QS_E_RANGE, "workspace %zu bytes < %zu needed", ws_bytes, need);
-    // Group by SVD size class (stable), keep each task's original index for statuses.
-    std::vector<DevTask> dt(ntasks);
-    std::vector<int> cls(ntasks);
     int k = 0;
-    for (int c : {8, 16, 32, 64, 128})
+    for (int c = 0; c < 5; ++c) {
+        counts[c] = 0;
         for (int i = 0; i < ntasks; ++i)
-            if (chi_class(tasks[i].chi_max) == c) {
-                dt[k].t = tasks[i];
-                dt[k].orig = i;
-                cls[k] = c;
+            if (chi_class(tasks[i].chi_max) == kClasses[c]) {
+                std::memset(&out[k], 0, sizeof(DevTask));
+                out[k].t = tasks[i];
+                out[k].orig = i;
+                ++counts[c];
                 ++k;
             }
-    hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
-    char* ws = static_cast<char*>(workspace);
-    DevTask* dtasks = reinterpret_cast<DevTask*>(ws);
+    }
+    *chi_out = chi;
+    return QS_OK;
+}
+
+// Launches a planned batch whose table is already in device memory: the contraction
+// (MFMA up to chi 64 when fp32 is asked, FP64 otherwise and for the chi-128 class,
+// which is grouped last), then one SVD launch per size class.  No host transfers.
+int launch_batch(const DevTask* dtasks, int ntasks, const int32_t* counts, int chi,
+                 int precision, char* ws, int32_t* status_dev, hipStream_t hs) {
     cd2* theta = reinterpret_cast<cd2*>(ws + sv_blob_offset((size_t)ntasks * sizeof(DevTask)));
     const size_t slot = theta_slot_elems(chi);
-    QS_HIP(hipMemcpyAsync(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask),
-                          hipMemcpyHostToDevice, hs));
-    // The MFMA contraction stages its operands in LDS up to chi 64; tasks of the chi-128
-    // class (grouped last) take the FP64 contraction whatever the requested precision.
-    int nreg = 0;
-    while (nreg < ntasks && cls[nreg] <= kRegChi) ++nreg;
+    const int nreg = ntasks - counts[4];
     if (precision == QS_PREC_FP32_MFMA && nreg > 0)
         hipLaunchKernelGGL(mps_contract_f32_mfma_kernel, dim3(nreg), dim3(kMfmaThreads), 0, hs,
                            dtasks, theta, slot, status_dev);
@@ -973,14 +977,91 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
                            dtasks + f64_first, theta + (size_t)f64_first * slot, slot,
                            status_dev);
     QS_HIP(hipGetLastError());
-    for (int b = 0; b < ntasks;) {
-        int e = b;
-        while (e < ntasks && cls[e] == cls[b]) ++e;
-        launch_svd(cls[b], dtasks + b, e - b, theta + (size_t)b * slot, slot, status_dev, hs);
+    int b = 0;
+    for (int c = 0; c < 5; ++c) {
+        if (counts[c] == 0) continue;
+        launch_svd(kClasses[c], dtasks + b, counts[c], theta + (size_t)b * slot, slot,
+                   status_dev, hs);
         QS_HIP(hipGetLastError());
-        b = e;
+        b += counts[c];
     }
     return QS_OK;
+}
+
+int check_batch_args(int ntasks, int precision, const void* workspace) {
+    QS_REQUIRE(ntasks >= 0, QS_E_ARG, "negative task count");
+    QS_REQUIRE(workspace != nullptr || ntasks == 0, QS_E_ARG, "null workspace");
+    QS_REQUIRE(precision == QS_PREC_FP64 || precision == QS_PREC_FP32_MFMA, QS_E_ARG,
+               "unknown precision %d", precision);
+    return QS_OK;
+}
+}  // namespace
+
+int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, void* workspace,
+                        size_t ws_bytes, int32_t* status_dev, void* stream) {
+    clear_error();
+    int st = check_batch_args(ntasks, precision, workspace);
+    if (st) return st;
+    if (ntasks == 0) return QS_OK;
+    std::vector<DevTask> dt(ntasks);
+    int32_t counts[5];
+    int chi = 1;
+    st = plan_batch(tasks, ntasks, dt.data(), counts, &chi);
+    if (st) return st;
+    const size_t need = qs_mps_workspace_bytes(ntasks, chi);
+    QS_REQUIRE(ws_bytes >= need, QS_E_RANGE, "workspace %zu bytes < %zu needed", ws_bytes, need);
+    hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
+    char* ws = static_cast<char*>(workspace);
+    DevTask* dtasks = reinterpret_cast<DevTask*>(ws);
+    QS_HIP(hipMemcpyAsync(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask),
+                          hipMemcpyHostToDevice, hs));
+    return launch_batch(dtasks, ntasks, counts, chi, precision, ws, status_dev, hs);
+}
+
+size_t qs_mps_batch_table_bytes(int ntasks) {
+    return ntasks > 0 ? (size_t)ntasks * sizeof(DevTask) : 0;
+}
+
+int qs_mps_batch_plan(const qs_mps_task* tasks, int ntasks, void* table, size_t table_bytes,
+                      int32_t* counts, int* chi_max_out) {
+    clear_error();
+    QS_REQUIRE(ntasks >= 0, QS_E_ARG, "negative task count");
+    QS_REQUIRE(counts != nullptr && chi_max_out != nullptr, QS_E_ARG, "null outputs");
+    QS_REQUIRE(ntasks == 0 || table != nullptr, QS_E_ARG, "null table");
+    QS_REQUIRE(table_bytes >= qs_mps_batch_table_bytes(ntasks), QS_E_RANGE,
+               "table %zu bytes < %zu needed", table_bytes, qs_mps_batch_table_bytes(ntasks));
+    if (ntasks == 0) {
+        for (int c = 0; c < 5; ++c) counts[c] = 0;
+        *chi_max_out = 1;
+        return QS_OK;
+    }
+    return plan_batch(tasks, ntasks, static_cast<DevTask*>(table), counts, chi_max_out);
+}
+
+int qs_mps_batch_launch(const void* table_dev, int ntasks, const int32_t* counts, int chi_max,
+                        int precision, void* workspace, size_t ws_bytes, int32_t* status_dev,
+                        void* stream) {
+    clear_error();
+    int st = check_batch_args(ntasks, precision, workspace);
+    if (st) return st;
+    if (ntasks == 0) return QS_OK;
+    QS_REQUIRE(table_dev != nullptr && counts != nullptr, QS_E_ARG, "null table or counts");
+    QS_REQUIRE(chi_max >= 1 && chi_max <= kMaxChi, QS_E_CHI, "chi_max %d outside [1, %d]",
+               chi_max, kMaxChi);
+    int sum = 0;
+    for (int c = 0; c < 5; ++c) {
+        QS_REQUIRE(counts[c] >= 0, QS_E_ARG, "negative class count");
+        sum += counts[c];
+    }
+    QS_REQUIRE(sum == ntasks, QS_E_ARG, "class counts sum to %d, not %d", sum, ntasks);
+    for (int c = 0; c < 5; ++c)
+        QS_REQUIRE(counts[c] == 0 || kClasses[c] <= chi_class(chi_max), QS_E_ARG,
+                   "tasks of chi class %d exceed chi_max %d", kClasses[c], chi_max);
+    const size_t need = qs_mps_workspace_bytes(ntasks, chi_max);
+    QS_REQUIRE(ws_bytes >= need, QS_E_RANGE, "workspace %zu bytes < %zu needed", ws_bytes, need);
+    return launch_batch(static_cast<const DevTask*>(table_dev), ntasks, counts, chi_max,
+                        precision, static_cast<char*>(workspace), status_dev,
+                        reinterpret_cast<hipStream_t>(stream));
 }
 
 int qs_mps_apply_1q(double* sites, const int32_t* bond, int64_t site_stride, int site,

@@ -7,6 +7,11 @@ the theta workspace stays bounded); within a chunk the library groups tasks by
 SVD size class.  Task tables are built as NumPy structured arrays with the
 qs_mps_task layout, and gates are generated per (circuit, layer) in vectorised
 form, so the host keeps up with the device at millions of updates.
+
+Graph mode (build_graph / replay) plans every round on the host first
+(qs_mps_batch_plan), uploads all launch tables as one device buffer, and records
+the launches of the whole run (qs_mps_batch_launch, which touches no host memory)
+into one CUDA/HIP graph; a replay is then a single submission for all rounds.
 """
 
 from __future__ import annotations
@@ -122,6 +127,62 @@ class MultiRun:
     def run(self, rounds: Optional[int] = None) -> None:
         for r in range(self.rounds if rounds is None else rounds):
             self._submit(self._round_tasks(r))
+
+    # -- graph mode --------------------------------------------------------------
+    def build_graph(self, rounds: Optional[int] = None):
+        """Plan every round, upload the launch tables and record the run as a graph.
+
+        The graph covers exactly what run() would do for the same rounds (same gates,
+        same dlog slots); replay() executes it.  Returns the torch CUDAGraph.
+        """
+        import torch
+        lib = _lib.lib()
+        chunks = []                  # (table byte offset, ntasks, counts, chi, status offset)
+        tables = []
+        off = 0
+        soff = 0
+        for r in range(self.rounds if rounds is None else rounds):
+            tasks = self._round_tasks(r)
+            for b in range(0, tasks.size, self.max_tasks):
+                chunk = np.ascontiguousarray(tasks[b:b + self.max_tasks])
+                nb = lib.qs_mps_batch_table_bytes(chunk.size)
+                buf = np.zeros(nb, dtype=np.uint8)
+                counts = (ctypes.c_int32 * 5)()
+                chi = ctypes.c_int(0)
+                _lib.call('qs_mps_batch_plan', ctypes.c_void_p(chunk.ctypes.data), chunk.size,
+                          ctypes.c_void_p(buf.ctypes.data), nb, counts, ctypes.byref(chi))
+                tables.append(buf)
+                chunks.append((off, chunk.size, counts, chi.value, soff))
+                off += nb
+                soff += chunk.size
+        dev = f'cuda:{self.device}'
+        host = np.concatenate(tables) if tables else np.zeros(1, dtype=np.uint8)
+        self.g_tables = torch.from_numpy(host).to(dev)
+        self.g_status = torch.zeros(max(soff, 1), dtype=torch.int32, device=dev)
+        self.g_chunks = chunks
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.graph(g, stream=side):
+            stream = _lib.current_stream(self.device)
+            tbase = self.g_tables.data_ptr()
+            sbase = self.g_status.data_ptr()
+            for toff, n, counts, chi, so in chunks:
+                _lib.call('qs_mps_batch_launch', ctypes.c_void_p(tbase + toff), n, counts, chi,
+                          self.prec, ctypes.c_void_p(self.ws.data_ptr()), self.ws.numel(),
+                          ctypes.c_void_p(sbase + 4 * so), stream)
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        self.graph = g
+        return g
+
+    def replay(self) -> None:
+        self.graph.replay()
+
+    def check_graph(self) -> None:
+        worst = int(self.g_status.min().item())
+        if worst < 0:
+            raise RuntimeError(f'a two-site update failed with status {worst}')
 
     def check(self) -> None:
         worst = int(self.err.cpu().item())

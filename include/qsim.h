@@ -192,6 +192,27 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision,
                         void* workspace, size_t ws_bytes, int32_t* status_dev,
                         void* stream);
 
+/* Graph-capturable form of qs_mps_update_batch, in two steps (no counterpart in the
+ * reference, whose host loop launches one update at a time and waits for it,
+ * fpga_mps_simulator.py:566-634).
+ * qs_mps_batch_plan validates `tasks` on the host exactly as qs_mps_update_batch does
+ * and writes the launch table (tasks grouped by SVD size class) into `table`, host
+ * memory of at least qs_mps_batch_table_bytes(ntasks) bytes; counts[5] receives the
+ * number of tasks of class 8, 16, 32, 64 and 128, *chi_max_out the largest chi_max.
+ * The caller copies the table to device memory.
+ * qs_mps_batch_launch runs the batch from that device copy: it reads no host memory
+ * beyond its arguments, copies nothing and allocates nothing, so it can be recorded
+ * into a HIP graph and replayed while the table, chains, dlog slots, workspace and
+ * status buffer stay allocated.  `chi_max` sizes the per-task workspace slots
+ * (qs_mps_workspace_bytes(ntasks, chi_max)); a task whose chi_max needs a larger slot
+ * gets status QS_E_CHI and writes nothing.  Statuses as for qs_mps_update_batch. */
+size_t qs_mps_batch_table_bytes(int ntasks);
+int qs_mps_batch_plan(const qs_mps_task* tasks, int ntasks, void* table, size_t table_bytes,
+                      int32_t counts[5], int* chi_max_out);
+int qs_mps_batch_launch(const void* table_dev, int ntasks, const int32_t counts[5],
+                        int chi_max, int precision, void* workspace, size_t ws_bytes,
+                        int32_t* status_dev, void* stream);
+
 /* SVD tile shape, the autotuner's knob (no counterpart in the reference, whose SVD is
  * LAPACK gesdd on the host).  The register SVD of size class 8, 16, 32 or 64 runs one
  * team of `lanes` lanes per column pair (lanes * class threads per task, 2 class / lanes

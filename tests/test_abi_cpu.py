@@ -69,6 +69,54 @@ def test_svd_team_width_table():
             mps_tune.set_lanes(c, w)
 
 
+def test_batch_plan_groups_by_svd_class_without_device():
+    """qs_mps_batch_plan runs on the host: class grouping, original indices, errors."""
+    import numpy as np
+    lib = _lib.load_library()
+    chis = [8, 64, 16, 128, 8, 32, 100]
+    tasks = (_lib.QsMpsTask * len(chis))()
+    for k, c in enumerate(chis):
+        t = tasks[k]
+        t.sites, t.bond, t.dlog = 0x10000, 0x20000, 0x30000 + 8 * k
+        t.site_stride = 2 * 128 * 128
+        t.site = 3 * k
+        t.chi_max = c
+        t.cutoff = 1e-10
+    nb = lib.qs_mps_batch_table_bytes(len(chis))
+    rec = nb // len(chis)
+    assert rec >= ctypes.sizeof(_lib.QsMpsTask) + 4
+    buf = np.zeros(nb, dtype=np.uint8)
+    counts = (ctypes.c_int32 * 5)()
+    chi = ctypes.c_int(0)
+    st = lib.qs_mps_batch_plan(tasks, len(chis), ctypes.c_void_p(buf.ctypes.data), nb, counts,
+                               ctypes.byref(chi))
+    assert st == 0, lib.qs_last_error()
+    assert list(counts) == [2, 1, 1, 1, 2] and chi.value == 128
+    size = ctypes.sizeof(_lib.QsMpsTask)
+    orig = [int(np.frombuffer(buf[k * rec + size:k * rec + size + 4].tobytes(), np.int32)[0])
+            for k in range(len(chis))]
+    assert orig == [0, 4, 2, 5, 1, 3, 6]
+    # the table holds each task verbatim
+    for k, o in enumerate(orig):
+        assert buf[k * rec:k * rec + size].tobytes() == bytes(tasks[o])
+    assert lib.qs_mps_batch_plan(tasks, len(chis), ctypes.c_void_p(buf.ctypes.data), nb - 1,
+                                 counts, ctypes.byref(chi)) == _lib.QS_E_RANGE
+    tasks[1].site = tasks[0].site + 1                       # overlapping sites
+    assert lib.qs_mps_batch_plan(tasks, len(chis), ctypes.c_void_p(buf.ctypes.data), nb, counts,
+                                 ctypes.byref(chi)) == _lib.QS_E_ARG
+    # launch-side argument checks come before any device work
+    bad = (ctypes.c_int32 * 5)(1, 0, 0, 0, 0)
+    ws = ctypes.c_void_p(0x40000)
+    assert lib.qs_mps_batch_launch(ctypes.c_void_p(0x50000), 2, bad, 8, 0, ws, 1 << 30, None,
+                                   None) == _lib.QS_E_ARG        # counts sum to 1, not 2
+    hi = (ctypes.c_int32 * 5)(0, 0, 0, 0, 2)
+    assert lib.qs_mps_batch_launch(ctypes.c_void_p(0x50000), 2, hi, 64, 0, ws, 1 << 30, None,
+                                   None) == _lib.QS_E_ARG        # class 128 above chi_max 64
+    ok = (ctypes.c_int32 * 5)(2, 0, 0, 0, 0)
+    assert lib.qs_mps_batch_launch(ctypes.c_void_p(0x50000), 2, ok, 8, 0, ws, 16, None,
+                                   None) == _lib.QS_E_RANGE      # workspace too small
+
+
 def test_argument_errors_are_reported_without_device():
     lib = _lib.load_library()
     g = (_lib.QsGate * 1)()

@@ -43,3 +43,34 @@ def test_multi_circuit_batch_matches_oracle(precision):
         ov = mps_ref.overlap(got, ref)
         fid = abs(ov) ** 2 / (mps_ref.overlap(got, got).real * mps_ref.overlap(ref, ref).real)
         assert 1 - fid < (1e-10 if precision == 'fp64' else 1e-6), (c, 1 - fid)
+
+
+def test_graph_replay_matches_eager_run():
+    """The whole multi-circuit run recorded as one graph: every replay (from the same
+    start, restored by eager copies between replays) equals the eager run bit for bit."""
+    import torch
+    from qsim_amd.mps_multi import CircuitSpec, MultiRun
+    specs = [CircuitSpec(9, 4, 7, 1), CircuitSpec(14, 8, 12, 2), CircuitSpec(30, 16, 9, 3),
+             CircuitSpec(21, 32, 14, 4), CircuitSpec(12, 64, 16, 5), CircuitSpec(16, 100, 14, 6),
+             CircuitSpec(18, 128, 10, 8)]
+    eager = MultiRun(specs, cutoff=1e-10, precision='fp32', max_tasks=25)
+    eager.run()
+    eager.check()
+    gr = MultiRun(specs, cutoff=1e-10, precision='fp32', max_tasks=25)
+    start = [(ch.sites.clone(), ch.bond.clone()) for ch in gr.chains]
+    gr.build_graph()
+    assert len(gr.g_chunks) > max(s.depth for s in specs)      # several chunks per round
+    for rep in range(3):
+        if rep:
+            for ch, (s0, b0) in zip(gr.chains, start):
+                ch.sites.copy_(s0)
+                ch.bond.copy_(b0)
+            gr.g_status.zero_()
+            torch.cuda.synchronize()
+        gr.replay()
+        torch.cuda.synchronize()
+        gr.check_graph()
+        for c in range(len(specs)):
+            assert torch.equal(gr.chains[c].bond, eager.chains[c].bond), (rep, c)
+            assert torch.equal(gr.chains[c].sites, eager.chains[c].sites), (rep, c)
+        assert np.array_equal(gr.norm_logs(), eager.norm_logs()), rep
