@@ -271,9 +271,12 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
                     const DevPhase& Q = plan.phases[P.phase0 + G.qa];
                     os << ",\"phase\":{\"sign\":" << Q.sign << ",\"slot\":" << Q.slot
                        << ",\"m0\":" << Q.m0 << ",\"mb\":[";
-                    for (int k = 0; k < SV_TILE_BITS - SV_REG_BITS; ++k)
+                    for (int k = 0; k < SV_THREAD_BITS; ++k)
                         os << (k ? "," : "") << Q.mb[k];
-                    os << "],\"pp\":[" << Q.pp[0] << "," << Q.pp[1] << "]"
+                    os << "],\"pp\":[";
+                    for (int k = 0; k < SV_THREAD_BITS; ++k)
+                        os << (k ? "," : "") << Q.pp[k];
+                    os << "]"
                        << ",\"c\":[" << Q.c[0] << "," << Q.c[1] << "],\"kr\":[";
                     for (int k = 0; k < 4; ++k)
                         os << (k ? "," : "") << Q.kr[k][0] << "," << Q.kr[k][1];

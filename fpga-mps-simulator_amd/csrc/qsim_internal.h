@@ -49,9 +49,10 @@ void clear_error();
 // ----------------------------------------------------------------- statevector
 
 constexpr int SV_MAX_QUBITS = 34;      // 2^34 * 16 B = 256 GiB, inside 288 GB of HBM
-constexpr int SV_TILE_BITS = 12;       // qubits per fused pass tile (4096 amplitudes)
+constexpr int SV_TILE_BITS = 12;       // qubits per fused pass tile (4096 amplitudes, 64 KiB)
 constexpr int SV_REG_BITS = 4;         // amplitudes per thread = 16
 constexpr int SV_LOW_BITS = 4;         // qubits 0..3 are in every tile (256 B runs)
+constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bits of a full tile
 
 // Device op of a fused pass (one record per op, executed in order by every thread).
 //   DG_U1X    one-qubit gates on every register bit; the 2x2 matrix of register bit b
@@ -90,16 +91,16 @@ struct alignas(16) DevGate {
 struct DevLayout {
     int32_t regs[4];       // local positions of register bits 0..R-1, ascending
     int32_t rq[4];         // global qubits of register bits
-    int32_t tpos[8];       // local position of thread bit b
-    int32_t tq[8];         // global qubit of thread bit b
+    int32_t tpos[SV_THREAD_BITS];   // local position of thread bit b
+    int32_t tq[SV_THREAD_BITS];     // global qubit of thread bit b
 };
 static_assert(sizeof(DevLayout) <= sizeof(double) * 32, "DevLayout must fit in DevGate::m");
 
 // A unitary diagonal group is a product of phase factors f^(bit products).  Factors are
 // classified by where their qubits live in the pass: block bits (not in the tile),
 // thread bits (tile positions that index the thread) or register bits.
-//   block classes, folded per workgroup by sv_phase_block_kernel into 13 factors
-//     [A_B, lambda_0..lambda_7 (thread bits), kappa_0..kappa_3 (register bits)]:
+//   block classes, folded per workgroup by sv_phase_block_kernel into
+//   PHASE_BLOCK_VALUES factors [A_B, lambda_b (thread bits b), kappa_r (register bits r)]:
 //     PT_BB: a, b global (b < 0: one-body)       -> A_B
 //     PT_BT: a thread bit, b global              -> lambda_a
 //     PT_BR: a register bit, b global            -> kappa_a
@@ -108,7 +109,8 @@ static_assert(sizeof(DevLayout) <= sizeof(double) * 32, "DevLayout must fit in D
 //     PT_TT: a, b thread bits                    -> A
 //     PT_TR: a register bit, b thread bit        -> kappa_a
 enum : int32_t { PT_BB = 0, PT_BT = 1, PT_BR = 2, PT_T1 = 3, PT_TT = 4, PT_TR = 5 };
-constexpr int PHASE_BLOCK_VALUES = 13;
+constexpr int PHASE_BLOCK_VALUES = 1 + SV_THREAD_BITS + SV_REG_BITS;
+constexpr int PHASE_KAPPA0 = 1 + SV_THREAD_BITS;   // index of kappa_0
 constexpr int PHASE_THREAD_VALUES = 5;
 
 struct alignas(16) DevTerm {
@@ -129,11 +131,11 @@ struct alignas(16) DevPhase {
     int32_t pad;
     // SIGN groups: the per-thread mask as the kernel rebuilds it, with t the thread
     // index: m0 ^ (XOR of mb[b] over set bits b of t) ^ (0xFFFF if the parity of
-    // t_a t_b over the thread-pair terms is odd); bit b' of byte a of pp (pp[a / 4],
-    // byte a % 4) marks the pair (a, b'), a < b'.
+    // t_a t_b over the thread-pair terms is odd); bit b' of pp[a] marks the pair
+    // (a, b'), a < b'.
     uint32_t m0;
-    uint32_t mb[SV_TILE_BITS - SV_REG_BITS];
-    uint32_t pp[2];
+    uint32_t mb[SV_THREAD_BITS];
+    uint32_t pp[SV_THREAD_BITS];
     double c[2];           // constant factor
     double kr[4][2];       // constant per-register-bit factors
     double crr[6][2];      // register-pair factors: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)

@@ -70,8 +70,10 @@ __device__ __forceinline__ void make_layout(Layout<R>& L, const DevLayout& D, in
     }
     int loc = 0;
     uint64_t g = 0;
+    // thread bits beyond the tile's thread count are 0 (a narrow tile, R < 4, has one
+    // thread)
 #pragma unroll
-    for (int b = 0; b < SV_TILE_BITS - R; ++b) {
+    for (int b = 0; b < SV_THREAD_BITS; ++b) {
         const int bit = (t >> b) & 1;
         loc |= bit << D.tpos[b];
         g |= (uint64_t)bit << D.tq[b];
@@ -321,7 +323,7 @@ __device__ __forceinline__ constexpr unsigned reg_col_mask(int r) {
 
 // A unitary diagonal group (DevPhase).  The host folds the constant, thread-class,
 // register and register-pair factors into one table entry per thread; block-class
-// factors come from the prologue's per-tile table bf = [A_B, lambda_0..7, kappa_0..3]
+// factors come from the prologue's per-tile table bf = [A_B, lambda_q, kappa_r]
 // (lambda_q applies when thread bit q is set, kappa_r to register bit r).
 
 // DG_PHASE_SIGN: every factor is +-1; the group is a 16-bit mask of the register
@@ -330,16 +332,25 @@ __device__ __forceinline__ constexpr unsigned reg_col_mask(int r) {
 // queue behind the tile's HBM traffic.
 __device__ __forceinline__ unsigned sign_mask(const DevPhase& Q, int t) {
     unsigned M = Q.m0;
+    // Uniform branches: most groups have no thread-bit or no thread-pair terms.
+    unsigned any_mb = 0, any_pp = 0;
 #pragma unroll
-    for (int b = 0; b < SV_TILE_BITS - SV_REG_BITS; ++b)
-        M ^= ((t >> b) & 1) ? Q.mb[b] : 0u;
-    unsigned par = 0;
-#pragma unroll
-    for (int a = 0; a < SV_TILE_BITS - SV_REG_BITS - 1; ++a) {
-        const unsigned row = (Q.pp[a >> 2] >> (8 * (a & 3))) & 0xFFu;
-        par ^= (unsigned)(t >> a) & (unsigned)__builtin_popcount(row & (unsigned)t);
+    for (int b = 0; b < SV_THREAD_BITS; ++b) {
+        any_mb |= Q.mb[b];
+        any_pp |= Q.pp[b];
     }
-    return M ^ ((par & 1u) ? 0xFFFFu : 0u);
+    if (any_mb) {
+#pragma unroll
+        for (int b = 0; b < SV_THREAD_BITS; ++b) M ^= ((t >> b) & 1) ? Q.mb[b] : 0u;
+    }
+    if (any_pp) {
+        unsigned par = 0;
+#pragma unroll
+        for (int a = 0; a < SV_THREAD_BITS - 1; ++a)
+            par ^= (unsigned)(t >> a) & (unsigned)__builtin_popcount(Q.pp[a] & (unsigned)t);
+        M ^= (par & 1u) ? 0xFFFFu : 0u;
+    }
+    return M;
 }
 
 template <int R>
@@ -349,12 +360,14 @@ __device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], unsigned M,
         // value 0 of a sign group: A_B and the packed negative lambda / kappa bits
         const cd2 b0 = bf[0];
         const unsigned bits = (unsigned)b0.y;
-        const bool neg = (b0.x < 0.0) != (__builtin_popcount((unsigned)t & bits & 0xFFu) & 1);
+        constexpr unsigned kThreadMask = (1u << SV_THREAD_BITS) - 1u;
+        const bool neg =
+            (b0.x < 0.0) != (__builtin_popcount((unsigned)t & bits & kThreadMask) & 1);
         M ^= neg ? 0xFFFFu : 0u;
-        if (bits & 0x100u) M ^= reg_col_mask(0);
-        if (R > 1 && (bits & 0x200u)) M ^= reg_col_mask(1);
-        if (R > 2 && (bits & 0x400u)) M ^= reg_col_mask(2);
-        if (R > 3 && (bits & 0x800u)) M ^= reg_col_mask(3);
+        if (bits & (1u << SV_THREAD_BITS)) M ^= reg_col_mask(0);
+        if (R > 1 && (bits & (2u << SV_THREAD_BITS))) M ^= reg_col_mask(1);
+        if (R > 2 && (bits & (4u << SV_THREAD_BITS))) M ^= reg_col_mask(2);
+        if (R > 3 && (bits & (8u << SV_THREAD_BITS))) M ^= reg_col_mask(3);
     }
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
@@ -375,14 +388,14 @@ __device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const DevPhas
     if (bf) {
         A = cmul(A, bf[0]);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < SV_THREAD_BITS; ++q) {
             const cd2 Af = cmul(A, bf[1 + q]);
             A = ((t >> q) & 1) ? Af : A;
         }
-        k0 = cmul(k0, bf[9]);
-        if (R > 1) k1 = cmul(k1, bf[10]);
-        if (R > 2) k2 = cmul(k2, bf[11]);
-        if (R > 3) k3 = cmul(k3, bf[12]);
+        k0 = cmul(k0, bf[PHASE_KAPPA0]);
+        if (R > 1) k1 = cmul(k1, bf[PHASE_KAPPA0 + 1]);
+        if (R > 2) k2 = cmul(k2, bf[PHASE_KAPPA0 + 2]);
+        if (R > 3) k3 = cmul(k3, bf[PHASE_KAPPA0 + 3]);
     }
     const cd2 c01 = mk2(Q.crr[0][0], Q.crr[0][1]), c02 = mk2(Q.crr[1][0], Q.crr[1][1]);
     const cd2 c03 = mk2(Q.crr[2][0], Q.crr[2][1]), c12 = mk2(Q.crr[3][0], Q.crr[3][1]);
@@ -424,7 +437,7 @@ __device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const DevPhas
 }
 
 // Block-class phase factors of every phase group of one pass, one thread per
-// (slot, workgroup): [A_B, lambda_0..7, kappa_0..3].
+// (slot, workgroup): [A_B, lambda_q (thread bits q), kappa_r (register bits r)].
 __global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
                                       const DevPhase* __restrict__ phases,
                                       const DevTerm* __restrict__ terms,
@@ -449,7 +462,7 @@ __global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
             const bool on = ((base >> T.a) & 1ull) && (T.b < 0 || ((base >> T.b) & 1ull));
             if (on) v[0] = cmul(v[0], f);
         } else if ((base >> T.b) & 1ull) {
-            const int idx = (T.cls == PT_BT) ? 1 + T.a : 9 + T.a;
+            const int idx = (T.cls == PT_BT) ? 1 + T.a : PHASE_KAPPA0 + T.a;
 #pragma unroll
             for (int i = 1; i < PHASE_BLOCK_VALUES; ++i)
                 if (i == idx) v[i] = cmul(v[i], f);
@@ -457,7 +470,8 @@ __global__ void sv_phase_block_kernel(int nq, const DevPass* __restrict__ P,
     }
     if (Q->sign) {
         // Sign groups read only value 0: its imaginary part (zero for a +-1 factor)
-        // carries the negative lambda_q as bits 0..7 and negative kappa_r as bits 8..11.
+        // carries the negative lambda_q as bits q and negative kappa_r as bits
+        // SV_THREAD_BITS + r.
         uint32_t bits = 0;
 #pragma unroll
         for (int i = 1; i < PHASE_BLOCK_VALUES; ++i)

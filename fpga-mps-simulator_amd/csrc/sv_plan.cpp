@@ -292,7 +292,7 @@ void emit_phase_group(const std::vector<const HostGate*>& diag, const int* regs,
                 ph.mb[x.b] ^= col[x.a];
             } else if (x.cls == PT_TT) {
                 const int a = std::min(x.a, x.b), b = std::max(x.a, x.b);
-                ph.pp[a / 4] ^= 1u << (8 * (a % 4) + b);
+                ph.pp[a] ^= 1u << b;
             }
         }
     } else {

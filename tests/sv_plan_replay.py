@@ -151,9 +151,8 @@ def _sign_mask_from_descriptor(ph, t):
         if (t >> b) & 1:
             M ^= int(mb)
     par = 0
-    for a in range(len(ph['mb']) - 1):
-        row = (int(ph['pp'][a // 4]) >> (8 * (a % 4))) & 0xFF
-        par ^= ((t >> a) & 1) & bin(row & t).count('1')
+    for a, row in enumerate(ph['pp']):
+        par ^= ((t >> a) & 1) & bin(int(row) & t).count('1')
     return M ^ (0xFFFF if par & 1 else 0)
 
 

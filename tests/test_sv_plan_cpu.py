@@ -58,7 +58,8 @@ def run_plan_cpu(nq, gates):
 
 
 @pytest.mark.parametrize('nq,ng,seed', [(1, 10, 0), (2, 30, 1), (3, 60, 2), (5, 120, 3),
-                                        (12, 150, 4), (13, 200, 5), (15, 300, 6)])
+                                        (12, 150, 4), (13, 200, 5), (14, 250, 7),
+                                        (15, 300, 6), (16, 300, 8)])
 def test_plan_replay_matches_oracle(nq, ng, seed):
     _lib_or_skip()
     rng = np.random.default_rng(seed)
@@ -66,8 +67,8 @@ def test_plan_replay_matches_oracle(nq, ng, seed):
     got, plan = run_plan_cpu(nq, gates)
     ref = sv_ref.run_circuit(nq, gates)
     assert np.max(np.abs(got - ref)) < 1e-12
-    if nq > 12:
-        assert all(p['K'] == 12 for p in plan['passes'])
+    # tiles of 12 qubits (4096 amplitudes) once the state has more
+    assert all(p['K'] == min(nq, 12) for p in plan['passes'])
 
 
 def test_plan_u3_cz_layers_pass_count():
