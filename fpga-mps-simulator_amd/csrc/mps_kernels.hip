@@ -629,6 +629,18 @@ __device__ __forceinline__ double big_sum(double v) {
     return v;
 }
 
+// Column blocks of the chi-128 class: 16 columns of up to 256 rows; a wave holds a
+// column as 4 rows per lane.
+constexpr int kBlkCols = 16;
+constexpr int kBlkLd = 2 * kMaxChi;
+constexpr int kBlkRpl = kBlkLd / 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o /= 2) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 __device__ __forceinline__ int big_player(int slot, int r, int P) {
     return slot == 0 ? 0 : 1 + (slot - 1 + r) % (2 * P - 1);
 }
@@ -648,6 +660,7 @@ __global__ void __launch_bounds__(kBigThreads)
 mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size_t slot,
                    int32_t* __restrict__ status) {
     __shared__ BigShared sh;
+    __shared__ cd2 blk[2 * kBlkCols * kBlkLd];   // one pair of column blocks (128 KiB)
     const DevTask& DT = tasks[blockIdx.x];
     const qs_mps_task& T = DT.t;
     Dims d;
@@ -681,51 +694,133 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
     const double zero_sq = 1e-28 * sh.fro;
     const double tol = 2.220446049250313e-16 * sqrt((double)m);
 
+    // Blocked one-sided Jacobi.  theta's columns are cut into blocks of kBlkCols; a
+    // sweep is a round-robin tournament over the blocks.  For each pair of blocks the
+    // workgroup loads both into LDS, rotates (in the tournament's first round, where
+    // every block appears once) all pairs inside each block, then every cross pair in
+    // kBlkCols cyclic rounds (wave w keeps column w of the first block in registers
+    // and meets column (w + k) mod kBlkCols of the second in round k), and writes both
+    // back to W.  W is read and written once per block pair instead of once per
+    // column round, so the traffic of a sweep falls from about 2n full passes over
+    // theta to nb - 1.
+    const int nb = (n + kBlkCols - 1) / kBlkCols;
+    const int nbp = nb + (nb & 1);                // even, a dummy block when nb is odd
+    const int PB = nbp / 2;
+    const int wave = tid / 64, lane = tid % 64;
+    auto load_block = [&](int blkid, int half) {
+        cd2* dst = blk + half * kBlkCols * kBlkLd;
+        for (int idx = tid; idx < kBlkCols * kBlkLd; idx += kBigThreads) {
+            const int i = idx / kBlkLd, r = idx - i * kBlkLd;
+            const int col = blkid * kBlkCols + i;
+            dst[idx] = (blkid < nb && col < n && r < m) ? W[(size_t)col * m + r] : mk2(0, 0);
+        }
+    };
+    auto store_block = [&](int blkid, int half) {
+        if (blkid >= nb) return;
+        const cd2* src = blk + half * kBlkCols * kBlkLd;
+        for (int idx = tid; idx < kBlkCols * kBlkLd; idx += kBigThreads) {
+            const int i = idx / kBlkLd, r = idx - i * kBlkLd;
+            const int col = blkid * kBlkCols + i;
+            if (col < n && r < m) W[(size_t)col * m + r] = src[idx];
+        }
+    };
+    // Rotation of the pair (x, y) held as kBlkRpl rows per lane of one wave; true if
+    // the pair was rotated.
+    auto rotate_pair = [&](cd2 (&x)[kBlkRpl], cd2 (&y)[kBlkRpl]) -> bool {
+        double al = 0, be = 0, gr = 0, gi = 0;
+#pragma unroll
+        for (int k = 0; k < kBlkRpl; ++k) {
+            al = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, al));
+            be = fma(y[k].x, y[k].x, fma(y[k].y, y[k].y, be));
+            gr = fma(x[k].x, y[k].x, fma(x[k].y, y[k].y, gr));
+            gi = fma(x[k].x, y[k].y, fma(-x[k].y, y[k].x, gi));
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        gr = wave_sum(gr);
+        gi = wave_sum(gi);
+        double c, sn, er, ei;
+        if (!jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, sn, er, ei)) return false;
+        const cd2 sem = mk2(sn * er, -sn * ei);
+        const cd2 sep = mk2(sn * er, sn * ei);
+#pragma unroll
+        for (int k = 0; k < kBlkRpl; ++k) {
+            const cd2 u = x[k], v = y[k];
+            const cd2 sy = cmul(sem, v), sx = cmul(sep, u);
+            x[k] = mk2(c * u.x - sy.x, c * u.y - sy.y);
+            y[k] = mk2(c * v.x + sx.x, c * v.y + sx.y);
+        }
+        return true;
+    };
+
     int sweeps = 0;
     for (; sweeps < kMaxSweeps; ++sweeps) {
         if (tid == 0) sh.rotated = 0;
         __syncthreads();
-        const int rounds = P > 1 ? 2 * P - 1 : 1;
-        for (int rd = 0; rd < rounds; ++rd) {
-            for (int pi = team; pi < P; pi += kBigTeams) {
-                const int cp = big_player(pi, rd, P), cq = big_player(2 * P - 1 - pi, rd, P);
-                cd2* ca = W + (size_t)cp * m;
-                cd2* cb = W + (size_t)cq * m;
-                cd2 a[kBigRows], b[kBigRows];
-                double al = 0, be = 0, gr = 0, gi = 0;
+        bool any_local = false;
+        const int orounds = nbp - 1;
+        for (int ro = 0; ro < orounds; ++ro) {
+            for (int pi = 0; pi < PB; ++pi) {
+                const int bi = big_player(pi, ro, PB), bj = big_player(nbp - 1 - pi, ro, PB);
+                load_block(bi, 0);
+                load_block(bj, 1);
+                __syncthreads();
+                if (ro == 0) {
+                    // pairs inside each block: waves 0..7 the first, 8..15 the second
+                    const int half = wave / 8, pr = wave % 8;
+                    const bool real_blk = (half == 0 ? bi : bj) < nb;
+                    cd2* base = blk + half * kBlkCols * kBlkLd;
+                    for (int rd = 0; rd < kBlkCols - 1; ++rd) {
+                        if (real_blk) {
+                            const int ca = big_player(pr, rd, kBlkCols / 2);
+                            const int cb = big_player(kBlkCols - 1 - pr, rd, kBlkCols / 2);
+                            cd2 x[kBlkRpl], y[kBlkRpl];
 #pragma unroll
-                for (int k = 0; k < kBigRows; ++k) {
-                    const int r = l16 + kBigLanes * k;
-                    a[k] = r < m ? ca[r] : mk2(0, 0);
-                    b[k] = r < m ? cb[r] : mk2(0, 0);
-                    al = fma(a[k].x, a[k].x, fma(a[k].y, a[k].y, al));
-                    be = fma(b[k].x, b[k].x, fma(b[k].y, b[k].y, be));
-                    gr = fma(a[k].x, b[k].x, fma(a[k].y, b[k].y, gr));
-                    gi = fma(a[k].x, b[k].y, fma(-a[k].y, b[k].x, gi));
-                }
-                al = big_sum(al);
-                be = big_sum(be);
-                gr = big_sum(gr);
-                gi = big_sum(gi);
-                double c, s, er, ei;
-                if (jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, s, er, ei)) {
-                    const cd2 sem = mk2(s * er, -s * ei);
-                    const cd2 sep = mk2(s * er, s * ei);
+                            for (int k = 0; k < kBlkRpl; ++k) {
+                                x[k] = base[ca * kBlkLd + lane + 64 * k];
+                                y[k] = base[cb * kBlkLd + lane + 64 * k];
+                            }
+                            if (rotate_pair(x, y)) {
+                                any_local = true;
 #pragma unroll
-                    for (int k = 0; k < kBigRows; ++k) {
-                        const int r = l16 + kBigLanes * k;
-                        const cd2 x = a[k], y = b[k];
-                        const cd2 sy = cmul(sem, y), sx = cmul(sep, x);
-                        if (r < m) {
-                            ca[r] = mk2(c * x.x - sy.x, c * x.y - sy.y);
-                            cb[r] = mk2(c * y.x + sx.x, c * y.y + sx.y);
+                                for (int k = 0; k < kBlkRpl; ++k) {
+                                    base[ca * kBlkLd + lane + 64 * k] = x[k];
+                                    base[cb * kBlkLd + lane + 64 * k] = y[k];
+                                }
+                            }
                         }
+                        __syncthreads();
                     }
-                    if (l16 == 0) sh.rotated = 1;
                 }
+                if (bi < nb && bj < nb) {
+                    cd2 x[kBlkRpl];
+#pragma unroll
+                    for (int k = 0; k < kBlkRpl; ++k) x[k] = blk[wave * kBlkLd + lane + 64 * k];
+                    cd2* second = blk + kBlkCols * kBlkLd;
+                    for (int kk = 0; kk < kBlkCols; ++kk) {
+                        const int jc = (wave + kk) % kBlkCols;
+                        cd2 y[kBlkRpl];
+#pragma unroll
+                        for (int k = 0; k < kBlkRpl; ++k) y[k] = second[jc * kBlkLd + lane + 64 * k];
+                        if (rotate_pair(x, y)) {
+                            any_local = true;
+#pragma unroll
+                            for (int k = 0; k < kBlkRpl; ++k)
+                                second[jc * kBlkLd + lane + 64 * k] = y[k];
+                        }
+                        __syncthreads();
+                    }
+#pragma unroll
+                    for (int k = 0; k < kBlkRpl; ++k) blk[wave * kBlkLd + lane + 64 * k] = x[k];
+                    __syncthreads();
+                }
+                store_block(bi, 0);
+                store_block(bj, 1);
+                __syncthreads();
             }
-            __syncthreads();
         }
+        if (any_local && lane == 0) sh.rotated = 1;
+        __syncthreads();
         const int any = sh.rotated;
         __syncthreads();
         if (!any) break;
