@@ -16,7 +16,15 @@ circuit once.  The line also carries:
                  exp(-0.1iH) gates) for the batched FP32-MFMA path and the
                  plain FP64 path (one update per launch), when built.
 
+Also measured live, as a reference for the roofline: `inplace_rmw_gbps`, an in-place
+read-modify-write of the same 4 GiB state by torch's elementwise kernel (x *= 1),
+i.e. what a plain streaming kernel reaches on this HBM.
+
+`--multi NCIRC` prints a separate line for the batched multi-circuit MPS workload
+(NCIRC random circuits, chi_max 16..128, all rounds recorded as one graph).
+
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mps]
+                       [--multi NCIRC]
 """
 
 from __future__ import annotations
@@ -48,6 +56,9 @@ def parse():
     ap.add_argument('--no-mps', action='store_true', help='skip the MPS leg')
     ap.add_argument('--cpu-gates', type=int, default=6,
                     help='logical gates of the workload timed on the CPU')
+    ap.add_argument('--multi', type=int, default=0, metavar='NCIRC',
+                    help='instead of the headline line, print one JSON line for NCIRC '
+                         'independent MPS circuits (chi_max 16..128) replayed as one graph')
     ap.add_argument('--calibration-copy', action='store_true',
                     help='after the SV leg, copy the 4 GiB state once (a copy kernel of '
                          'known size for scaling the HBM byte counters of a profile)')
@@ -81,6 +92,15 @@ def sv_leg(steps: int, warmup: int, calibration_copy: bool = False):
     time_passes(False)
     gpu_ms = ev0.elapsed_time(ev1)
     norm = sv.norm_sq()
+    # reference: a plain in-place streaming kernel over the same buffer
+    r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    sv.tensor.mul_(1.0)
+    r0.record(stream)
+    for _ in range(3):
+        sv.tensor.mul_(1.0)
+    r1.record(stream)
+    torch.cuda.synchronize()
+    rmw_gbps = 3 * 2 * 16 * (1 << SV_QUBITS) / (r0.elapsed_time(r1) / 1e3) / 1e9
     if calibration_copy:
         sv.tensor.clone()
         torch.cuda.synchronize()
@@ -91,7 +111,7 @@ def sv_leg(steps: int, warmup: int, calibration_copy: bool = False):
     return {
         'wall_s': wall, 'gpu_ms': gpu_ms, 'npass': npass, 'ngates': len(gates),
         'per_launch_s': per_launch_s, 'bytes_per_pass': bytes_per_pass,
-        'norm_sq': norm,
+        'norm_sq': norm, 'inplace_rmw_gbps': rmw_gbps,
     }
 
 
@@ -148,6 +168,10 @@ def main():
         print(json.dumps({'error': 'no GPU visible'}))
         sys.exit(2)
 
+    if args.multi > 0:
+        from qsim_amd import mps_bench
+        print(json.dumps(mps_bench.run_multi(args.multi)))
+        return
     sv = sv_leg(args.steps, args.warmup, args.calibration_copy)
     value = sv['ngates'] * args.steps / (sv['gpu_ms'] / 1e3)
     achieved = sv['bytes_per_pass'] / sv['per_launch_s'] / 1e9
@@ -173,7 +197,8 @@ def main():
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBPS,
                      'traffic': traffic_from_profiles(),
                      'kernel': 'sv_pass_kernel<4>',
-                     'per_launch_us': sv['per_launch_s'] * 1e6},
+                     'per_launch_us': sv['per_launch_s'] * 1e6,
+                     'inplace_rmw_gbps': sv['inplace_rmw_gbps']},
         'check': {'norm_sq_after': sv['norm_sq']},
     }
     if not args.no_mps:

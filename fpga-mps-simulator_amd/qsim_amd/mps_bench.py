@@ -10,7 +10,12 @@ random Hermitian), chi_max 64, cutoff 1e-10: 20 x 500 + 20 x 499 = 19 980 update
                  replaying the earlier layers untimed, so the sample sees the same bond
                  dimensions as the batched runs at those layers
 
-Per-layer times come from HIP events on the stream the kernels run on.
+Per-layer times come from HIP events on the stream the kernels run on.  Before timing,
+the SVD team width of the chi-64 class is tuned (mps_tune) and reported.
+
+run_multi(): many independent circuits (n ~ U{50..2000}, chi_max in {16, 32, 64, 128},
+depth ~ U{10..60}, exp(-0.1iH) gates, complex-FP32 contraction + FP64 SVD), all rounds
+recorded as one graph and replayed; updates/s over the replay.
 """
 
 from __future__ import annotations
@@ -96,6 +101,8 @@ def run(plain_layers=(30, 31)) -> dict:
     w.apply_2q_batch([(0, gates[0][2]), (2, gates[1][2])])
     torch.cuda.synchronize()
 
+    from . import mps_tune
+    tuned = mps_tune.autotune(classes=(CHI,), precision='fp32')
     ch32, t32, wall32 = run_batched(layers, 'fp32')
     ch64, t64, wall64 = run_batched(layers, 'fp64')
     n_plain, t_plain = run_plain_sample(layers, list(plain_layers))
@@ -124,4 +131,45 @@ def run(plain_layers=(30, 31)) -> dict:
         'max_bond': max(bonds), 'mean_bond': float(np.mean(bonds)),
         'norm_log_fp64': ch64.norm_log, 'norm_log_fp32': ch32.norm_log,
         'infidelity_fp32_vs_fp64': float(1 - fid),
+        'svd_team_width': {str(c): r['lanes'] for c, r in tuned.items()},
+    }
+
+
+def run_multi(ncirc: int = 256, seed: int = 7, precision: str = 'fp32',
+              max_tasks: int = 4096) -> dict:
+    """The batched multi-circuit workload, planned and recorded as one graph."""
+    import torch
+    from . import mps_tune
+    from .mps_multi import MultiRun, random_specs
+    specs = random_specs(ncirc, seed=seed, chis=(16, 32, 64, 128))
+    tuned = mps_tune.autotune(precision=precision)
+    run = MultiRun(specs, cutoff=1e-10, precision=precision, max_tasks=max_tasks)
+    t0 = time.perf_counter()
+    run.build_graph()
+    plan_s = time.perf_counter() - t0
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    run.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    run.check_graph()
+    secs = e0.elapsed_time(e1) / 1e3
+    per_class = {}
+    for c in (16, 32, 64, 128):
+        idx = [i for i, s in enumerate(specs) if s.chi_max == c]
+        per_class[str(c)] = {'circuits': len(idx),
+                             'max_bond': max((run.chains[i].max_bond() for i in idx), default=0)}
+    return {
+        'workload': f'{ncirc} independent circuits, n ~ U{{50..2000}}, chi_max in '
+                    f'{{16, 32, 64, 128}}, depth ~ U{{10..60}}, exp(-0.1iH) gates, cutoff 1e-10, '
+                    f'{precision} contraction + FP64 SVD',
+        'updates': run._next,
+        'updates_per_s': run._next / secs,
+        'seconds': secs,
+        'graph_chunks': len(run.g_chunks),
+        'plan_and_record_seconds': plan_s,
+        'classes': per_class,
+        'svd_team_width': {str(c): r['lanes'] for c, r in tuned.items()},
     }
