@@ -246,6 +246,35 @@ class MPSEnergyGrad:
                                          self.h, self.eps)
 
 
+def torch_energy(module: MPSEnergyGrad):
+    """A torch.autograd.Function computing E(theta) on the GPU MPS engine.
+
+    The forward records the tape; the backward runs the reverse sweep through the
+    recorded one-qubit rotations, two-site contractions and truncated SVDs.
+    """
+    import torch
+
+    class _Energy(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta):
+            th = theta.detach().cpu().numpy().astype(np.float64)
+            finals, tape = module._run(th, record=True)
+            ctx.finals, ctx.tape, ctx.theta = finals, tape, th
+            with torch.no_grad():
+                e = float(tfim_energy(finals, module.J, module.h))
+            return torch.tensor(e, dtype=theta.dtype, device=theta.device)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            _, g = energy_and_grad_from_tape(ctx.finals, ctx.tape, ctx.theta,
+                                             module.c.nparams, module.J, module.h,
+                                             module.eps)
+            ctx.finals = ctx.tape = None
+            return grad_out * torch.as_tensor(g, dtype=grad_out.dtype, device=grad_out.device)
+
+    return _Energy.apply
+
+
 def energy_and_grad_from_tape(finals, tape, theta, nparams: int, J: float = 1.0,
                               h: float = 1.0, eps: float = 1e-24) -> Tuple[float, np.ndarray]:
     """E and dE/dtheta from the final site tensors and the forward tape (reverse sweep).

@@ -60,3 +60,20 @@ def test_mps_gradient_200q_chi32_matches_finite_differences():
         tm[k] -= math.pi / 2
         ps = 0.5 * (eng.energy(tp) - eng.energy(tm))
         assert abs(g[k] - ps) < 1e-8, (k, g[k], ps)
+
+
+def test_mps_gradient_through_torch_autograd():
+    import torch
+    from qsim_amd.adjoint import hardware_efficient_ansatz
+    from qsim_amd.mps_grad import MPSEnergyGrad, torch_energy
+    nq, layers = 12, 3
+    rng = np.random.default_rng(17)
+    c = hardware_efficient_ansatz(nq, layers)
+    theta0 = rng.uniform(-math.pi, math.pi, c.nparams)
+    eng = MPSEnergyGrad(c, chi_max=16, cutoff=1e-12)
+    e_ref, g_ref = eng.energy_and_grad(theta0)
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    e = torch_energy(eng)(theta)
+    (2.0 * e).backward()
+    assert abs(e.item() - e_ref) < 1e-12
+    assert np.max(np.abs(theta.grad.numpy() - 2.0 * g_ref)) < 1e-12
