@@ -288,8 +288,7 @@ __device__ __forceinline__ void apply_u2(cd2 (&a)[1 << R], const DevGate& G,
         case 1 * 4 + 3: if constexpr (R >= 4) { QS_CASE("u2 13 <"); apply_u2_bits<R, S1, S3>(a, G, L, base); QS_CASE("u2 13 >"); } break;
         case 3 * 4 + 1: if constexpr (R >= 4) { QS_CASE("u2 31 <"); apply_u2_bits<R, S3, S1>(a, G, L, base); QS_CASE("u2 31 >"); } break;
         case 2 * 4 + 3: if constexpr (R >= 4) { QS_CASE("u2 23 <"); apply_u2_bits<R, S2, S3>(a, G, L, base); QS_CASE("u2 23 >"); } break;
-        case 3 * 4 + 2: if constexpr (R >= 4) { QS_CASE("u2 32 <"); apply_u2_bits<R, S3, S2>(a, G, L, base); QS_CASE("u2 32 >"); } break;
-        default: break;
+        default: if constexpr (R >= 4) { QS_CASE("u2 32 <"); apply_u2_bits<R, S3, S2>(a, G, L, base); QS_CASE("u2 32 >"); } break;
     }
 }
 

@@ -548,9 +548,23 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
 
     StageScheduler::State st = S.initial();
     int cur[4] = {0, 0, 0, 0};
+    // Constant factors of the rotations emitted since the last DG_SCALE.  A scaled
+    // rotation grows amplitudes by up to sqrt(2), so the factor is applied whenever it
+    // falls below kScaleFloor (amplitudes within 1e30 of their true size, far from
+    // overflow, however many rotations a pass holds) and once at the end of the pass.
+    constexpr double kScaleFloor = 1e-30;
     cd scale = 1.0;
-    for (const Item* it : inc)
-        if (it->rot) scale *= it->scale;
+    auto emit_scale = [&]() {
+        if (scale == cd(1.0, 0.0)) return;
+        DevGate d{};
+        d.kind = DG_SCALE;
+        d.r0 = scale.imag() == 0.0 ? 1 : 0;
+        d.qa = d.qb = -1;
+        d.m[0] = scale.real();
+        d.m[1] = scale.imag();
+        plan->gates.push_back(d);
+        scale = 1.0;
+    };
     // Uncontrolled one-qubit gates gathered per register bit until another op is
     // emitted; three or four bits become one DG_U1X, fewer become DG_U1S ops.
     int u1_mask = 0;
@@ -626,6 +640,8 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
             d.m[2] = inc[j]->post.real();
             d.m[3] = inc[j]->post.imag();
             plan->gates.push_back(d);
+            scale *= inc[j]->scale;
+            if (std::abs(scale) < kScaleFloor) emit_scale();
             return;
         }
         if (g.kind == QS_GATE_U1 && g.cmask == 0) {
@@ -702,15 +718,7 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
         close_u1();
         if (si + 1 == masks.size()) {
             emit_diagonals(S.flush_ready(st));
-            if (scale != cd(1.0, 0.0)) {   // constant factors of this pass's rotations
-                DevGate d{};
-                d.kind = DG_SCALE;
-                d.r0 = scale.imag() == 0.0 ? 1 : 0;
-                d.qa = d.qb = -1;
-                d.m[0] = scale.real();
-                d.m[1] = scale.imag();
-                plan->gates.push_back(d);
-            }
+            emit_scale();
             if ((curmask & lowmask) && curmask != defmask) emit_layout(defmask);
         }
         stg.g1 = (int)plan->gates.size() - P.gate0;

@@ -236,6 +236,18 @@ def test_random_circuits_match_oracle(fpga, nq, ng, seed):
     assert np.max(np.abs(res['statevector'] - ref)) < 1e-12
 
 
+def test_deep_single_pass_circuit_matches_oracle(fpga):
+    # 9000 rotations in one pass: the planner divides the scaled rotations' growth out
+    rng = np.random.default_rng(41)
+    nq = 6
+    g = [('u3', [int(rng.integers(nq))], [float(v) for v in rng.uniform(0, 2 * math.pi, 3)])
+         for _ in range(9000)]
+    res = run(fpga, nq, g)
+    ref = sv_ref.run_circuit(nq, g)
+    assert np.all(np.isfinite(res['statevector']))
+    assert np.max(np.abs(res['statevector'] - ref)) < 1e-10
+
+
 def test_words_format_matches_oracle_f32():
     """The 8-word instruction entry point reproduces the FPGA's float32 parameters."""
     from qsim_amd.sv import StateVector

@@ -119,3 +119,18 @@ def test_bad_gates_raise():
     bad = [(recs[0][0], 7, 0, 0, recs[0][4])]
     with pytest.raises(_lib.QsError):
         plan_json(2, bad)
+
+
+def test_deep_single_pass_circuit_stays_finite():
+    """Thousands of rotations in one pass (a 6-qubit state is one tile): the scaled
+    rotations' growth is divided out along the way, so nothing overflows."""
+    _lib_or_skip()
+    rng = np.random.default_rng(41)
+    nq = 6
+    gates = [('u3', [int(rng.integers(nq))], [float(v) for v in rng.uniform(0, 2 * math.pi, 3)])
+             for _ in range(9000)]
+    got, plan = run_plan_cpu(nq, gates)
+    assert len(plan['passes']) == 1
+    ref = sv_ref.run_circuit(nq, gates)
+    assert np.all(np.isfinite(got))
+    assert np.max(np.abs(got - ref)) < 1e-10
