@@ -11,7 +11,8 @@ circuit once.  The line also carries:
                  timed steps, against 8 TB/s; `traffic` is the measured HBM bytes
                  per launch from the committed rocprofv3 counter summary,
   cpu_baseline : the NumPy oracle's pair updates on the host, on a bounded
-                 sample of the same 28-qubit workload,
+                 sample of the same 28-qubit workload (and, under `mps`, the MPS
+                 oracle's two-site updates at chi=64 on a sample of one layer),
   mps          : two-site MPS updates/s at chi=64 (1000 qubits, brick-wall of
                  exp(-0.1iH) gates) for the batched FP32-MFMA path and the
                  plain FP64 path (one update per launch), when built.
@@ -136,6 +137,36 @@ def cpu_leg(ngates: int):
                       f'{dt:.1f} s'}
 
 
+def cpu_mps_leg(nsample: int = 200, layer: int = 30):
+    """The MPS oracle (NumPy/SciPy restatement of mps_core) on the host: `nsample`
+    two-site updates of layer `layer` of the chi=64 workload, starting from the state the
+    GPU's FP64 path reaches after the earlier layers (copied to the host)."""
+    import numpy as np
+    import torch
+    from oracle import mps_ref
+    from qsim_amd import mps_bench, workloads
+    from qsim_amd.mps import MPSChain
+    gates = workloads.brickwall_gates(mps_bench.N_SITES, mps_bench.DEPTH, mps_bench.SEED)
+    layers = mps_bench._layers(gates)
+    ch = MPSChain(mps_bench.N_SITES, mps_bench.CHI, mps_bench.CUTOFF, precision='fp64')
+    for batch in layers[:layer]:
+        ch.apply_2q_batch(batch)
+    torch.cuda.synchronize()
+    host = ch.to_host()
+    ref = mps_ref.MPS(mps_bench.N_SITES, mps_bench.CHI, mps_bench.CUTOFF)
+    ref.sites = host.sites
+    sample = layers[layer][:nsample]
+    t0 = time.perf_counter()
+    for i, g in sample:
+        ref.apply_adjacent_2q(i, g)
+    dt = time.perf_counter() - t0
+    threads = os.environ.get('OMP_NUM_THREADS') or os.environ.get('OPENBLAS_NUM_THREADS')
+    return {'value': len(sample) / dt, 'unit': 'updates/s',
+            'cores': int(threads) if threads else os.cpu_count(), 'kind': 'port',
+            'sample': f'{len(sample)} two-site updates (chi 64, layer {layer}) of the '
+                      f'1000-qubit workload, NumPy/SciPy (LAPACK gesdd), {dt:.1f} s'}
+
+
 def traffic_from_profiles():
     """HBM bytes per pass launch from the committed rocprofv3 PMC summary, if present
     (the line `hbm_bytes_per_launch: N` of profiles/sv_pass_28q_traffic.md)."""
@@ -207,6 +238,8 @@ def main():
             out['mps'] = m
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_leg(args.cpu_gates)
+        if not args.no_mps and 'mps' in out:
+            out['cpu_baseline']['mps'] = cpu_mps_leg()
     print(json.dumps(out))
 
 

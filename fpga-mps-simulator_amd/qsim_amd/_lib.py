@@ -83,6 +83,14 @@ _PROTOS = {
     'qs_sv_apply_tfim': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_void_p]),
     'qs_mps_max_chi': (ctypes.c_int, []),
+    'qs_mps_env_bytes': (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
+    'qs_mps_right_envs': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_size_t, ctypes.c_void_p]),
+    'qs_mps_sample': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_void_p]),
     'qs_mps_batch_table_bytes': (ctypes.c_size_t, [ctypes.c_int]),
     'qs_mps_batch_plan': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32),

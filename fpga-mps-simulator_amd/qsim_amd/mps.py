@@ -168,6 +168,54 @@ class MPSChain:
     def to_host(self) -> 'HostMPS':
         return HostMPS(self.site_tensors(), self.norm_log)
 
+    # -- shot sampling ----------------------------------------------------------
+    def sample(self, shots: int, measured: Optional[List[int]] = None,
+               rng: Optional[np.random.Generator] = None,
+               max_shots_per_launch: int = 65536) -> List[str]:
+        """Sequential conditional sampling on the device (mps_core.py:371-420).
+
+        Bitstrings as the reference returns them (qubit n-1 first; with `measured`, the
+        measured qubits in descending order).  The uniforms are drawn from `rng` in the
+        reference's order (one per site per shot, shot-major), so a seeded generator
+        gives the reference's samples.
+        """
+        import torch
+        rng = rng if rng is not None else np.random.default_rng()
+        n = self.n
+        measured = list(range(n)) if measured is None else [int(q) for q in measured]
+        for q in measured:
+            if not 0 <= q < n:
+                raise ValueError(f'measured qubit {q} outside the chain of {n}')
+        dev = self.sites.device
+        lib = _lib.lib()
+        nbytes = lib.qs_mps_env_bytes(n, self.chi_max)
+        envs = torch.empty(nbytes // 16, dtype=torch.complex128, device=dev)
+        stream = self._stream()
+        _lib.call('qs_mps_right_envs', ctypes.c_void_p(self.sites.data_ptr()),
+                  ctypes.c_void_p(self.bond.data_ptr()), self.stride, n, self.chi_max,
+                  ctypes.c_void_p(envs.data_ptr()), nbytes, stream)
+        cols = (list(range(n - 1, -1, -1)) if set(measured) == set(range(n))
+                else sorted(measured, reverse=True))
+        out: List[str] = []
+        for b0 in range(0, shots, max_shots_per_launch):
+            k = min(max_shots_per_launch, shots - b0)
+            u = torch.from_numpy(rng.random(k * n)).to(dev)
+            bits = torch.empty(k * n, dtype=torch.uint8, device=dev)
+            _lib.call('qs_mps_sample', ctypes.c_void_p(self.sites.data_ptr()),
+                      ctypes.c_void_p(self.bond.data_ptr()), self.stride, n, self.chi_max,
+                      ctypes.c_void_p(envs.data_ptr()), ctypes.c_void_p(u.data_ptr()), k,
+                      ctypes.c_void_p(bits.data_ptr()), stream)
+            b = bits.cpu().numpy().reshape(k, n)[:, cols] + ord('0')
+            out.extend(row.tobytes().decode() for row in b)
+        return out
+
+    def get_counts(self, shots: int, measured: Optional[List[int]] = None,
+                   rng: Optional[np.random.Generator] = None) -> Dict[str, int]:
+        c: Dict[str, int] = {}
+        for s in self.sample(shots, measured, rng):
+            c[s] = c.get(s, 0) + 1
+        return c
+
 
 class HostMPS:
     """Host copy of a chain with the read-outs of mps_core.MPSState."""
@@ -192,57 +240,6 @@ class HostMPS:
             acc = np.tensordot(acc, t, axes=([1], [0])).reshape(-1, t.shape[2])
         sv = acc[:, 0] * math.exp(self.norm_log)
         return sv.reshape([2] * self.n).transpose(list(range(self.n - 1, -1, -1))).reshape(-1)
-
-    def right_environments(self) -> List[np.ndarray]:
-        envs: List = [None] * (self.n + 1)
-        envs[self.n] = np.ones((1, 1), dtype=np.complex128)
-        for i in range(self.n - 1, -1, -1):
-            a = self.sites[i]
-            r = np.einsum('asb,bc,dsc->ad', a, envs[i + 1], a.conj())
-            tr = float(np.real(np.trace(r)))
-            if tr > 1e-300:
-                r = r / tr
-            envs[i] = r
-        return envs
-
-    def sample(self, shots: int, measured: Optional[List[int]] = None,
-               rng: Optional[np.random.Generator] = None) -> List[str]:
-        """Sequential conditional sampling (mps_core.py:371-420)."""
-        rng = rng if rng is not None else np.random.default_rng()
-        measured = list(range(self.n)) if measured is None else list(measured)
-        full = set(measured) == set(range(self.n))
-        envs = self.right_environments()
-        out = []
-        for _ in range(shots):
-            v = np.ones(1, dtype=np.complex128)
-            bits = [0] * self.n
-            for i in range(self.n):
-                a = self.sites[i]
-                r = envs[i + 1]
-                pr = np.zeros(2)
-                for s in (0, 1):
-                    w = v @ a[:, s, :]
-                    pr[s] = np.real(w @ r @ w.conj())
-                pr = np.maximum(pr, 0.0)
-                tot = pr.sum()
-                pr = np.array([0.5, 0.5]) if tot < 1e-15 else pr / tot
-                o = int(rng.choice(2, p=pr))
-                bits[i] = o
-                v = v @ a[:, o, :]
-                nv = np.linalg.norm(v)
-                if nv > 1e-15:
-                    v = v / nv
-            if full:
-                out.append(''.join(str(bits[q]) for q in range(self.n - 1, -1, -1)))
-            else:
-                out.append(''.join(str(bits[q]) for q in sorted(measured, reverse=True)))
-        return out
-
-    def get_counts(self, shots, measured=None, rng=None) -> Dict[str, int]:
-        c: Dict[str, int] = {}
-        for s in self.sample(shots, measured, rng):
-            c[s] = c.get(s, 0) + 1
-        return c
 
 
 def overlap(bra, ket) -> complex:

@@ -213,6 +213,26 @@ int qs_mps_batch_launch(const void* table_dev, int ntasks, const int32_t counts[
                         int chi_max, int precision, void* workspace, size_t ws_bytes,
                         int32_t* status_dev, void* stream);
 
+/* Shot sampling of a chain on the device (mps_core.py:354-420, called by the driver at
+ * fpga_mps_simulator.py:639-643, where it runs on the host).
+ * qs_mps_right_envs writes the right environments R_0..R_n of the chain (sites, bond,
+ * site_stride as for qs_mps_task; chi_max <= 128) into `envs`, device memory of at least
+ * qs_mps_env_bytes(n, chi_max) bytes: R_n = [[1]], R_i = sum_s A_i[:, s, :] R_{i+1}
+ * A_i[:, s, :]^H divided by its trace (when above 1e-300), each (bond[i] x bond[i])
+ * complex128 with row stride chi_max, at envs + i * chi_max^2 complex elements.
+ * qs_mps_sample then draws `shots` bitstrings by sequential conditional sampling: for
+ * shot t and site i it reads the uniform uniforms[t * n + i] (device, in [0, 1)) and
+ * takes outcome 1 exactly when u >= p_0 / (p_0 + p_1), as numpy's
+ * Generator.choice(2, p=[p_0, p_1]) does with the same uniform; bits[t * n + i]
+ * (device uint8) receives the outcome.  Drawing the uniforms in that order from the
+ * reference's generator reproduces its samples. */
+size_t qs_mps_env_bytes(int n, int chi_max);
+int qs_mps_right_envs(const double* sites, const int32_t* bond, int64_t site_stride, int n,
+                      int chi_max, double* envs, size_t env_bytes, void* stream);
+int qs_mps_sample(const double* sites, const int32_t* bond, int64_t site_stride, int n,
+                  int chi_max, const double* envs, const double* uniforms, int shots,
+                  uint8_t* bits, void* stream);
+
 /* SVD tile shape, the autotuner's knob (no counterpart in the reference, whose SVD is
  * LAPACK gesdd on the host).  The register SVD of size class 8, 16, 32 or 64 runs one
  * team of `lanes` lanes per column pair (lanes * class threads per task, 2 class / lanes

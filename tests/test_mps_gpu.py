@@ -269,3 +269,52 @@ def test_chi_limit_raises():
         FPGAMPSSimulator(chi_max=129)
     with pytest.raises(ValueError):
         FPGAMPSSimulator(engine='cpu')
+
+
+# ---------------------------------------------------------------- shot sampling
+def _oracle_chain(n, chi, cut, ups):
+    ref = mps_ref.MPS(n, chi, cut)
+    for _, i, g in ups:
+        ref.apply_adjacent_2q(i, g)
+    return ref
+
+
+@pytest.mark.parametrize('n,chi,depth,seed', [(10, 16, 6, 3), (14, 8, 10, 4)])
+def test_gpu_sampling_reproduces_oracle_samples(n, chi, depth, seed):
+    """Same generator seed -> the same bitstrings as sequential sampling on the host
+    (mps_core.py:371-420), for the full register and for a measured subset."""
+    ups = workloads.brickwall_gates(n, depth, seed=seed, kind='haar')
+    ch = chain_run(n, chi, 1e-12, ups)
+    ref = _oracle_chain(n, chi, 1e-12, ups)
+    got = ch.sample(3000, None, np.random.default_rng(11))
+    exp = ref.sample(3000, None, np.random.default_rng(11))
+    assert got == exp
+    meas = [n - 1, 2, 0, 5]
+    got = ch.sample(500, meas, np.random.default_rng(12))
+    exp = ref.sample(500, meas, np.random.default_rng(12))
+    assert got == exp and all(len(b) == 4 for b in got)
+
+
+def test_gpu_sampling_ghz():
+    from qsim_amd.mps import MPSChain
+    n = 12
+    ch = MPSChain(n, 4, 1e-12)
+    h = np.array([[1, 1], [1, -1]], dtype=complex) / math.sqrt(2)
+    ch.apply_1q(0, h)
+    cx = np.eye(4, dtype=complex)[[0, 1, 3, 2]]        # control = left site
+    for i in range(n - 1):
+        ch.apply_2q(i, cx)
+    c = ch.get_counts(4000, None, np.random.default_rng(5))
+    assert set(c) == {'0' * n, '1' * n}
+    assert abs(c['0' * n] - 2000) < 200
+
+
+def test_gpu_sampling_200_qubits_chi32_matches_oracle():
+    n, chi, depth = 200, 32, 12
+    ups = workloads.brickwall_gates(n, depth, seed=21)
+    ch = chain_run(n, chi, 1e-10, ups)
+    ref = mps_ref.MPS(n, chi, 1e-10)
+    ref.sites = [t.copy() for t in ch.to_host().sites]     # the device chain, on the host
+    got = ch.sample(48, None, np.random.default_rng(2))
+    exp = ref.sample(48, None, np.random.default_rng(2))
+    assert got == exp
