@@ -160,8 +160,16 @@ def cpu_mps_leg(nsample: int = 200, layer: int = 30):
     for i, g in sample:
         ref.apply_adjacent_2q(i, g)
     dt = time.perf_counter() - t0
+    # host sampling of the same chain (right environments + `shots` shots)
+    shots = 4
+    t1 = time.perf_counter()
+    ref.sample(shots, None, np.random.default_rng(mps_bench.SEED))
+    ts = time.perf_counter() - t1
     threads = os.environ.get('OMP_NUM_THREADS') or os.environ.get('OPENBLAS_NUM_THREADS')
     return {'value': len(sample) / dt, 'unit': 'updates/s',
+            'sampling_shots_per_s': shots / ts,
+            'sampling_sample': f'{shots} shots of the 1000-qubit chi 64 chain, right '
+                               f'environments included, {ts:.1f} s',
             'cores': int(threads) if threads else os.cpu_count(), 'kind': 'port',
             'sample': f'{len(sample)} two-site updates (chi 64, layer {layer}) of the '
                       f'1000-qubit workload, NumPy/SciPy (LAPACK gesdd), {dt:.1f} s'}
