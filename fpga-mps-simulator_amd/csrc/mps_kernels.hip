@@ -1010,6 +1010,46 @@ int qs_mps_set_svd_lanes(int chi_class, int lanes) {
 namespace {
 constexpr int kClasses[5] = {8, 16, 32, 64, 128};
 
+// Kernel timing (qs_mps_time_kernels): events from a per-thread pool bracket every
+// contraction and SVD launch of a batch on its stream, with the launch's kind and tasks.
+struct MpsTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<int32_t> kind, tasks;
+    size_t used = 0;
+    ~MpsTimer() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+};
+thread_local MpsTimer g_mps_timer;
+
+// Records the start event of a timed launch (or nothing when timing is off).
+int timer_begin(hipStream_t hs, hipEvent_t* e1) {
+    *e1 = nullptr;
+    if (!g_mps_timer.on) return QS_OK;
+    hipEvent_t e0 = g_mps_timer.next();
+    *e1 = g_mps_timer.next();
+    QS_REQUIRE(e0 && *e1, QS_E_HIP, "hipEventCreate failed");
+    QS_HIP(hipEventRecord(e0, hs));
+    return QS_OK;
+}
+
+int timer_end(hipStream_t hs, hipEvent_t e1, int kind, int ntasks) {
+    if (!e1) return QS_OK;
+    QS_HIP(hipEventRecord(e1, hs));
+    g_mps_timer.kind.push_back(kind);
+    g_mps_timer.tasks.push_back(ntasks);
+    return QS_OK;
+}
+
 // Validates `tasks` and writes them into `out` grouped by SVD size class (stable within
 // a class), with each task's original index; counts[c] = tasks of class kClasses[c].
 int plan_batch(const qs_mps_task* tasks, int ntasks, DevTask* out, int32_t* counts,
@@ -1063,20 +1103,30 @@ int launch_batch(const DevTask* dtasks, int ntasks, const int32_t* counts, int c
     cd2* theta = reinterpret_cast<cd2*>(ws + sv_blob_offset((size_t)ntasks * sizeof(DevTask)));
     const size_t slot = theta_slot_elems(chi);
     const int nreg = ntasks - counts[4];
-    if (precision == QS_PREC_FP32_MFMA && nreg > 0)
+    hipEvent_t e1 = nullptr;
+    int st = QS_OK;
+    if (precision == QS_PREC_FP32_MFMA && nreg > 0) {
+        if ((st = timer_begin(hs, &e1))) return st;
         hipLaunchKernelGGL(mps_contract_f32_mfma_kernel, dim3(nreg), dim3(kMfmaThreads), 0, hs,
                            dtasks, theta, slot, status_dev);
+        if ((st = timer_end(hs, e1, 0, nreg))) return st;
+    }
     const int f64_first = precision == QS_PREC_FP32_MFMA ? nreg : 0;
-    if (ntasks > f64_first)
+    if (ntasks > f64_first) {
+        if ((st = timer_begin(hs, &e1))) return st;
         hipLaunchKernelGGL(mps_contract_fp64_kernel, dim3(ntasks - f64_first), dim3(256), 0, hs,
                            dtasks + f64_first, theta + (size_t)f64_first * slot, slot,
                            status_dev);
+        if ((st = timer_end(hs, e1, 1, ntasks - f64_first))) return st;
+    }
     QS_HIP(hipGetLastError());
     int b = 0;
     for (int c = 0; c < 5; ++c) {
         if (counts[c] == 0) continue;
+        if ((st = timer_begin(hs, &e1))) return st;
         launch_svd(kClasses[c], dtasks + b, counts[c], theta + (size_t)b * slot, slot,
                    status_dev, hs);
+        if ((st = timer_end(hs, e1, 2, counts[c]))) return st;
         QS_HIP(hipGetLastError());
         b += counts[c];
     }
@@ -1111,6 +1161,36 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
     QS_HIP(hipMemcpyAsync(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask),
                           hipMemcpyHostToDevice, hs));
     return launch_batch(dtasks, ntasks, counts, chi, precision, ws, status_dev, hs);
+}
+
+int qs_mps_time_kernels(int enable) {
+    clear_error();
+    g_mps_timer.on = enable != 0;
+    g_mps_timer.used = 0;
+    g_mps_timer.kind.clear();
+    g_mps_timer.tasks.clear();
+    return QS_OK;
+}
+
+int qs_mps_kernel_times(int32_t* kind, int32_t* tasks, double* ms, int cap, int* count) {
+    clear_error();
+    QS_REQUIRE(count != nullptr && cap >= 0 &&
+                   (cap == 0 || (kind != nullptr && tasks != nullptr && ms != nullptr)),
+               QS_E_ARG, "bad output arguments");
+    const int n = (int)g_mps_timer.kind.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        QS_HIP(hipEventSynchronize(g_mps_timer.pool[2 * i + 1]));
+        float t = 0.0f;
+        QS_HIP(hipEventElapsedTime(&t, g_mps_timer.pool[2 * i], g_mps_timer.pool[2 * i + 1]));
+        kind[i] = g_mps_timer.kind[i];
+        tasks[i] = g_mps_timer.tasks[i];
+        ms[i] = t;
+    }
+    *count = n;
+    g_mps_timer.used = 0;
+    g_mps_timer.kind.clear();
+    g_mps_timer.tasks.clear();
+    return QS_OK;
 }
 
 size_t qs_mps_batch_table_bytes(int ntasks) {

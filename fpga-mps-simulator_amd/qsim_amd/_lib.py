@@ -83,6 +83,10 @@ _PROTOS = {
     'qs_sv_apply_tfim': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_void_p]),
     'qs_mps_max_chi': (ctypes.c_int, []),
+    'qs_mps_time_kernels': (ctypes.c_int, [ctypes.c_int]),
+    'qs_mps_kernel_times': (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32),
+                                           ctypes.POINTER(ctypes.c_int32), _c_double_p,
+                                           ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     'qs_mps_env_bytes': (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
     'qs_mps_right_envs': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_void_p,

@@ -242,6 +242,33 @@ class HostMPS:
         return sv.reshape([2] * self.n).transpose(list(range(self.n - 1, -1, -1))).reshape(-1)
 
 
+def z_expectations(host: 'HostMPS') -> np.ndarray:
+    """<Z_i> / <psi|psi> for every site of a host chain, by left and right transfer
+    matrices (normalised at every step; O(n chi^3))."""
+    n = host.n
+    right = [None] * (n + 1)
+    right[n] = np.ones((1, 1), dtype=np.complex128)
+    for i in range(n - 1, -1, -1):
+        a = host.sites[i]
+        t = np.tensordot(a, right[i + 1], axes=([2], [0]))          # (l, s, r')
+        r = np.tensordot(t, a.conj(), axes=([1, 2], [1, 2]))        # (l, l')
+        right[i] = r / max(abs(np.trace(r)), 1e-300)
+    out = np.zeros(n)
+    left = np.ones((1, 1), dtype=np.complex128)
+    z = np.array([1.0, -1.0])
+    for i in range(n):
+        a = host.sites[i]
+        t = np.tensordot(left, a, axes=([1], [0]))                  # (l', s, r)
+        u = np.tensordot(t, right[i + 1], axes=([2], [0]))          # (l', s, r')
+        w = np.tensordot(a.conj(), u, axes=([0, 2], [0, 2]))        # (s*, s)
+        num = float(np.real(z[0] * w[0, 0] + z[1] * w[1, 1]))
+        den = float(np.real(w[0, 0] + w[1, 1]))
+        out[i] = num / den if den != 0 else 0.0
+        nl = np.tensordot(a.conj(), t, axes=([0, 1], [0, 1]))       # (r*, r)
+        left = nl / max(abs(np.trace(nl)), 1e-300)
+    return out
+
+
 def overlap(bra, ket) -> complex:
     """<bra|ket> of two chains (host copies) by transfer matrices, with exp(norm_log)."""
     env = np.ones((1, 1), dtype=np.complex128)

@@ -32,6 +32,7 @@ CHI = 64
 CUTOFF = 1e-10
 SEED = 42
 SHOTS = 1024
+MFMA_FP32_PEAK_TFLOPS = 157.3        # MI355X dense FP32 matrix
 
 
 def contraction_flops(cl: int, cm: int, cr: int) -> float:
@@ -64,6 +65,39 @@ def run_batched(layers, precision: str):
     per_layer = [evs[k].elapsed_time(evs[k + 1]) / 1e3 for k in range(len(layers))]
     ch.check_status()
     return ch, per_layer, wall
+
+
+def run_contraction_timed(layers):
+    """The FP32 batched run again with every contraction and SVD launch timed by HIP
+    events, and the bond dimensions read before each layer, so the contraction's
+    algorithmic flops (32 chi_l chi_r (chi_m + 4) per update) are exact."""
+    import ctypes
+    import torch
+    from . import _lib
+    from .mps import MPSChain
+    ch = MPSChain(N_SITES, CHI, CUTOFF, precision='fp32')
+    flops = 0.0
+    _lib.call('qs_mps_time_kernels', 1)
+    try:
+        for batch in layers:
+            b = ch.bond.cpu().numpy().astype(np.float64)
+            for i, _ in batch:
+                flops += contraction_flops(b[i], b[i + 1], b[i + 2])
+            ch.apply_2q_batch(batch)
+        torch.cuda.synchronize()
+        cap = 8 * len(layers)
+        kind = (ctypes.c_int32 * cap)()
+        tasks = (ctypes.c_int32 * cap)()
+        ms = (ctypes.c_double * cap)()
+        cnt = ctypes.c_int(0)
+        _lib.call('qs_mps_kernel_times', kind, tasks, ms, cap, ctypes.byref(cnt))
+    finally:
+        _lib.call('qs_mps_time_kernels', 0)
+    ch.check_status()
+    n = min(cnt.value, cap)
+    t_con = sum(ms[k] for k in range(n) if kind[k] == 0) / 1e3
+    t_svd = sum(ms[k] for k in range(n) if kind[k] == 2) / 1e3
+    return flops, t_con, t_svd
 
 
 def run_plain_sample(layers, sample_layers: List[int]):
@@ -107,6 +141,7 @@ def run(plain_layers=(30, 31)) -> dict:
     ch32, t32, wall32 = run_batched(layers, 'fp32')
     ch64, t64, wall64 = run_batched(layers, 'fp64')
     n_plain, t_plain = run_plain_sample(layers, list(plain_layers))
+    flops, t_con, t_svd = run_contraction_timed(layers)
     # shot sampling of the final 1000-qubit state (right environments + 1024 shots)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -116,6 +151,8 @@ def run(plain_layers=(30, 31)) -> dict:
     bonds = ch64.bond_dims()
     h32, h64 = ch32.to_host(), ch64.to_host()
     fid = abs(overlap(h32, h64)) ** 2 / (overlap(h32, h32).real * overlap(h64, h64).real)
+    from .mps import z_expectations
+    dz = float(np.max(np.abs(z_expectations(h32) - z_expectations(h64))))
     s32 = sum(t32)
     s64 = sum(t64)
     samp32 = sum(t32[k] for k in plain_layers)
@@ -138,6 +175,16 @@ def run(plain_layers=(30, 31)) -> dict:
         'max_bond': max(bonds), 'mean_bond': float(np.mean(bonds)),
         'norm_log_fp64': ch64.norm_log, 'norm_log_fp32': ch32.norm_log,
         'infidelity_fp32_vs_fp64': float(1 - fid),
+        'max_abs_z_error_fp32_vs_fp64': dz,
+        'contraction': {
+            'kernel': 'mps_contract_f32_mfma_kernel',
+            'algorithmic_tflop': flops / 1e12,
+            'seconds': t_con,
+            'tflops': flops / t_con / 1e12,
+            'mfma_peak_tflops': MFMA_FP32_PEAK_TFLOPS,
+            'mfma_frac': flops / t_con / 1e12 / MFMA_FP32_PEAK_TFLOPS,
+            'svd_seconds_same_run': t_svd,
+        },
         'svd_team_width': {str(c): r['lanes'] for c, r in tuned.items()},
         'sampling_shots_per_s': SHOTS / t_samp,
         'sampling_sample': f'{SHOTS} shots of the final {N_SITES}-qubit chi {CHI} state, '

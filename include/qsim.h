@@ -213,6 +213,15 @@ int qs_mps_batch_launch(const void* table_dev, int ntasks, const int32_t counts[
                         int chi_max, int precision, void* workspace, size_t ws_bytes,
                         int32_t* status_dev, void* stream);
 
+/* Kernel timing of the MPS batches (instrumentation, no counterpart in the reference).
+ * While enabled, every contraction and SVD launch of qs_mps_update_batch /
+ * qs_mps_batch_launch is bracketed by HIP events on its stream.  qs_mps_kernel_times
+ * waits for them and returns, in launch order, up to `cap` entries: kind (0 = complex
+ * FP32 contraction on MFMA, 1 = FP64 contraction, 2 = SVD), the launch's task count and
+ * its duration in ms; *count receives the number recorded.  Reading clears the record. */
+int qs_mps_time_kernels(int enable);
+int qs_mps_kernel_times(int32_t* kind, int32_t* tasks, double* ms, int cap, int* count);
+
 /* Shot sampling of a chain on the device (mps_core.py:354-420, called by the driver at
  * fpga_mps_simulator.py:639-643, where it runs on the host).
  * qs_mps_right_envs writes the right environments R_0..R_n of the chain (sites, bond,

@@ -228,3 +228,17 @@ def test_tfim_energy_matches_dense():
         e -= mps_ref.pauli_expectation_sv(sv, ''.join('Z' if n - 1 - k in (i, i + 1) else 'I'
                                                       for k in range(n)))
     assert abs(mps_ref.tfim_energy(m) - e) < 1e-12
+
+
+def test_product_z_expectations_match_oracle():
+    """qsim_amd.mps.z_expectations (host read-out of a device chain) on random chains."""
+    from qsim_amd.mps import HostMPS, z_expectations
+    rng = np.random.default_rng(3)
+    for dims in ([1, 2, 4, 7, 8, 8, 6, 4, 2, 1], [1, 2, 2, 1], [1, 1]):
+        n = len(dims) - 1
+        sites = [rng.standard_normal((dims[i], 2, dims[i + 1])) +
+                 1j * rng.standard_normal((dims[i], 2, dims[i + 1])) for i in range(n)]
+        m = mps_ref.MPS(n, 16, 0)
+        m.sites = [s.copy() for s in sites]
+        got = z_expectations(HostMPS(sites, 0.0))
+        assert np.max(np.abs(got - m.z_expectations())) < 1e-12
