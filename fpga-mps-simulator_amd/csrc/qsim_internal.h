@@ -66,13 +66,15 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //   DG_PHASE / DG_PHASE_SIGN  unitary diagonal group, DevPhase record qa; the SIGN form
 //             has only +-1 factors
 //   DG_LAYOUT transpose through LDS to the DevLayout stored at the start of m
-//   DG_ROT    scaled real rotation on register bit r0, t = m[0]: r1 bit 0 clear maps a
-//             pair (x0, x1) to (x0 - t x1, x1 + t x0), set to (t x0 - x1, x0 + t x1);
-//             r1 bit 1 set multiplies the second output by k = (m[2], m[3])
 //   DG_SCALE  every amplitude times (m[0], m[1]); r0 = 1 when the factor is real
+//   DG_ROT + 4 r0 + r1 (16 kinds): scaled real rotation on register bit r0, t = m[0]:
+//             r1 bit 0 clear maps a pair (x0, x1) to (x0 - t x1, x1 + t x0), set to
+//             (t x0 - x1, x0 + t x1); r1 bit 1 set multiplies the second output by
+//             k = (m[2], m[3]).  Bit and variant live in the kind so that the kernel
+//             selects the straight-line body with one switch.
 enum : int32_t {
     DG_U1 = 0, DG_U2 = 1, DG_DIAG1 = 2, DG_DIAG2 = 3, DG_U1X = 4, DG_PHASE = 5,
-    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8, DG_ROT = 9, DG_SCALE = 10
+    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8, DG_SCALE = 10, DG_ROT = 16
 };
 
 struct alignas(16) DevGate {

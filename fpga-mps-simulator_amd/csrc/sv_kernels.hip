@@ -229,31 +229,37 @@ __device__ __forceinline__ void apply_u1s(cd2 (&a)[1 << R], const DevGate& G) {
     }
 }
 
+// DG_ROT + 4 bit + variant: one switch over the sixteen bodies.
 #define QS_ROT_CASES(B, TAG)                                                              \
-    case 4 * B + 0: QS_CASE("rot " TAG "a <"); rot_bit<R, B, false, false>(a, t, k); QS_CASE("rot " TAG "a >"); break; \
-    case 4 * B + 1: QS_CASE("rot " TAG "b <"); rot_bit<R, B, true, false>(a, t, k); QS_CASE("rot " TAG "b >"); break;  \
-    case 4 * B + 2: QS_CASE("rot " TAG "c <"); rot_bit<R, B, false, true>(a, t, k); QS_CASE("rot " TAG "c >"); break;  \
-    case 4 * B + 3: QS_CASE("rot " TAG "d <"); rot_bit<R, B, true, true>(a, t, k); QS_CASE("rot " TAG "d >"); break;
+    case DG_ROT + 4 * B + 0: QS_CASE("rot " TAG "a <"); rot_bit<R, B, false, false>(a, t, k); QS_CASE("rot " TAG "a >"); break; \
+    case DG_ROT + 4 * B + 1: QS_CASE("rot " TAG "b <"); rot_bit<R, B, true, false>(a, t, k); QS_CASE("rot " TAG "b >"); break;  \
+    case DG_ROT + 4 * B + 2: QS_CASE("rot " TAG "c <"); rot_bit<R, B, false, true>(a, t, k); QS_CASE("rot " TAG "c >"); break;  \
+    case DG_ROT + 4 * B + 3: QS_CASE("rot " TAG "d <"); rot_bit<R, B, true, true>(a, t, k); QS_CASE("rot " TAG "d >"); break;
 
 template <int R>
-__device__ __forceinline__ void apply_rot(cd2 (&a)[1 << R], const DevGate& G, int r0, int r1,
+__device__ __forceinline__ void apply_rot(cd2 (&a)[1 << R], const DevGate& G, int kind,
                                           double t, cd2 k) {
     if constexpr (R == 4) {
-        switch (r0 * 4 + r1) {
-            QS_ROT_CASES(0, "0") QS_ROT_CASES(1, "1") QS_ROT_CASES(2, "2") QS_ROT_CASES(3, "3")
-            default: break;
+        switch (kind) {
+            QS_ROT_CASES(0, "0") QS_ROT_CASES(1, "1") QS_ROT_CASES(2, "2")
+            case DG_ROT + 12: QS_CASE("rot 3a <"); rot_bit<R, 3, false, false>(a, t, k); QS_CASE("rot 3a >"); break;
+            case DG_ROT + 13: QS_CASE("rot 3b <"); rot_bit<R, 3, true, false>(a, t, k); QS_CASE("rot 3b >"); break;
+            case DG_ROT + 14: QS_CASE("rot 3c <"); rot_bit<R, 3, false, true>(a, t, k); QS_CASE("rot 3c >"); break;
+            default: QS_CASE("rot 3d <"); rot_bit<R, 3, true, true>(a, t, k); QS_CASE("rot 3d >"); break;
         }
     } else {
         // narrow tiles (fewer than four register bits): general 2x2 on the bit
+        const int r1 = (kind - DG_ROT) & 3;
         double m[8] = {1.0, 0.0, -t, 0.0, t, 0.0, 1.0, 0.0};
         if (r1 & 1) {
             m[0] = t; m[2] = -1.0; m[4] = 1.0; m[6] = t;
         }
         if (r1 & 2) {
-            const cd2 r0 = cmul(k, mk2(m[4], m[5])), r1 = cmul(k, mk2(m[6], m[7]));
-            m[4] = r0.x; m[5] = r0.y; m[6] = r1.x; m[7] = r1.y;
+            const cd2 r0 = cmul(k, mk2(m[4], m[5])), r1v = cmul(k, mk2(m[6], m[7]));
+            m[4] = r0.x; m[5] = r0.y; m[6] = r1v.x; m[7] = r1v.y;
         }
         DevGate H = G;
+        H.r0 = (kind - DG_ROT) >> 2;
         for (int i = 0; i < 8; ++i) H.m[i] = m[i];
         apply_u1s<R>(a, H);
     }
@@ -515,8 +521,8 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
         // consumed here, so all of them are loaded (and waited for) before the branch
         asm volatile("" ::"s"(kind), "s"(r0), "s"(r1), "s"(qa), "s"(m0), "s"(m1), "s"(m2),
                      "s"(m3));
-        if (kind == DG_ROT) {
-            apply_rot<R>(a, G, r0, r1, m0, mk2(m2, m3));
+        if (kind >= DG_ROT) {
+            apply_rot<R>(a, G, kind, m0, mk2(m2, m3));
         } else if (kind == DG_U1X) {
             u1_bit<R, 0>(a, G.m);
             if constexpr (R > 1) u1_bit<R, (R > 1 ? 1 : 0)>(a, G.m + 8);

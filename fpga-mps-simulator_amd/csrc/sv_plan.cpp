@@ -631,10 +631,10 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
         if (inc[j]->rot) {
             close_u1();
             DevGate d{};
-            d.kind = DG_ROT;
             d.r0 = r0;
             const bool post = inc[j]->post != cd(1.0, 0.0);
             d.r1 = (inc[j]->rot - 1) | (post ? 2 : 0);
+            d.kind = DG_ROT + 4 * d.r0 + d.r1;
             d.qa = d.qb = -1;
             d.m[0] = inc[j]->t;
             d.m[2] = inc[j]->post.real();

@@ -8,8 +8,10 @@ target outside the tile) shows up as a mismatch against the oracle.
 
 import numpy as np
 
-(DG_U1, DG_U2, DG_DIAG1, DG_DIAG2, DG_U1X, DG_PHASE, DG_LAYOUT, DG_PHASE_SIGN, DG_U1S, DG_ROT,
- DG_SCALE) = range(11)
+(DG_U1, DG_U2, DG_DIAG1, DG_DIAG2, DG_U1X, DG_PHASE, DG_LAYOUT, DG_PHASE_SIGN,
+ DG_U1S) = range(9)
+DG_SCALE = 10
+DG_ROT = 16          # DG_ROT + 4 * register bit + variant
 PT_BB, PT_BT, PT_BR, PT_T1, PT_TT, PT_TR = 0, 1, 2, 3, 4, 5
 PAIRS = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
 
@@ -70,10 +72,11 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                         if not (g['r0'] >> b) & 1:
                             assert np.array_equal(mb, np.eye(2))
                         _apply_1q(st, idx, sel, tq[regs[b]], mb)
-                elif g['kind'] == DG_ROT:
+                elif DG_ROT <= g['kind'] < DG_ROT + 16:
                     # scaled rotation: form A [[1, -t], [t, 1]], form B [[t, -1], [1, t]];
-                    # r1 bit 1: second row times k
+                    # r1 bit 1: second row times k; bit and variant are also in the kind
                     assert cm == 0 and 0 <= g['r0'] < R
+                    assert g['kind'] == DG_ROT + 4 * g['r0'] + g['r1']
                     t = g['m'][0]
                     m = (np.array([[t, -1], [1, t]], dtype=complex) if g['r1'] & 1
                          else np.array([[1, -t], [t, 1]], dtype=complex))
@@ -112,7 +115,7 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                 else:
                     raise AssertionError(f"unknown kind {g['kind']}")
                 # every non-diagonal target must be a tile qubit
-                if g['kind'] in (DG_U1, DG_U2, DG_U1S, DG_ROT):
+                if g['kind'] in (DG_U1, DG_U2, DG_U1S) or g['kind'] >= DG_ROT:
                     assert (tile >> tq[regs[g['r0']]]) & 1
                 if g['kind'] == DG_U1:
                     assert cm != 0, 'uncontrolled one-qubit gates travel as U1X, U1S or ROT'
