@@ -72,9 +72,14 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //             (t x0 - x1, x0 + t x1); r1 bit 1 set multiplies the second output by
 //             k = (m[2], m[3]).  Bit and variant live in the kind so that the kernel
 //             selects the straight-line body with one switch.
+//   DG_ROTS + mask - 1 (15 kinds): first-form rotations with output factors on every
+//             register bit of mask = r0 (1..15), one op: bit b has t = m[b] and
+//             k = (m[4 + 2b], m[5 + 2b]).  Rotations on distinct bits commute, so a run
+//             of them shares one dispatch.
 enum : int32_t {
     DG_U1 = 0, DG_U2 = 1, DG_DIAG1 = 2, DG_DIAG2 = 3, DG_U1X = 4, DG_PHASE = 5,
-    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8, DG_SCALE = 10, DG_ROT = 16
+    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8, DG_SCALE = 10, DG_ROT = 16,
+    DG_ROTS = 32
 };
 
 struct alignas(16) DevGate {

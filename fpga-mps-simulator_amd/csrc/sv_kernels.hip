@@ -266,6 +266,43 @@ __device__ __forceinline__ void apply_rot(cd2 (&a)[1 << R], const DevGate& G, in
 }
 #undef QS_ROT_CASES
 
+// First-form rotations with output factors on the register bits of M, in one body.
+template <int R, int M>
+__device__ __forceinline__ void rots_body(cd2 (&a)[1 << R], const double* t, const cd2* k) {
+    if constexpr ((M & 1) != 0) rot_bit<R, 0, false, true>(a, t[0], k[0]);
+    if constexpr ((M & 2) != 0 && R > 1) rot_bit<R, (R > 1 ? 1 : 0), false, true>(a, t[1], k[1]);
+    if constexpr ((M & 4) != 0 && R > 2) rot_bit<R, (R > 2 ? 2 : 0), false, true>(a, t[2], k[2]);
+    if constexpr ((M & 8) != 0 && R > 3) rot_bit<R, (R > 3 ? 3 : 0), false, true>(a, t[3], k[3]);
+}
+
+#define QS_ROTS_CASE(M) \
+    case DG_ROTS + M - 1: QS_CASE("rots " #M " <"); rots_body<R, M>(a, t, k); QS_CASE("rots " #M " >"); break;
+
+// DG_ROTS + mask - 1: t from the op head, output factors from the next 64 bytes.
+template <int R>
+__device__ __forceinline__ void apply_rots(cd2 (&a)[1 << R], const DevGate& G, int kind,
+                                           const double* t) {
+    const double k0r = G.m[4], k0i = G.m[5], k1r = G.m[6], k1i = G.m[7];
+    const double k2r = G.m[8], k2i = G.m[9], k3r = G.m[10], k3i = G.m[11];
+    // one batch of scalar loads for the eight values
+    asm volatile("" ::"s"(k0r), "s"(k0i), "s"(k1r), "s"(k1i), "s"(k2r), "s"(k2i), "s"(k3r),
+                 "s"(k3i));
+    const cd2 k[4] = {mk2(k0r, k0i), mk2(k1r, k1i), mk2(k2r, k2i), mk2(k3r, k3i)};
+    if constexpr (R == 4) {
+        switch (kind) {
+            QS_ROTS_CASE(1) QS_ROTS_CASE(2) QS_ROTS_CASE(3) QS_ROTS_CASE(4) QS_ROTS_CASE(5)
+            QS_ROTS_CASE(6) QS_ROTS_CASE(7) QS_ROTS_CASE(8) QS_ROTS_CASE(9) QS_ROTS_CASE(10)
+            QS_ROTS_CASE(11) QS_ROTS_CASE(12) QS_ROTS_CASE(13) QS_ROTS_CASE(14)
+            default: QS_CASE("rots 15 <"); rots_body<R, 15>(a, t, k); QS_CASE("rots 15 >"); break;
+        }
+    } else {
+        const int mask = kind - DG_ROTS + 1;
+        for (int b = 0; b < R; ++b)
+            if ((mask >> b) & 1) apply_rot<R>(a, G, DG_ROT + 4 * b + 2, t[b], k[b]);
+    }
+}
+#undef QS_ROTS_CASE
+
 template <int R>
 __device__ __forceinline__ void apply_cu1(cd2 (&a)[1 << R], const DevGate& G,
                                           const Layout<R>& L, uint64_t base) {
@@ -521,7 +558,9 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
         // consumed here, so all of them are loaded (and waited for) before the branch
         asm volatile("" ::"s"(kind), "s"(r0), "s"(r1), "s"(qa), "s"(m0), "s"(m1), "s"(m2),
                      "s"(m3));
-        if (kind >= DG_ROT) {
+        if (kind >= DG_ROTS) {
+            apply_rots<R>(a, G, kind, H.m);
+        } else if (kind >= DG_ROT) {
             apply_rot<R>(a, G, kind, m0, mk2(m2, m3));
         } else if (kind == DG_U1X) {
             u1_bit<R, 0>(a, G.m);

@@ -549,9 +549,10 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
     StageScheduler::State st = S.initial();
     int cur[4] = {0, 0, 0, 0};
     // Constant factors of the rotations emitted since the last DG_SCALE.  A scaled
-    // rotation grows amplitudes by up to sqrt(2), so the factor is applied whenever it
-    // falls below kScaleFloor (amplitudes within 1e30 of their true size, far from
-    // overflow, however many rotations a pass holds) and once at the end of the pass.
+    // rotation grows amplitudes by the inverse of its factor (at most 1e3 sqrt(2)), so
+    // the factor is applied whenever it falls below kScaleFloor (amplitudes within
+    // about 1e33 of their true size, far from overflow, however many rotations a pass
+    // holds) and once at the end of the pass.
     constexpr double kScaleFloor = 1e-30;
     cd scale = 1.0;
     auto emit_scale = [&]() {
@@ -625,22 +626,63 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
             plan->gates.push_back(d);
         }
     };
+    // Index of the last rotation op that later first-form rotations with an output
+    // factor may join (a DG_ROT of that variant or a DG_ROTS), valid while it is the
+    // last op emitted.
+    int open_rots = -1;
     auto emit_gate = [&](int j) {
         const HostGate& g = inc[j]->g;
         const int r0 = reg_index(local_pos(P.tq, K, g.q0));
         if (inc[j]->rot) {
             close_u1();
-            DevGate d{};
-            d.r0 = r0;
-            const bool post = inc[j]->post != cd(1.0, 0.0);
-            d.r1 = (inc[j]->rot - 1) | (post ? 2 : 0);
-            d.kind = DG_ROT + 4 * d.r0 + d.r1;
-            d.qa = d.qb = -1;
-            d.m[0] = inc[j]->t;
-            d.m[2] = inc[j]->post.real();
-            d.m[3] = inc[j]->post.imag();
-            plan->gates.push_back(d);
-            scale *= inc[j]->scale;
+            // Second form as first form: [[t, -1], [1, t]] = t [[1, -1/t], [1/t, 1]],
+            // with t joining the constant factor.  The rounding error stays that of the
+            // second form; only amplitudes grow, by at most 1/kSecondFormMin per
+            // rotation before the next DG_SCALE.
+            constexpr double kSecondFormMin = 1e-3;
+            int form = inc[j]->rot;
+            double t = inc[j]->t;
+            cd sc = inc[j]->scale;
+            if (form == 2 && t >= kSecondFormMin) {
+                form = 1;
+                sc *= t;
+                t = 1.0 / t;
+            }
+            const cd k = inc[j]->post;
+            const bool post = k != cd(1.0, 0.0);
+            const bool joins = form == 1 && post;
+            DevGate* o = (joins && open_rots >= 0 && open_rots == (int)plan->gates.size() - 1)
+                             ? &plan->gates[open_rots]
+                             : nullptr;
+            const int omask = !o ? 0 : o->kind >= DG_ROTS ? o->r0 : 1 << o->r0;
+            if (o && !((omask >> r0) & 1)) {
+                if (o->kind < DG_ROTS) {          // the single rotation becomes bit r0 of a run
+                    const int b = o->r0;
+                    const double tb = o->m[0], kr = o->m[2], ki = o->m[3];
+                    std::fill(o->m, o->m + 12, 0.0);
+                    o->m[b] = tb;
+                    o->m[4 + 2 * b] = kr;
+                    o->m[5 + 2 * b] = ki;
+                    o->r1 = 0;
+                }
+                o->r0 = omask | (1 << r0);
+                o->kind = DG_ROTS + o->r0 - 1;
+                o->m[r0] = t;
+                o->m[4 + 2 * r0] = k.real();
+                o->m[5 + 2 * r0] = k.imag();
+            } else {
+                DevGate d{};
+                d.r0 = r0;
+                d.r1 = (form - 1) | (post ? 2 : 0);
+                d.kind = DG_ROT + 4 * d.r0 + d.r1;
+                d.qa = d.qb = -1;
+                d.m[0] = t;
+                d.m[2] = k.real();
+                d.m[3] = k.imag();
+                plan->gates.push_back(d);
+                open_rots = joins ? (int)plan->gates.size() - 1 : -1;
+            }
+            scale *= sc;
             if (std::abs(scale) < kScaleFloor) emit_scale();
             return;
         }

@@ -12,6 +12,8 @@ import numpy as np
  DG_U1S) = range(9)
 DG_SCALE = 10
 DG_ROT = 16          # DG_ROT + 4 * register bit + variant
+DG_ROTS = 32         # DG_ROTS + mask - 1: first-form rotations with factors on mask's bits
+SECOND_FORM_MIN = 1e-3
 PT_BB, PT_BT, PT_BR, PT_T1, PT_TT, PT_TR = 0, 1, 2, 3, 4, 5
 PAIRS = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
 
@@ -80,10 +82,25 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                     t = g['m'][0]
                     m = (np.array([[t, -1], [1, t]], dtype=complex) if g['r1'] & 1
                          else np.array([[1, -t], [t, 1]], dtype=complex))
-                    assert 0.0 <= t <= 1.0
+                    # the planner writes second-form rotations in the first form unless
+                    # t is tiny, so first-form t reaches 1 / SECOND_FORM_MIN
+                    assert (0.0 <= t < SECOND_FORM_MIN if g['r1'] & 1
+                            else 0.0 <= t <= 1.0 / SECOND_FORM_MIN)
                     if g['r1'] & 2:
                         m[1] *= g['m'][2] + 1j * g['m'][3]
                     _apply_1q(st, idx, sel, tq[regs[g['r0']]], m)
+                elif DG_ROTS <= g['kind'] < DG_ROTS + 15:
+                    mask = g['r0']
+                    assert cm == 0 and g['kind'] == DG_ROTS + mask - 1 and 0 < mask < (1 << R)
+                    assert bin(mask).count('1') >= 2, 'a lone rotation stays a DG_ROT'
+                    for b in range(R):
+                        if (mask >> b) & 1:
+                            t = g['m'][b]
+                            assert 0.0 <= t <= 1.0 / SECOND_FORM_MIN
+                            m = np.array([[1, -t], [t, 1]], dtype=complex)
+                            m[1] *= g['m'][4 + 2 * b] + 1j * g['m'][5 + 2 * b]
+                            _apply_1q(st, idx, sel, tq[regs[b]], m)
+                            assert (tile >> tq[regs[b]]) & 1
                 elif g['kind'] == DG_SCALE:
                     f = g['m'][0] + 1j * g['m'][1]
                     assert (g['r0'] == 1) == (g['m'][1] == 0.0)
@@ -115,7 +132,7 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                 else:
                     raise AssertionError(f"unknown kind {g['kind']}")
                 # every non-diagonal target must be a tile qubit
-                if g['kind'] in (DG_U1, DG_U2, DG_U1S) or g['kind'] >= DG_ROT:
+                if g['kind'] in (DG_U1, DG_U2, DG_U1S) or DG_ROT <= g['kind'] < DG_ROTS:
                     assert (tile >> tq[regs[g['r0']]]) & 1
                 if g['kind'] == DG_U1:
                     assert cm != 0, 'uncontrolled one-qubit gates travel as U1X, U1S or ROT'
