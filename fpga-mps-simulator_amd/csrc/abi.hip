@@ -265,7 +265,9 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
                 os << (gi > S.g0 ? "," : "") << "{\"kind\":" << G.kind << ",\"r0\":" << G.r0
                    << ",\"r1\":" << G.r1 << ",\"qa\":" << G.qa << ",\"qb\":" << G.qb
                    << ",\"cmask\":" << G.cmask << ",\"m\":[";
-                for (int k = 0; k < 32; ++k) os << (k ? "," : "") << G.m[k];
+                // a SIGN op's m holds its SignDesc words (printed under "phase")
+                for (int k = 0; k < 32; ++k)
+                    os << (k ? "," : "") << (G.kind == DG_PHASE_SIGN ? 0.0 : G.m[k]);
                 os << "]";
                 if (G.kind == DG_PHASE || G.kind == DG_PHASE_SIGN) {
                     const DevPhase& Q = plan.phases[P.phase0 + G.qa];

@@ -610,15 +610,14 @@ using namespace qs;
 // ------------------------------------------------- Jacobi SVD, chi_max up to 128
 // theta up to 256 x 256 (1 MiB) does not fit on chip, so the columns live in a working
 // copy W (column-major, next to theta in the task's workspace slot, L2-resident while
-// the task runs).  Columns are paired by index with the round-robin tournament: in round
-// r the player in slot s >= 1 is 1 + (s - 1 + r) mod (2P - 1), slot 0 keeps player 0,
-// and pair i is (slot i, slot 2P - 1 - i); every column belongs to one pair per round,
-// so a round is read-rotate-write in place and one barrier.  32 teams of 32 lanes; team
-// t handles pairs t, t + 32, ... (8 rows per lane per column in registers).
+// the task runs) and travel through LDS in blocks of kBlkCols columns (below).  Pairings
+// follow the round-robin tournament: in round r the player in slot s >= 1 is
+// 1 + (s - 1 + r) mod (2P - 1), slot 0 keeps player 0, and pair i is
+// (slot i, slot 2P - 1 - i), so every player meets every other once per 2P - 1 rounds.
+// 1024 threads; the Frobenius norm and the final column norms use 32 teams of 32 lanes.
 constexpr int kBigLanes = 32;
 constexpr int kBigTeams = 32;
 constexpr int kBigThreads = kBigTeams * kBigLanes;    // 1024
-constexpr int kBigRows = 2 * kMaxChi / kBigLanes;     // 8 rows per lane
 
 __device__ __forceinline__ double big_sum(double v) {
     v += __shfl_xor(v, 16, kBigLanes);
@@ -669,7 +668,6 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
         return;
     }
     const int m = 2 * d.cl, n = 2 * d.cr;
-    const int P = n / 2;
     const int tid = threadIdx.x;
     const int team = tid / kBigLanes, l16 = tid % kBigLanes;
     const cd2* th = ws + (size_t)blockIdx.x * slot;

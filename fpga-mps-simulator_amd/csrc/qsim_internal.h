@@ -64,7 +64,7 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //   DG_DIAG1/2 general diagonal on global qubits qa (MSB for DIAG2) and qb, evaluated
 //             per amplitude (non-unit entries or several controls)
 //   DG_PHASE / DG_PHASE_SIGN  unitary diagonal group, DevPhase record qa; the SIGN form
-//             has only +-1 factors
+//             has only +-1 factors and carries its SignDesc at the start of m
 //   DG_LAYOUT transpose through LDS to the DevLayout stored at the start of m
 //   DG_SCALE  every amplitude times (m[0], m[1]); r0 = 1 when the factor is real
 //   DG_ROT + 4 r0 + r1 (16 kinds): scaled real rotation on register bit r0, t = m[0]:
@@ -129,6 +129,15 @@ struct alignas(16) DevTerm {
 // with one entry per thread: for SIGN groups a 16-bit mask of the register indices to
 // negate (constant, thread, register and register-pair factors together), otherwise
 // the complex factors (A_t, kappa_0..3 of thread t), PHASE_THREAD_VALUES per thread.
+// The SIGN-group part of a DevPhase, also written at the start of the group's op
+// record (DevGate::m) so that it arrives with the op head in one batch of scalar loads.
+struct SignDesc {
+    uint32_t m0;
+    uint32_t mb[SV_THREAD_BITS];
+    uint32_t pp[SV_THREAD_BITS];
+    int32_t slot;
+};
+
 struct alignas(16) DevPhase {
     int32_t t0, nt;        // thread-class terms in the pass's term table
     int32_t b0, nb;        // block-class terms in the pass's term table

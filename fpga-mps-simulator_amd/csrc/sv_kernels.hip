@@ -82,14 +82,16 @@ __device__ __forceinline__ void make_layout(Layout<R>& L, const DevLayout& D, in
     L.glob_t = g;
 }
 
-// Leading part of a DevGate record (kind through m[3]).
+// Leading part of a DevGate record (kind through m[11]): everything DG_ROT, DG_ROTS,
+// DG_PHASE_SIGN (its SignDesc) and DG_SCALE read.
 struct alignas(16) OpHead {
     int32_t kind, r0, r1, qa, qb, pad;
     uint64_t cmask;
-    double m[4];
+    double m[12];
 };
-static_assert(sizeof(OpHead) == 64 && offsetof(DevGate, m) == offsetof(OpHead, m),
+static_assert(sizeof(OpHead) == 128 && offsetof(DevGate, m) == offsetof(OpHead, m),
               "OpHead must be the leading part of DevGate");
+static_assert(sizeof(SignDesc) <= sizeof(double) * 12, "SignDesc must lie in the op head");
 
 // Global index bits of tile `tile`: bit i of the tile index lands on block qubit bq[i].
 // Unrolled over the pass's runs of consecutive block qubits so that the run table
@@ -278,16 +280,12 @@ __device__ __forceinline__ void rots_body(cd2 (&a)[1 << R], const double* t, con
 #define QS_ROTS_CASE(M) \
     case DG_ROTS + M - 1: QS_CASE("rots " #M " <"); rots_body<R, M>(a, t, k); QS_CASE("rots " #M " >"); break;
 
-// DG_ROTS + mask - 1: t from the op head, output factors from the next 64 bytes.
+// DG_ROTS + mask - 1: t = m[0..3] and the output factors m[4..11], all in the op head.
 template <int R>
 __device__ __forceinline__ void apply_rots(cd2 (&a)[1 << R], const DevGate& G, int kind,
-                                           const double* t) {
-    const double k0r = G.m[4], k0i = G.m[5], k1r = G.m[6], k1i = G.m[7];
-    const double k2r = G.m[8], k2i = G.m[9], k3r = G.m[10], k3i = G.m[11];
-    // one batch of scalar loads for the eight values
-    asm volatile("" ::"s"(k0r), "s"(k0i), "s"(k1r), "s"(k1i), "s"(k2r), "s"(k2i), "s"(k3r),
-                 "s"(k3i));
-    const cd2 k[4] = {mk2(k0r, k0i), mk2(k1r, k1i), mk2(k2r, k2i), mk2(k3r, k3i)};
+                                           const double* m) {
+    const double* t = m;
+    const cd2 k[4] = {mk2(m[4], m[5]), mk2(m[6], m[7]), mk2(m[8], m[9]), mk2(m[10], m[11])};
     if constexpr (R == 4) {
         switch (kind) {
             QS_ROTS_CASE(1) QS_ROTS_CASE(2) QS_ROTS_CASE(3) QS_ROTS_CASE(4) QS_ROTS_CASE(5)
@@ -372,7 +370,7 @@ __device__ __forceinline__ constexpr unsigned reg_col_mask(int r) {
 // indices to negate.  Thread t's mask is rebuilt from the group's descriptor (uniform,
 // scalar loads) rather than read from a per-thread table: a vector load here would
 // queue behind the tile's HBM traffic.
-__device__ __forceinline__ unsigned sign_mask(const DevPhase& Q, int t) {
+__device__ __forceinline__ unsigned sign_mask(const SignDesc& Q, int t) {
     unsigned M = Q.m0;
     // Uniform branches: most groups have no thread-bit or no thread-pair terms.
     unsigned any_mb = 0, any_pp = 0;
@@ -557,7 +555,8 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
         const double m0 = H.m[0], m1 = H.m[1], m2 = H.m[2], m3 = H.m[3];
         // consumed here, so all of them are loaded (and waited for) before the branch
         asm volatile("" ::"s"(kind), "s"(r0), "s"(r1), "s"(qa), "s"(m0), "s"(m1), "s"(m2),
-                     "s"(m3));
+                     "s"(m3), "s"(H.m[4]), "s"(H.m[5]), "s"(H.m[6]), "s"(H.m[7]),
+                     "s"(H.m[8]), "s"(H.m[9]), "s"(H.m[10]), "s"(H.m[11]));
         if (kind >= DG_ROTS) {
             apply_rots<R>(a, G, kind, H.m);
         } else if (kind >= DG_ROT) {
@@ -569,19 +568,22 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
             if constexpr (R > 3) u1_bit<R, (R > 3 ? 3 : 0)>(a, G.m + 24);
         } else if (kind == DG_U1S) {
             apply_u1s<R>(a, G);
-        } else if (kind == DG_PHASE_SIGN || kind == DG_PHASE) {
+        } else if (kind == DG_PHASE_SIGN) {
+            const SignDesc& Q = *reinterpret_cast<const SignDesc*>(G.m);
+            const const_cd2_ptr bf =
+                Q.slot >= 0 ? (const_cd2_ptr)(tile_factors +
+                                              (size_t)Q.slot * ntiles * PHASE_BLOCK_VALUES)
+                            : nullptr;
+            apply_phase_sign<R>(a, sign_mask(Q, t), bf, t);
+        } else if (kind == DG_PHASE) {
             const DevPhase& Q = ph[qa];
             const const_cd2_ptr bf =
                 Q.slot >= 0 ? (const_cd2_ptr)(tile_factors +
                                               (size_t)Q.slot * ntiles * PHASE_BLOCK_VALUES)
                             : nullptr;
-            if (kind == DG_PHASE_SIGN) {
-                apply_phase_sign<R>(a, sign_mask(Q, t), bf, t);
-            } else {
-                const cd2* e = tfactors + (size_t)(Q.tab + t) * PHASE_THREAD_VALUES;
-                const cd2 f[PHASE_THREAD_VALUES] = {e[0], e[1], e[2], e[3], e[4]};
-                apply_phase_cplx<R>(a, Q, f, bf, t);
-            }
+            const cd2* e = tfactors + (size_t)(Q.tab + t) * PHASE_THREAD_VALUES;
+            const cd2 f[PHASE_THREAD_VALUES] = {e[0], e[1], e[2], e[3], e[4]};
+            apply_phase_cplx<R>(a, Q, f, bf, t);
         } else if (kind == DG_LAYOUT) {
 #pragma unroll
             for (int j = 0; j < NR; ++j) lds[lds_slot(Lcur.loc(j))] = a[j];

@@ -316,6 +316,15 @@ void emit_phase_group(const std::vector<const HostGate*>& diag, const int* regs,
     ph.slot = bt.empty() ? -1 : P.nslot++;
     DevGate d{};
     d.kind = sign ? DG_PHASE_SIGN : DG_PHASE;
+    if (sign) {
+        SignDesc sd{};
+        sd.m0 = ph.m0;
+        std::memcpy(sd.mb, ph.mb, sizeof(sd.mb));
+        std::memcpy(sd.pp, ph.pp, sizeof(sd.pp));
+        sd.slot = ph.slot;
+        static_assert(sizeof(SignDesc) <= sizeof(d.m), "SignDesc must fit in DevGate::m");
+        std::memcpy(d.m, &sd, sizeof(sd));
+    }
     d.qa = (int)plan->phases.size() - P.phase0;
     d.qb = -1;
     plan->phases.push_back(ph);
