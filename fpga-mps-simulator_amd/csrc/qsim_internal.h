@@ -61,6 +61,8 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //   DG_U1S    uncontrolled one-qubit gate on register bit r0, matrix m[0..7]
 //   DG_U1     controlled one-qubit gate on register bit r0, control mask cmask
 //   DG_U2     two-qubit gate on register bits r0 (row-index MSB) and r1, controls cmask
+//   DG_SWAP   uncontrolled SWAP of register-bit pair r0 of (0,1) (0,2) (0,3) (1,2)
+//             (1,3) (2,3), four-bit register blocks only: a register permutation
 //   DG_DIAG1/2 general diagonal on global qubits qa (MSB for DIAG2) and qb, evaluated
 //             per amplitude (non-unit entries or several controls)
 //   DG_PHASE / DG_PHASE_SIGN  unitary diagonal group, DevPhase record qa; the SIGN form
@@ -78,7 +80,7 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //             of them shares one dispatch.
 enum : int32_t {
     DG_U1 = 0, DG_U2 = 1, DG_DIAG1 = 2, DG_DIAG2 = 3, DG_U1X = 4, DG_PHASE = 5,
-    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8, DG_SCALE = 10, DG_ROT = 16,
+    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8, DG_SWAP = 9, DG_SCALE = 10, DG_ROT = 16,
     DG_ROTS = 32
 };
 

@@ -333,6 +333,36 @@ __device__ __forceinline__ void apply_u2(cd2 (&a)[1 << R], const DevGate& G,
     }
 }
 
+// Uncontrolled SWAP of register bits B0 and B1: a permutation of the register block.
+template <int R, int B0, int B1>
+__device__ __forceinline__ void swap_bits(cd2 (&a)[1 << R]) {
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        if (((j >> B0) & 1) && !((j >> B1) & 1)) {   // compile-time after unrolling
+            const int k = j ^ (1 << B0) ^ (1 << B1);
+            const cd2 x = a[j];
+            a[j] = a[k];
+            a[k] = x;
+        }
+    }
+}
+
+// DG_SWAP: r0 indexes the register-bit pairs (0,1) (0,2) (0,3) (1,2) (1,3) (2,3);
+// the planner emits it for four-bit register blocks only.
+template <int R>
+__device__ __forceinline__ void apply_swap(cd2 (&a)[1 << R], int pair) {
+    if constexpr (R == 4) {
+        switch (pair) {
+            case 0: QS_CASE("swap 01 <"); swap_bits<R, 0, 1>(a); QS_CASE("swap 01 >"); break;
+            case 1: QS_CASE("swap 02 <"); swap_bits<R, 0, 2>(a); QS_CASE("swap 02 >"); break;
+            case 2: QS_CASE("swap 03 <"); swap_bits<R, 0, 3>(a); QS_CASE("swap 03 >"); break;
+            case 3: QS_CASE("swap 12 <"); swap_bits<R, 1, 2>(a); QS_CASE("swap 12 >"); break;
+            case 4: QS_CASE("swap 13 <"); swap_bits<R, 1, 3>(a); QS_CASE("swap 13 >"); break;
+            default: QS_CASE("swap 23 <"); swap_bits<R, 2, 3>(a); QS_CASE("swap 23 >"); break;
+        }
+    }
+}
+
 // General diagonal (non-unit entries or several controls), per amplitude; amplitudes
 // without every control bit are multiplied by exactly 1.
 template <int R>
@@ -596,6 +626,8 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
             apply_cu1<R>(a, G, Lcur, base);
         } else if (kind == DG_U2) {
             apply_u2<R>(a, G, Lcur, base);
+        } else if (kind == DG_SWAP) {
+            apply_swap<R>(a, r0);
         } else if (kind == DG_SCALE) {
             const cd2 f = mk2(m0, m1);
             if (r0) {

@@ -114,6 +114,19 @@ bool split_rotation(const double* m, int& form, double& t, cd& pre, cd& post, cd
     return true;
 }
 
+// True for the SWAP matrix exactly (row-major complex 4x4).
+bool is_swap(const double* m) {
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            const double want = (r == c && (r == 0 || r == 3)) || (r == 1 && c == 2) ||
+                                        (r == 2 && c == 1)
+                                    ? 1.0
+                                    : 0.0;
+            if (m[2 * (4 * r + c)] != want || m[2 * (4 * r + c) + 1] != 0.0) return false;
+        }
+    return true;
+}
+
 // ------------------------------------------------------------------ phase groups
 
 struct PTerm {
@@ -715,6 +728,14 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
             d.kind = DG_U2;
             d.r1 = reg_index(local_pos(P.tq, K, g.q1));
             std::memcpy(d.m, g.m, sizeof(double) * 32);
+            if (R == 4 && g.cmask == 0 && is_swap(g.m)) {
+                static const int pair_of[4][4] = {
+                    {-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
+                d.kind = DG_SWAP;
+                d.r0 = pair_of[d.r0][d.r1];
+                d.r1 = 0;
+                std::fill(d.m, d.m + 32, 0.0);
+            }
         }
         plan->gates.push_back(d);
     };

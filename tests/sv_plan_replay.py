@@ -9,7 +9,7 @@ target outside the tile) shows up as a mismatch against the oracle.
 import numpy as np
 
 (DG_U1, DG_U2, DG_DIAG1, DG_DIAG2, DG_U1X, DG_PHASE, DG_LAYOUT, DG_PHASE_SIGN,
- DG_U1S) = range(9)
+ DG_U1S, DG_SWAP) = range(10)
 DG_SCALE = 10
 DG_ROT = 16          # DG_ROT + 4 * register bit + variant
 DG_ROTS = 32         # DG_ROTS + mask - 1: first-form rotations with factors on mask's bits
@@ -118,6 +118,14 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                     out = m @ v
                     for k in range(4):
                         st[ii[k]] = out[k]
+                elif g['kind'] == DG_SWAP:
+                    assert cm == 0 and R == 4
+                    ba, bb = PAIRS[g['r0']]
+                    qa, qb = tq[regs[ba]], tq[regs[bb]]
+                    assert (tile >> qa) & 1 and (tile >> qb) & 1
+                    i10 = idx[(((idx >> qa) & 1) == 1) & (((idx >> qb) & 1) == 0)]
+                    i01 = i10 ^ (1 << qa) ^ (1 << qb)
+                    st[i10], st[i01] = st[i01].copy(), st[i10].copy()
                 elif g['kind'] in (DG_PHASE, DG_PHASE_SIGN):
                     assert (g['kind'] == DG_PHASE_SIGN) == bool(g['phase']['sign'])
                     st *= phase_factor(g['phase'], tq, regs, idx)
