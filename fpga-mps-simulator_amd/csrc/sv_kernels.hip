@@ -92,6 +92,9 @@ struct alignas(16) OpHead {
 static_assert(sizeof(OpHead) == 128 && offsetof(DevGate, m) == offsetof(OpHead, m),
               "OpHead must be the leading part of DevGate");
 static_assert(sizeof(SignDesc) <= sizeof(double) * 12, "SignDesc must lie in the op head");
+static_assert(offsetof(SignDesc, mb) == 4 && offsetof(SignDesc, pp) == 4 * (1 + SV_THREAD_BITS) &&
+                  offsetof(SignDesc, slot) == 4 * (1 + 2 * SV_THREAD_BITS),
+              "SignDesc is read as consecutive 32-bit words");
 
 // Global index bits of tile `tile`: bit i of the tile index lands on block qubit bq[i].
 // Unrolled over the pass's runs of consecutive block qubits so that the run table
@@ -410,17 +413,18 @@ __device__ __forceinline__ unsigned sign_mask(const SignDesc& Q, int t) {
         any_pp |= Q.pp[b];
     }
     if (any_mb) {
+        // M ^ (all-ones if thread bit b is set) & mb[b]: one three-input bit op per bit
 #pragma unroll
-        for (int b = 0; b < SV_THREAD_BITS; ++b) M ^= ((t >> b) & 1) ? Q.mb[b] : 0u;
+        for (int b = 0; b < SV_THREAD_BITS; ++b) M ^= (0u - ((unsigned)(t >> b) & 1u)) & Q.mb[b];
     }
     if (any_pp) {
         unsigned par = 0;
 #pragma unroll
         for (int a = 0; a < SV_THREAD_BITS - 1; ++a)
             par ^= (unsigned)(t >> a) & (unsigned)__builtin_popcount(Q.pp[a] & (unsigned)t);
-        M ^= (par & 1u) ? 0xFFFFu : 0u;
+        M ^= 0u - (par & 1u);
     }
-    return M;
+    return M & 0xFFFFu;
 }
 
 template <int R>
@@ -439,10 +443,18 @@ __device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], unsigned M,
         if (R > 2 && (bits & (4u << SV_THREAD_BITS))) M ^= reg_col_mask(2);
         if (R > 3 && (bits & (8u << SV_THREAD_BITS))) M ^= reg_col_mask(3);
     }
+    // Factor +-1 of amplitude j: bit j of M moved to the sign bit of 1.0's high word (an
+    // AND and one shift-OR; left to itself the compiler splits the shift-OR in two),
+    // then two FP64 products.  The products write fresh registers whole; flipping only
+    // the high words would copy the low ones.
+    const unsigned one_hi = 0x3FF00000u;
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
-        const double sg = ((M >> j) & 1u) ? -1.0 : 1.0;
-        a[j] = a[j] * sg;
+        unsigned hi;
+        asm("v_lshl_or_b32 %0, %1, %2, %3"
+            : "=v"(hi)
+            : "v"(M & (1u << j)), "i"(31 - j), "s"(one_hi));
+        a[j] = a[j] * __hiloint2double((int)hi, 0);
     }
 }
 
@@ -599,7 +611,20 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
         } else if (kind == DG_U1S) {
             apply_u1s<R>(a, G);
         } else if (kind == DG_PHASE_SIGN) {
-            const SignDesc& Q = *reinterpret_cast<const SignDesc*>(G.m);
+            // the descriptor's 32-bit words from the head's registers (word 2i is the
+            // low half of m[i])
+            SignDesc Q;
+            auto word = [&](int w) {
+                return (unsigned)((w & 1) ? __double2hiint(H.m[w >> 1])
+                                          : __double2loint(H.m[w >> 1]));
+            };
+            Q.m0 = word(0);
+#pragma unroll
+            for (int b = 0; b < SV_THREAD_BITS; ++b) {
+                Q.mb[b] = word(1 + b);
+                Q.pp[b] = word(1 + SV_THREAD_BITS + b);
+            }
+            Q.slot = (int)word(1 + 2 * SV_THREAD_BITS);
             const const_cd2_ptr bf =
                 Q.slot >= 0 ? (const_cd2_ptr)(tile_factors +
                                               (size_t)Q.slot * ntiles * PHASE_BLOCK_VALUES)
