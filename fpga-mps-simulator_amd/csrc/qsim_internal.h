@@ -66,7 +66,9 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //   DG_DIAG1/2 general diagonal on global qubits qa (MSB for DIAG2) and qb, evaluated
 //             per amplitude (non-unit entries or several controls)
 //   DG_PHASE / DG_PHASE_SIGN  unitary diagonal group, DevPhase record qa; the SIGN form
-//             has only +-1 factors and carries its SignDesc at the start of m
+//             has only +-1 factors and carries its SignDesc at the start of m; DG_PHASE
+//             carries what the kernel reads of the group: its per-thread table entry
+//             qb, block-factor slot aux and register-pair factors crr in m[0..11]
 //   DG_LAYOUT transpose through LDS to the DevLayout stored at the start of m
 //   DG_SCALE  every amplitude times (m[0], m[1]); r0 = 1 when the factor is real
 //   DG_ROT + 4 r0 + r1 (16 kinds): scaled real rotation on register bit r0, t = m[0]:
@@ -88,7 +90,7 @@ struct alignas(16) DevGate {
     int32_t kind;
     int32_t r0, r1;
     int32_t qa, qb;
-    int32_t pad;
+    int32_t aux;           // DG_PHASE: the group's block-factor slot (-1: none)
     uint64_t cmask;        // global control mask
     double m[32];
 };

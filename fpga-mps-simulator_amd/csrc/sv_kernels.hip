@@ -83,9 +83,10 @@ __device__ __forceinline__ void make_layout(Layout<R>& L, const DevLayout& D, in
 }
 
 // Leading part of a DevGate record (kind through m[11]): everything DG_ROT, DG_ROTS,
-// DG_PHASE_SIGN (its SignDesc) and DG_SCALE read.
+// DG_PHASE_SIGN (its SignDesc), DG_PHASE (table entry, slot, register-pair factors) and
+// DG_SCALE read.
 struct alignas(16) OpHead {
-    int32_t kind, r0, r1, qa, qb, pad;
+    int32_t kind, r0, r1, qa, qb, aux;
     uint64_t cmask;
     double m[12];
 };
@@ -463,7 +464,7 @@ __device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], unsigned M,
 // (g[j + 2^t] = g[j] * kappa_t * prod_{r in j} c_rt) and applied once per amplitude.
 // `e` is this thread's table entry [A_t, kappa_0..3].
 template <int R>
-__device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const DevPhase& Q,
+__device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const double* crr,
                                                  const cd2 (&e)[PHASE_THREAD_VALUES],
                                                  const_cd2_ptr bf, int t) {
     cd2 A = e[0], k0 = e[1], k1 = e[2], k2 = e[3], k3 = e[4];
@@ -479,9 +480,10 @@ __device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const DevPhas
         if (R > 2) k2 = cmul(k2, bf[PHASE_KAPPA0 + 2]);
         if (R > 3) k3 = cmul(k3, bf[PHASE_KAPPA0 + 3]);
     }
-    const cd2 c01 = mk2(Q.crr[0][0], Q.crr[0][1]), c02 = mk2(Q.crr[1][0], Q.crr[1][1]);
-    const cd2 c03 = mk2(Q.crr[2][0], Q.crr[2][1]), c12 = mk2(Q.crr[3][0], Q.crr[3][1]);
-    const cd2 c13 = mk2(Q.crr[4][0], Q.crr[4][1]), c23 = mk2(Q.crr[5][0], Q.crr[5][1]);
+    // register-pair factors (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
+    const cd2 c01 = mk2(crr[0], crr[1]), c02 = mk2(crr[2], crr[3]);
+    const cd2 c03 = mk2(crr[4], crr[5]), c12 = mk2(crr[6], crr[7]);
+    const cd2 c13 = mk2(crr[8], crr[9]), c23 = mk2(crr[10], crr[11]);
     // Factors of indices 0..7 are kept for the top bit; each is applied once known.
     const cd2 g0 = A, g1 = cmul(A, k0);
     a[0] = cmul(g0, a[0]);
@@ -578,7 +580,6 @@ __device__ __forceinline__ void lds_barrier() {
 template <int R>
 __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
                                         const DevGate* __restrict__ gt, int nop,
-                                        const DevPhase* __restrict__ ph,
                                         const cd2* __restrict__ tfactors,
                                         const cd2* __restrict__ tile_factors, uint32_t ntiles,
                                         uint64_t base, cd2* __restrict__ lds, int t) {
@@ -596,9 +597,10 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
         const int kind = H.kind, r0 = H.r0, r1 = H.r1, qa = H.qa;
         const double m0 = H.m[0], m1 = H.m[1], m2 = H.m[2], m3 = H.m[3];
         // consumed here, so all of them are loaded (and waited for) before the branch
-        asm volatile("" ::"s"(kind), "s"(r0), "s"(r1), "s"(qa), "s"(m0), "s"(m1), "s"(m2),
-                     "s"(m3), "s"(H.m[4]), "s"(H.m[5]), "s"(H.m[6]), "s"(H.m[7]),
-                     "s"(H.m[8]), "s"(H.m[9]), "s"(H.m[10]), "s"(H.m[11]));
+        asm volatile("" ::"s"(kind), "s"(r0), "s"(r1), "s"(qa), "s"(H.qb), "s"(H.aux),
+                     "s"(m0), "s"(m1), "s"(m2), "s"(m3), "s"(H.m[4]), "s"(H.m[5]),
+                     "s"(H.m[6]), "s"(H.m[7]), "s"(H.m[8]), "s"(H.m[9]), "s"(H.m[10]),
+                     "s"(H.m[11]));
         if (kind >= DG_ROTS) {
             apply_rots<R>(a, G, kind, H.m);
         } else if (kind >= DG_ROT) {
@@ -631,14 +633,14 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
                             : nullptr;
             apply_phase_sign<R>(a, sign_mask(Q, t), bf, t);
         } else if (kind == DG_PHASE) {
-            const DevPhase& Q = ph[qa];
+            const int tab = H.qb, slot = H.aux;
             const const_cd2_ptr bf =
-                Q.slot >= 0 ? (const_cd2_ptr)(tile_factors +
-                                              (size_t)Q.slot * ntiles * PHASE_BLOCK_VALUES)
-                            : nullptr;
-            const cd2* e = tfactors + (size_t)(Q.tab + t) * PHASE_THREAD_VALUES;
+                slot >= 0 ? (const_cd2_ptr)(tile_factors +
+                                            (size_t)slot * ntiles * PHASE_BLOCK_VALUES)
+                          : nullptr;
+            const cd2* e = tfactors + (size_t)(tab + t) * PHASE_THREAD_VALUES;
             const cd2 f[PHASE_THREAD_VALUES] = {e[0], e[1], e[2], e[3], e[4]};
-            apply_phase_cplx<R>(a, Q, f, bf, t);
+            apply_phase_cplx<R>(a, H.m, f, bf, t);
         } else if (kind == DG_LAYOUT) {
 #pragma unroll
             for (int j = 0; j < NR; ++j) lds[lds_slot(Lcur.loc(j))] = a[j];
@@ -678,9 +680,8 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
 template <int R>
 __global__ void __launch_bounds__(1 << (SV_TILE_BITS - SV_REG_BITS))
 sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
-               const DevGate* __restrict__ gates, const DevPhase* __restrict__ phases,
-               const cd2* __restrict__ tfactors, const cd2* __restrict__ factors,
-               uint32_t ntiles) {
+               const DevGate* __restrict__ gates, const cd2* __restrict__ tfactors,
+               const cd2* __restrict__ factors, uint32_t ntiles) {
     constexpr int NR = 1 << R;
     __shared__ cd2 lds[kLdsSlots];
     const int t = threadIdx.x;
@@ -696,8 +697,8 @@ sv_pass_kernel(cd2* __restrict__ state, const DevPass* __restrict__ P,
     cd2 a[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) a[j] = __builtin_nontemporal_load(state + (base | Lcur.glob(j)));
-    run_ops<R>(a, Lcur, gates + P->gate0, P->nop, phases + P->phase0, tfactors, tile_factors,
-               ntiles, base, lds, t);
+    run_ops<R>(a, Lcur, gates + P->gate0, P->nop, tfactors, tile_factors, ntiles, base, lds,
+               t);
 #pragma unroll
     for (int j = 0; j < NR; ++j) __builtin_nontemporal_store(a[j], state + (base | Lcur.glob(j)));
 }
@@ -812,13 +813,13 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
         }
         switch (P.R) {
             case 1: hipLaunchKernelGGL(sv_pass_kernel<1>, dim3(grid), dim3(threads), 0, stream,
-                                       st, dP, dgates, dphases, dtf, dfactors, tiles); break;
+                                       st, dP, dgates, dtf, dfactors, tiles); break;
             case 2: hipLaunchKernelGGL(sv_pass_kernel<2>, dim3(grid), dim3(threads), 0, stream,
-                                       st, dP, dgates, dphases, dtf, dfactors, tiles); break;
+                                       st, dP, dgates, dtf, dfactors, tiles); break;
             case 3: hipLaunchKernelGGL(sv_pass_kernel<3>, dim3(grid), dim3(threads), 0, stream,
-                                       st, dP, dgates, dphases, dtf, dfactors, tiles); break;
+                                       st, dP, dgates, dtf, dfactors, tiles); break;
             default: hipLaunchKernelGGL(sv_pass_kernel<4>, dim3(grid), dim3(threads), 0, stream,
-                                        st, dP, dgates, dphases, dtf, dfactors, tiles); break;
+                                        st, dP, dgates, dtf, dfactors, tiles); break;
         }
         QS_HIP(hipGetLastError());
         if (e1) QS_HIP(hipEventRecord(e1, stream));

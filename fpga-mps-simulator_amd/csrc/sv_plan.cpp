@@ -340,6 +340,14 @@ void emit_phase_group(const std::vector<const HostGate*>& diag, const int* regs,
     }
     d.qa = (int)plan->phases.size() - P.phase0;
     d.qb = -1;
+    if (!sign) {
+        d.qb = ph.tab;
+        d.aux = ph.slot;
+        for (int i = 0; i < 6; ++i) {
+            d.m[2 * i] = ph.crr[i][0];
+            d.m[2 * i + 1] = ph.crr[i][1];
+        }
+    }
     plan->phases.push_back(ph);
     plan->gates.push_back(d);
 }
