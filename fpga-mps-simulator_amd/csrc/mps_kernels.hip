@@ -305,11 +305,28 @@ __device__ __forceinline__ bool jacobi_rotation(double al, double be, double gr,
     return true;
 }
 
-// Sum over the TL lanes of a team (TL a power of two, at most a wavefront).
+// A double moved across lanes by a DPP pattern (two 32-bit moves, no LDS traffic).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the TL lanes of a team (TL a power of two, at most a wavefront).  Inside a
+// 16-lane row the partners come from DPP patterns on the VALU: lane ^ 1 and lane ^ 2
+// (quad permutes), then 7 - i and 15 - i (half-row and row mirrors), which pair the
+// quads and half-rows once each holds a uniform partial sum; wider teams add the
+// rows through cross-lane shuffles.  Partners add the same two values, so every lane
+// of a team ends with bitwise the same sum.
 template <int TL>
 __device__ __forceinline__ double team_sum(double v) {
+    if constexpr (TL >= 2) v += dpp_f64<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+    if constexpr (TL >= 4) v += dpp_f64<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+    if constexpr (TL >= 8) v += dpp_f64<0x141>(v);   // row_half_mirror
+    if constexpr (TL >= 16) v += dpp_f64<0x140>(v);  // row_mirror
 #pragma unroll
-    for (int o = TL / 2; o >= 1; o /= 2) v += __shfl_xor(v, o, TL);
+    for (int o = 16; o < TL; o *= 2) v += __shfl_xor(v, o, TL);
     return v;
 }
 

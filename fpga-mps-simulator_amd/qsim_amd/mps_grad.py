@@ -283,9 +283,11 @@ def energy_and_grad_from_tape(finals, tape, theta, nparams: int, J: float = 1.0,
     ('2q', left site, G, A, B before, A', B' after).
     """
     import torch
-    leaves = [t.detach().requires_grad_(True) for t in finals]
-    e = tfim_energy(leaves, J, h)
-    gs = torch.autograd.grad(e, leaves)
+    # Called from autograd.Function.backward too, where grad mode is off.
+    with torch.enable_grad():
+        leaves = [t.detach().requires_grad_(True) for t in finals]
+        e = tfim_energy(leaves, J, h)
+        gs = torch.autograd.grad(e, leaves)
     cot = {i: g for i, g in enumerate(gs)}
     grad = np.zeros(nparams)
     for rec in reversed(tape):

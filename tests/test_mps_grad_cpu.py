@@ -134,3 +134,19 @@ def test_broadened_kernel_is_bounded_at_degeneracy():
     gS = torch.zeros(2, dtype=torch.float64)
     g = mps_grad.svd_backward(U, S, V, gU, gS, gV)
     assert torch.isfinite(g.real).all() and torch.isfinite(g.imag).all()
+
+
+def test_reverse_sweep_runs_with_grad_mode_off():
+    """torch.autograd.Function.backward runs with grad mode off; the reverse sweep's
+    energy gradient must still build its own graph there."""
+    import torch
+    rng = np.random.default_rng(5)
+    dims = [1, 2, 3, 2, 1]
+    sites = [torch.tensor(rng.standard_normal((dims[i], 2, dims[i + 1]))
+                          + 1j * rng.standard_normal((dims[i], 2, dims[i + 1])),
+                          dtype=torch.complex128) for i in range(4)]
+    e_on, g_on = mps_grad.energy_and_grad_from_tape(sites, [], np.zeros(0), 0)
+    with torch.no_grad():
+        e_off, g_off = mps_grad.energy_and_grad_from_tape(sites, [], np.zeros(0), 0)
+    assert e_on == e_off and g_on.shape == g_off.shape == (0,)
+    assert math.isfinite(e_on)
