@@ -331,14 +331,15 @@ __device__ __forceinline__ double team_sum(double v) {
 }
 
 // Size class CHI (8, 16, 32, 64): theta is at most 2CHI x 2CHI; CHI teams of TL lanes
-// (one team per column pair), 2CHI/TL rows per lane, LDS exchange buffer of CHI columns
-// of 2CHI rows plus one spare.  The team width TL is the kernel's tile shape: narrower
+// (one team per column pair), 2CHI/TL rows per lane, an LDS buffer of CHI columns of
+// 2CHI rows (column exchange, then the kept left singular vectors).  The team width TL is
+// the kernel's tile shape: narrower
 // teams mean fewer threads per task and cheaper reductions, wider ones less work per
 // lane; the launcher takes it from a per-class table (default 16, set by the autotuner).
 template <int CHI>
 struct SvdShared {
     static constexpr int LD = 2 * CHI;
-    cd2 buf[(CHI + 1) * LD];
+    cd2 buf[CHI * LD];
     double sval[2 * CHI];
     int order[2 * CHI];
     int rank[2 * CHI];
@@ -371,8 +372,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
     const int team = tid / TL, l16 = tid % TL;
     const bool act = team < P;
     const cd2* th = ws + (size_t)blockIdx.x * theta_slot;
-    cd2* slot0 = sh.buf;                     // P columns of 2CHI rows
-    cd2* spare = sh.buf + CHI * LD;
+    cd2* slot0 = sh.buf;                     // exchange slot of team t: column t
 
     cd2 ca[RPL], cb[RPL];
 #pragma unroll
@@ -401,80 +401,86 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
     const double zero_sq = 1e-28 * sh.fro;
     const double tol = 2.220446049250313e-16 * sqrt((double)m);
 
+    // Jacobi rotation of the pair (x, y), x at the lower position: a' = c a - s e^{-i phi} b,
+    // b' = s e^{i phi} a + c b with a = x, b = y.
+    auto rotate = [&](cd2 (&x)[RPL], cd2 (&y)[RPL]) __attribute__((always_inline)) {
+        double al = 0, be = 0, gr = 0, gi = 0;
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+            al = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, al));
+            be = fma(y[k].x, y[k].x, fma(y[k].y, y[k].y, be));
+            gr = fma(x[k].x, y[k].x, fma(x[k].y, y[k].y, gr));
+            gi = fma(x[k].x, y[k].y, fma(-x[k].y, y[k].x, gi));
+        }
+        al = team_sum<TL>(al);
+        be = team_sum<TL>(be);
+        gr = team_sum<TL>(gr);
+        gi = team_sum<TL>(gi);
+        double c, sn, er, ei;
+        if (jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, sn, er, ei)) {
+            const cd2 sem = mk2(sn * er, -sn * ei);
+            const cd2 sep = mk2(sn * er, sn * ei);
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) {
+                const cd2 u = x[k], v = y[k];
+                const cd2 sy = cmul(sem, v), sx = cmul(sep, u);
+                x[k] = mk2(c * u.x - sy.x, c * u.y - sy.y);
+                y[k] = mk2(c * v.x + sx.x, c * v.y + sx.y);
+            }
+            if (l16 == 0) sh.rotated = 1;
+        }
+    };
+    // Odd-even ordering.  Columns sit at positions 0..n-1, team t holding 2t and 2t+1.
+    // Step s rotates the pairs (i, i+1), i = s mod 2 (mod 2), and the two columns of a
+    // pair then trade positions; after n steps every pair has met exactly once (and the
+    // order is reversed).  Even steps pair a team's own two columns: no exchange and no
+    // barrier, and the trade only swaps which register holds which position.  Odd steps
+    // pair position 2t+1 of team t with 2t+2 of team t+1: team t+1 hands that column
+    // over through LDS, team t rotates the pair and hands the other column back (two
+    // barriers, one column each way).  lo / hi are the registers holding positions 2t /
+    // 2t+1.  Every team (idle ones too; slot t < CHI) parks lo in its slot and takes it
+    // back at the end, team 0 unchanged, so lo holds no registers while team t works on
+    // hi and the column it received.
+    auto odd_step = [&](cd2 (&lo)[RPL], cd2 (&hi)[RPL]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) slot0[team * LD + l16 + TL * k] = lo[k];
+        __syncthreads();
+        if (act && team < P - 1) {
+            cd2 x[RPL];
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) x[k] = slot0[(team + 1) * LD + l16 + TL * k];
+            rotate(hi, x);
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) {
+                slot0[(team + 1) * LD + l16 + TL * k] = hi[k];
+                hi[k] = x[k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) lo[k] = slot0[team * LD + l16 + TL * k];
+    };
+
     int sweeps = 0;
     for (; sweeps < kMaxSweeps; ++sweeps) {
         if (tid == 0) sh.rotated = 0;
         __syncthreads();
-        const int rounds = (P > 1) ? (2 * P - 1) : 1;
-        for (int rd = 0; rd < rounds; ++rd) {
-            if (act) {
-                double al = 0, be = 0, gr = 0, gi = 0;
+        // n = 2P steps: (even, odd) twice per trip with ca, cb as (lo, hi), then (hi, lo)
+        for (int s = 0; s + 4 <= 2 * P; s += 4) {
+            if (act) rotate(ca, cb);
+            odd_step(cb, ca);
+            if (act) rotate(cb, ca);
+            odd_step(ca, cb);
+        }
+        if (P & 1) {
+            // two steps left; they end with positions 2t, 2t+1 in cb, ca
+            if (act) rotate(ca, cb);
+            odd_step(cb, ca);
 #pragma unroll
-                for (int k = 0; k < RPL; ++k) {
-                    al = fma(ca[k].x, ca[k].x, fma(ca[k].y, ca[k].y, al));
-                    be = fma(cb[k].x, cb[k].x, fma(cb[k].y, cb[k].y, be));
-                    gr = fma(ca[k].x, cb[k].x, fma(ca[k].y, cb[k].y, gr));
-                    gi = fma(ca[k].x, cb[k].y, fma(-ca[k].y, cb[k].x, gi));
-                }
-                al = team_sum<TL>(al);
-                be = team_sum<TL>(be);
-                gr = team_sum<TL>(gr);
-                gi = team_sum<TL>(gi);
-                double c, s, er, ei;
-                if (jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, s, er, ei)) {
-                    // a' = c a - s e^{-i phi} b ;  b' = s e^{i phi} a + c b
-                    const cd2 sem = mk2(s * er, -s * ei);
-                    const cd2 sep = mk2(s * er, s * ei);
-#pragma unroll
-                    for (int k = 0; k < RPL; ++k) {
-                        const cd2 x = ca[k], y = cb[k];
-                        const cd2 sy = cmul(sem, y), sx = cmul(sep, x);
-                        ca[k] = mk2(c * x.x - sy.x, c * x.y - sy.y);
-                        cb[k] = mk2(c * y.x + sx.x, c * y.y + sx.y);
-                    }
-                    if (l16 == 0) sh.rotated = 1;
-                }
-            }
-            if (P > 1) {
-                // Round-robin move: bottoms shift towards team 0, tops away from it,
-                // team 0 keeps its top, bot[0] becomes top[1], top[P-1] becomes bot[P-1].
-                if (act) {
-#pragma unroll
-                    for (int k = 0; k < RPL; ++k) {
-                        const int r = l16 + TL * k;
-                        slot0[team * LD + r] = cb[k];
-                        if (team == 0) spare[r] = cb[k];
-                    }
-                }
-                __syncthreads();
-                if (act) {
-                    if (team < P - 1) {
-#pragma unroll
-                        for (int k = 0; k < RPL; ++k)
-                            cb[k] = slot0[(team + 1) * LD + l16 + TL * k];
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < RPL; ++k) cb[k] = ca[k];
-                    }
-                }
-                __syncthreads();
-                if (act) {
-#pragma unroll
-                    for (int k = 0; k < RPL; ++k)
-                        slot0[team * LD + l16 + TL * k] = ca[k];
-                }
-                __syncthreads();
-                if (act) {
-                    if (team >= 2) {
-#pragma unroll
-                        for (int k = 0; k < RPL; ++k)
-                            ca[k] = slot0[(team - 1) * LD + l16 + TL * k];
-                    } else if (team == 1) {
-#pragma unroll
-                        for (int k = 0; k < RPL; ++k) ca[k] = spare[l16 + TL * k];
-                    }
-                }
-                __syncthreads();
+            for (int k = 0; k < RPL; ++k) {
+                const cd2 x = ca[k];
+                ca[k] = cb[k];
+                cb[k] = x;
             }
         }
         __syncthreads();
