@@ -642,14 +642,7 @@ constexpr int kBigLanes = 32;
 constexpr int kBigTeams = 32;
 constexpr int kBigThreads = kBigTeams * kBigLanes;    // 1024
 
-__device__ __forceinline__ double big_sum(double v) {
-    v += __shfl_xor(v, 16, kBigLanes);
-    v += __shfl_xor(v, 8, kBigLanes);
-    v += __shfl_xor(v, 4, kBigLanes);
-    v += __shfl_xor(v, 2, kBigLanes);
-    v += __shfl_xor(v, 1, kBigLanes);
-    return v;
-}
+__device__ __forceinline__ double big_sum(double v) { return team_sum<kBigLanes>(v); }
 
 // Column blocks of the chi-128 class: 16 columns of up to 256 rows; a wave holds a
 // column as 4 rows per lane.
@@ -657,11 +650,7 @@ constexpr int kBlkCols = 16;
 constexpr int kBlkLd = 2 * kMaxChi;
 constexpr int kBlkRpl = kBlkLd / 64;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o /= 2) v += __shfl_xor(v, o, 64);
-    return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return team_sum<64>(v); }
 
 __device__ __forceinline__ int big_player(int slot, int r, int P) {
     return slot == 0 ? 0 : 1 + (slot - 1 + r) % (2 * P - 1);
