@@ -236,6 +236,17 @@ def test_random_circuits_match_oracle(fpga, nq, ng, seed):
     assert np.max(np.abs(res['statevector'] - ref)) < 1e-12
 
 
+@pytest.mark.parametrize('nq', [14, 18])
+def test_benchmark_structure_matches_oracle(fpga, nq):
+    # the 28-qubit benchmark's structure at sizes the oracle finishes: fused rotation
+    # runs, sign groups with thread and thread-pair terms, block-class phase factors,
+    # complex phase groups and register SWAPs of the QFT
+    g = workloads.u3_cz_layers(nq, 20, seed=5) + workloads.qft_gates(nq)
+    res = run(fpga, nq, g)
+    ref = sv_ref.run_circuit(nq, g)
+    assert np.max(np.abs(res['statevector'] - ref)) < 1e-12
+
+
 def test_deep_single_pass_circuit_matches_oracle(fpga):
     # 9000 rotations in one pass: the planner divides the scaled rotations' growth out
     rng = np.random.default_rng(41)

@@ -71,6 +71,22 @@ def test_plan_replay_matches_oracle(nq, ng, seed):
     assert all(p['K'] == min(nq, 12) for p in plan['passes'])
 
 
+def test_plan_replay_benchmark_structure():
+    # 20 x [U3 on every qubit + CZ ladder] + QFT at 14 qubits: fused rotation runs, sign
+    # and complex phase groups, block-class factors and register SWAPs, replayed as the
+    # kernel reads them
+    _lib_or_skip()
+    from qsim_amd import workloads
+    nq = 14
+    gates = workloads.u3_cz_layers(nq, 20, seed=5) + workloads.qft_gates(nq)
+    got, plan = run_plan_cpu(nq, gates)
+    ref = sv_ref.run_circuit(nq, gates)
+    assert np.max(np.abs(got - ref)) < 1e-12
+    kinds = {g['kind'] for p in plan['passes'] for s in p['stages'] for g in s['gates']}
+    assert 9 in kinds and 7 in kinds and 5 in kinds          # SWAP, sign and complex groups
+    assert any(k >= 32 for k in kinds)                        # fused rotation runs
+
+
 def test_plan_u3_cz_layers_pass_count():
     _lib_or_skip()
     nq = 20
