@@ -282,9 +282,12 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // with t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), zeta = (be - al) / (2|g|),
 // c = 1/sqrt(1 + t^2), s = c t.  Returns false (no rotation) when either column is a
 // numerical zero or |g| <= tol ||a|| ||b||.  c^2 + s^2 and |e^{i phi}| equal 1 to rounding.
+// tg = t |g| is what the rotation moves between the squared norms:
+// ||a'||^2 = al - tg, ||b'||^2 = be + tg.
 __device__ __forceinline__ bool jacobi_rotation(double al, double be, double gr, double gi,
                                                 double zero_sq, double tol, double& c,
-                                                double& s, double& er, double& ei) {
+                                                double& s, double& er, double& ei,
+                                                double& tg) {
     const double g2 = fma(gr, gr, gi * gi);
     if (!(al > zero_sq && be > zero_sq && g2 > tol * tol * al * be)) return false;
     const double rg = rsq_nr(g2);                    // 1 / |g|
@@ -302,6 +305,7 @@ __device__ __forceinline__ bool jacobi_rotation(double al, double be, double gr,
     s = c * t;
     er = gr * rg;
     ei = gi * rg;
+    tg = t * (g2 * rg);
     return true;
 }
 
@@ -344,6 +348,7 @@ struct SvdShared {
     int order[2 * CHI];
     int rank[2 * CHI];
     double fro_part[CHI];
+    double nslot[CHI];     // squared norm of the column in exchange slot t
     double fro;
     int rotated;
     int keep;
@@ -402,22 +407,24 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
     const double tol = 2.220446049250313e-16 * sqrt((double)m);
 
     // Jacobi rotation of the pair (x, y), x at the lower position: a' = c a - s e^{-i phi} b,
-    // b' = s e^{i phi} a + c b with a = x, b = y.
-    auto rotate = [&](cd2 (&x)[RPL], cd2 (&y)[RPL]) __attribute__((always_inline)) {
-        double al = 0, be = 0, gr = 0, gi = 0;
+    // b' = s e^{i phi} a + c b with a = x, b = y.  nx, ny are the columns' squared norms
+    // as tracked through the sweep (exact at its start, then moved by t |g| per rotation,
+    // as LAPACK's zgesvj does): only a^H b is summed here.  A sweep without rotations
+    // therefore tests every pair against exact norms.
+    auto rotate = [&](cd2 (&x)[RPL], cd2 (&y)[RPL], double& nx, double& ny)
+                      __attribute__((always_inline)) {
+        double gr = 0, gi = 0;
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
-            al = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, al));
-            be = fma(y[k].x, y[k].x, fma(y[k].y, y[k].y, be));
             gr = fma(x[k].x, y[k].x, fma(x[k].y, y[k].y, gr));
             gi = fma(x[k].x, y[k].y, fma(-x[k].y, y[k].x, gi));
         }
-        al = team_sum<TL>(al);
-        be = team_sum<TL>(be);
         gr = team_sum<TL>(gr);
         gi = team_sum<TL>(gi);
-        double c, sn, er, ei;
-        if (jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, sn, er, ei)) {
+        double c, sn, er, ei, tg;
+        if (jacobi_rotation(nx, ny, gr, gi, zero_sq, tol, c, sn, er, ei, tg)) {
+            nx = fmax(nx - tg, 0.0);
+            ny += tg;
             const cd2 sem = mk2(sn * er, -sn * ei);
             const cd2 sep = mk2(sn * er, sn * ei);
 #pragma unroll
@@ -441,41 +448,57 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
     // 2t+1.  Every team (idle ones too; slot t < CHI) parks lo in its slot and takes it
     // back at the end, team 0 unchanged, so lo holds no registers while team t works on
     // hi and the column it received.
-    auto odd_step = [&](cd2 (&lo)[RPL], cd2 (&hi)[RPL]) __attribute__((always_inline)) {
+    auto odd_step = [&](cd2 (&lo)[RPL], cd2 (&hi)[RPL], double& nlo, double& nhi)
+                        __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < RPL; ++k) slot0[team * LD + l16 + TL * k] = lo[k];
+        if (l16 == 0) sh.nslot[team] = nlo;
         __syncthreads();
         if (act && team < P - 1) {
             cd2 x[RPL];
 #pragma unroll
             for (int k = 0; k < RPL; ++k) x[k] = slot0[(team + 1) * LD + l16 + TL * k];
-            rotate(hi, x);
+            double nx = sh.nslot[team + 1];
+            rotate(hi, x, nhi, nx);
 #pragma unroll
             for (int k = 0; k < RPL; ++k) {
                 slot0[(team + 1) * LD + l16 + TL * k] = hi[k];
                 hi[k] = x[k];
             }
+            if (l16 == 0) sh.nslot[team + 1] = nhi;
+            nhi = nx;
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < RPL; ++k) lo[k] = slot0[team * LD + l16 + TL * k];
+        nlo = sh.nslot[team];
     };
 
     int sweeps = 0;
     for (; sweeps < kMaxSweeps; ++sweeps) {
         if (tid == 0) sh.rotated = 0;
+        // exact squared norms at the start of every sweep
+        double na = 0.0, nb = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+            na = fma(ca[k].x, ca[k].x, fma(ca[k].y, ca[k].y, na));
+            nb = fma(cb[k].x, cb[k].x, fma(cb[k].y, cb[k].y, nb));
+        }
+        na = team_sum<TL>(na);
+        nb = team_sum<TL>(nb);
         __syncthreads();
         // n = 2P steps: (even, odd) twice per trip with ca, cb as (lo, hi), then (hi, lo)
         for (int s = 0; s + 4 <= 2 * P; s += 4) {
-            if (act) rotate(ca, cb);
-            odd_step(cb, ca);
-            if (act) rotate(cb, ca);
-            odd_step(ca, cb);
+            if (act) rotate(ca, cb, na, nb);
+            odd_step(cb, ca, nb, na);
+            if (act) rotate(cb, ca, nb, na);
+            odd_step(ca, cb, na, nb);
         }
         if (P & 1) {
-            // two steps left; they end with positions 2t, 2t+1 in cb, ca
-            if (act) rotate(ca, cb);
-            odd_step(cb, ca);
+            // two steps left; they end with positions 2t, 2t+1 in cb, ca (the norms are
+            // recomputed by the next sweep)
+            if (act) rotate(ca, cb, na, nb);
+            odd_step(cb, ca, nb, na);
 #pragma unroll
             for (int k = 0; k < RPL; ++k) {
                 const cd2 x = ca[k];
@@ -749,8 +772,8 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
         be = wave_sum(be);
         gr = wave_sum(gr);
         gi = wave_sum(gi);
-        double c, sn, er, ei;
-        if (!jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, sn, er, ei)) return false;
+        double c, sn, er, ei, tg;
+        if (!jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, sn, er, ei, tg)) return false;
         const cd2 sem = mk2(sn * er, -sn * ei);
         const cd2 sep = mk2(sn * er, sn * ei);
 #pragma unroll
