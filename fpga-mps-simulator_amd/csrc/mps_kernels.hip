@@ -680,6 +680,7 @@ __device__ __forceinline__ int big_player(int slot, int r, int P) {
 }
 
 struct BigShared {
+    double bn[2 * kBlkCols];   // squared norms of the two LDS blocks' columns
     double sval[2 * kMaxChi];
     int order[2 * kMaxChi];
     int rank[2 * kMaxChi];
@@ -758,22 +759,23 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
         }
     };
     // Rotation of the pair (x, y) held as kBlkRpl rows per lane of one wave; true if
-    // the pair was rotated.
-    auto rotate_pair = [&](cd2 (&x)[kBlkRpl], cd2 (&y)[kBlkRpl]) -> bool {
-        double al = 0, be = 0, gr = 0, gi = 0;
+    // the pair was rotated.  nx, ny are the squared norms, exact when the block pair
+    // was loaded and moved by t |g| per rotation since, so only a^H b is summed (a sweep
+    // without rotations tests every pair against exact norms).
+    auto rotate_pair = [&](cd2 (&x)[kBlkRpl], cd2 (&y)[kBlkRpl], double& nx,
+                           double& ny) -> bool {
+        double gr = 0, gi = 0;
 #pragma unroll
         for (int k = 0; k < kBlkRpl; ++k) {
-            al = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, al));
-            be = fma(y[k].x, y[k].x, fma(y[k].y, y[k].y, be));
             gr = fma(x[k].x, y[k].x, fma(x[k].y, y[k].y, gr));
             gi = fma(x[k].x, y[k].y, fma(-x[k].y, y[k].x, gi));
         }
-        al = wave_sum(al);
-        be = wave_sum(be);
         gr = wave_sum(gr);
         gi = wave_sum(gi);
         double c, sn, er, ei, tg;
-        if (!jacobi_rotation(al, be, gr, gi, zero_sq, tol, c, sn, er, ei, tg)) return false;
+        if (!jacobi_rotation(nx, ny, gr, gi, zero_sq, tol, c, sn, er, ei, tg)) return false;
+        nx = fmax(nx - tg, 0.0);
+        ny += tg;
         const cd2 sem = mk2(sn * er, -sn * ei);
         const cd2 sep = mk2(sn * er, sn * ei);
 #pragma unroll
@@ -798,6 +800,20 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
                 load_block(bi, 0);
                 load_block(bj, 1);
                 __syncthreads();
+                // exact squared norms of the 32 columns: wave w takes w and w + 16
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int cidx = wave + kBlkCols * h;
+                    double v = 0.0;
+#pragma unroll
+                    for (int k = 0; k < kBlkRpl; ++k) {
+                        const cd2 z = blk[cidx * kBlkLd + lane + 64 * k];
+                        v = fma(z.x, z.x, fma(z.y, z.y, v));
+                    }
+                    v = wave_sum(v);
+                    if (lane == 0) sh.bn[cidx] = v;
+                }
+                __syncthreads();
                 if (ro == 0) {
                     // pairs inside each block: waves 0..7 the first, 8..15 the second
                     const int half = wave / 8, pr = wave % 8;
@@ -813,12 +829,18 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
                                 x[k] = base[ca * kBlkLd + lane + 64 * k];
                                 y[k] = base[cb * kBlkLd + lane + 64 * k];
                             }
-                            if (rotate_pair(x, y)) {
+                            double nx = sh.bn[half * kBlkCols + ca];
+                            double ny = sh.bn[half * kBlkCols + cb];
+                            if (rotate_pair(x, y, nx, ny)) {
                                 any_local = true;
 #pragma unroll
                                 for (int k = 0; k < kBlkRpl; ++k) {
                                     base[ca * kBlkLd + lane + 64 * k] = x[k];
                                     base[cb * kBlkLd + lane + 64 * k] = y[k];
+                                }
+                                if (lane == 0) {
+                                    sh.bn[half * kBlkCols + ca] = nx;
+                                    sh.bn[half * kBlkCols + cb] = ny;
                                 }
                             }
                         }
@@ -829,17 +851,20 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
                     cd2 x[kBlkRpl];
 #pragma unroll
                     for (int k = 0; k < kBlkRpl; ++k) x[k] = blk[wave * kBlkLd + lane + 64 * k];
+                    double nx = sh.bn[wave];
                     cd2* second = blk + kBlkCols * kBlkLd;
                     for (int kk = 0; kk < kBlkCols; ++kk) {
                         const int jc = (wave + kk) % kBlkCols;
                         cd2 y[kBlkRpl];
 #pragma unroll
                         for (int k = 0; k < kBlkRpl; ++k) y[k] = second[jc * kBlkLd + lane + 64 * k];
-                        if (rotate_pair(x, y)) {
+                        double ny = sh.bn[kBlkCols + jc];
+                        if (rotate_pair(x, y, nx, ny)) {
                             any_local = true;
 #pragma unroll
                             for (int k = 0; k < kBlkRpl; ++k)
                                 second[jc * kBlkLd + lane + 64 * k] = y[k];
+                            if (lane == 0) sh.bn[kBlkCols + jc] = ny;
                         }
                         __syncthreads();
                     }
