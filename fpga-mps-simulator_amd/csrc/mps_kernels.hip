@@ -9,12 +9,14 @@
 // One update = one workgroup per kernel:
 //   contraction: theta[(a,s0),(s1,d)] = sum_pq G[2 s0 + s1, 2 p + q] sum_b A[a,p,b] B[b,q,d]
 //                written as a (2chi_l x 2chi_r) complex128 matrix into the workspace;
-//   svd:         1024 threads = 64 teams of 16 lanes; team t holds two columns of theta
-//                (8 rows per lane) in registers and orthogonalises them with a complex
-//                Jacobi rotation; columns travel between teams in a round-robin
-//                tournament through LDS; after convergence column norms are the
+//   svd:         up to chi 64: 64 teams of TL lanes (the autotuned team width); team t
+//                holds two columns of theta in registers and orthogonalises pairs of
+//                columns with complex Jacobi rotations in the odd-even ordering (even
+//                steps inside a team, odd steps hand one column each way to the
+//                neighbouring team through LDS); after convergence column norms are the
 //                singular values, the kept columns (normalised) are U, and
-//                Vh = S^-1 U^H theta is formed from theta in the workspace.
+//                Vh = S^-1 U^H theta is formed from theta in the workspace.  chi 128:
+//                a blocked variant over a working copy of theta (mps_svd_big_kernel).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
