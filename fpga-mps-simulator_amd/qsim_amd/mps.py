@@ -166,7 +166,20 @@ class MPSChain:
         row[:t.size].copy_(torch.from_numpy(t.reshape(-1)))
 
     def to_host(self) -> 'HostMPS':
-        return HostMPS(self.site_tensors(), self.norm_log)
+        return HostMPS(self.site_tensors(), self.norm_log, self.chi_max, self.cutoff)
+
+    @classmethod
+    def from_host(cls, host: 'HostMPS', device: int = 0, precision: str = 'fp64') -> 'MPSChain':
+        """A device chain holding a host chain's site tensors (norm_log is not carried)."""
+        import torch
+        chi = max([1] + [t.shape[2] for t in host.sites[:-1]])
+        ch = cls(host.n, max(chi, min(host.chi_max, max_chi())), host.svd_cutoff, device,
+                 precision)
+        b = np.array([1] + [t.shape[2] for t in host.sites[:-1]] + [1], dtype=np.int32)
+        ch.bond.copy_(torch.from_numpy(b))
+        for i, t in enumerate(host.sites):
+            ch.set_site(i, t)
+        return ch
 
     # -- shot sampling ----------------------------------------------------------
     def sample(self, shots: int, measured: Optional[List[int]] = None,
@@ -218,18 +231,69 @@ class MPSChain:
 
 
 class HostMPS:
-    """Host copy of a chain with the read-outs of mps_core.MPSState."""
+    """Host copy of a chain with the interface of mps_core.MPSState (mps_core.py:172-494):
+    the object FPGAMPSResult.get_mps returns.  Site i is a (chi_l, 2, chi_r) complex128
+    array; the state is exp(norm_log) times the chain's contraction.  The canonical-form
+    helpers and read-outs follow the reference's definitions; shots are drawn on the
+    device (MPSChain.sample)."""
 
-    def __init__(self, sites: List[np.ndarray], norm_log: float):
+    def __init__(self, sites: List[np.ndarray], norm_log: float, chi_max: Optional[int] = None,
+                 svd_cutoff: float = 1e-12):
         self.sites = sites
         self.n = len(sites)
         self.norm_log = norm_log
+        self.chi_max = chi_max if chi_max is not None else max([1] + [t.shape[2] for t in sites])
+        self.svd_cutoff = svd_cutoff
+
+    def copy(self) -> 'HostMPS':
+        return HostMPS([t.copy() for t in self.sites], self.norm_log, self.chi_max,
+                       self.svd_cutoff)
+
+    def chi_l(self, i: int) -> int:
+        return self.sites[i].shape[0]
+
+    def chi_r(self, i: int) -> int:
+        return self.sites[i].shape[2]
 
     def bond_dims(self) -> List[int]:
         return [t.shape[2] for t in self.sites[:-1]]
 
     def max_bond(self) -> int:
         return 1 if self.n <= 1 else max(self.bond_dims())
+
+    # -- canonical forms (mps_core.py:221-263) ----------------------------------
+    def left_normalize_site(self, i: int) -> None:
+        """Site i as a (2 chi_l) x chi_r matrix = Q R (reduced QR): site i becomes Q and R
+        is absorbed into site i + 1 (mps_core.py:221-234)."""
+        a = self.sites[i]
+        cl, d, cr = a.shape
+        q, r = np.linalg.qr(a.reshape(cl * d, cr))
+        self.sites[i] = q.reshape(cl, d, q.shape[1])
+        if i + 1 < self.n:
+            self.sites[i + 1] = np.tensordot(r, self.sites[i + 1], axes=([1], [0]))
+
+    def right_normalize_site(self, i: int) -> None:
+        """Site i as a chi_l x (2 chi_r) matrix = L Q (QR of its adjoint): site i becomes Q
+        and L is absorbed into site i - 1 (mps_core.py:236-251)."""
+        a = self.sites[i]
+        cl, d, cr = a.shape
+        q, r = np.linalg.qr(a.reshape(cl, d * cr).conj().T)
+        self.sites[i] = q.conj().T.reshape(q.shape[1], d, cr)
+        if i >= 1:
+            self.sites[i - 1] = np.tensordot(self.sites[i - 1], r.conj().T, axes=([2], [0]))
+
+    def left_canonicalize(self) -> None:
+        for i in range(self.n - 1):
+            self.left_normalize_site(i)
+
+    def right_canonicalize(self) -> None:
+        for i in range(self.n - 1, 0, -1):
+            self.right_normalize_site(i)
+
+    def apply_1q(self, site: int, gate: np.ndarray) -> None:
+        """A'[l, s', r] = sum_s G[s', s] A[l, s, r] (mps_core.py:267-275)."""
+        self.sites[site] = np.einsum('ts,lsr->ltr', np.asarray(gate, dtype=np.complex128),
+                                     self.sites[site])
 
     def get_statevector(self) -> np.ndarray:
         """Dense state with qubit q <-> index bit q (mps_core.py:433-461)."""
@@ -240,6 +304,40 @@ class HostMPS:
             acc = np.tensordot(acc, t, axes=([1], [0])).reshape(-1, t.shape[2])
         sv = acc[:, 0] * math.exp(self.norm_log)
         return sv.reshape([2] * self.n).transpose(list(range(self.n - 1, -1, -1))).reshape(-1)
+
+    # -- bond read-outs (mps_core.py:465-494) -----------------------------------
+    def entanglement_entropy(self, bond: int) -> float:
+        """-sum p log p over the squared singular values (above 1e-15, normalised) of
+        sites 0..bond contracted into a 2^(bond+1) x chi matrix (mps_core.py:465-483)."""
+        if bond < 0 or bond >= self.n - 1:
+            raise ValueError(f"bond {bond} out of range [0, {self.n - 2}]")
+        acc = self.sites[0][0]
+        for t in self.sites[1:bond + 1]:
+            acc = np.tensordot(acc, t, axes=([1], [0])).reshape(-1, t.shape[2])
+        sv = np.linalg.svd(acc, compute_uv=False)
+        sv = sv[sv > 1e-15]
+        p = sv * sv
+        p = p / p.sum()
+        return -float(np.sum(p * np.log(np.maximum(p, 1e-300))))
+
+    def bond_spectrum(self, bond: int) -> np.ndarray:
+        """Singular values of site `bond` as a (2 chi_l) x chi_r matrix after sites
+        0..bond of a copy are left-normalised, that site included, as the reference
+        does (mps_core.py:485-494)."""
+        m = self.copy()
+        for i in range(bond + 1):
+            m.left_normalize_site(i)
+        a = m.sites[bond]
+        return np.linalg.svd(a.reshape(-1, a.shape[2]), compute_uv=False)
+
+    # -- shots (mps_core.py:371-431), drawn on the device ------------------------
+    def sample(self, num_shots: int = 1, measured_qubits: Optional[List[int]] = None,
+               rng: Optional[np.random.Generator] = None, device: int = 0) -> List[str]:
+        return MPSChain.from_host(self, device).sample(num_shots, measured_qubits, rng)
+
+    def get_counts(self, num_shots: int, measured_qubits: Optional[List[int]] = None,
+                   rng: Optional[np.random.Generator] = None, device: int = 0) -> Dict[str, int]:
+        return MPSChain.from_host(self, device).get_counts(num_shots, measured_qubits, rng)
 
 
 def z_expectations(host: 'HostMPS') -> np.ndarray:

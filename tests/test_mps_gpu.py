@@ -295,6 +295,24 @@ def test_gpu_sampling_reproduces_oracle_samples(n, chi, depth, seed):
     assert got == exp and all(len(b) == 4 for b in got)
 
 
+def test_host_copy_samples_and_readouts_match_oracle():
+    """The drop-in's get_mps() object: shots drawn through a device copy, and the
+    bond read-outs, against the oracle on the same chain."""
+    n, chi = 10, 16
+    ups = workloads.brickwall_gates(n, 6, seed=4, kind='haar')
+    host = chain_run(n, chi, 1e-12, ups).to_host()
+    ref = mps_ref.MPS(n, chi, 1e-12)
+    ref.sites = [t.copy() for t in host.sites]
+    ref.norm_log = host.norm_log
+    assert host.sample(800, None, np.random.default_rng(3)) == \
+        ref.sample(800, None, np.random.default_rng(3))
+    assert host.get_counts(300, [1, 7], np.random.default_rng(4)) == \
+        ref.get_counts(300, [1, 7], np.random.default_rng(4))
+    for b in range(n - 1):
+        assert abs(host.entanglement_entropy(b) - ref.entanglement_entropy(b)) < 1e-10
+        assert np.allclose(host.bond_spectrum(b), ref.bond_spectrum(b), rtol=1e-10, atol=1e-13)
+
+
 def test_gpu_sampling_ghz():
     from qsim_amd.mps import MPSChain
     n = 12
