@@ -370,4 +370,22 @@ int qs_sv_probabilities(const void* state, int nq, double* probs, void* stream) 
     return sv_probs(static_cast<const double2*>(state), nq, probs, as_stream(stream));
 }
 
+size_t qs_sv_sample_workspace_bytes(int nq) {
+    if (nq < 1 || nq > 34) return 0;
+    return sv_sample_workspace_bytes(nq);
+}
+
+int qs_sv_sample(const void* state, int nq, const double* uniforms, int shots,
+                 int64_t* indices, void* workspace, size_t workspace_bytes, void* stream) {
+    clear_error();
+    int st = check_state(state, nq);
+    if (st) return st;
+    QS_REQUIRE(shots >= 0, QS_E_ARG, "negative shot count %d", shots);
+    if (shots == 0) return QS_OK;
+    QS_REQUIRE(uniforms != nullptr && indices != nullptr && workspace != nullptr, QS_E_ARG,
+               "null uniforms, indices or workspace");
+    return sv_sample(static_cast<const double2*>(state), nq, uniforms, shots, indices,
+                     static_cast<double*>(workspace), workspace_bytes, as_stream(stream));
+}
+
 }  // extern "C"

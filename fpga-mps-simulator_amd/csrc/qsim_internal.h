@@ -241,5 +241,10 @@ int sv_scale(double2* state, int nq, double re, double im, hipStream_t stream);
 int sv_project(double2* state, int nq, int q, hipStream_t stream);
 int sv_probs(const double2* state, int nq, double* p, hipStream_t stream);
 int sv_norm_sq(const double2* state, int nq, double* out, hipStream_t stream);
+// Measurement shots (sv_sample.hip): workspace size, and the sampler (synchronous for the
+// normalisation check; the indices are written on `stream`).
+size_t sv_sample_workspace_bytes(int nq);
+int sv_sample(const double2* state, int nq, const double* uniforms, int shots, int64_t* out,
+              double* ws, size_t ws_bytes, hipStream_t stream);
 
 }  // namespace qs

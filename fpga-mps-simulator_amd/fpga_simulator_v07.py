@@ -337,14 +337,16 @@ class QuantumFPGACircuit:
         if nc <= 0:
             nc = num_qubits
         meas_pairs = meas if meas else [(q, q) for q in range(min(num_qubits, nc))]
-        probs = sv.real ** 2 + sv.imag ** 2
-        ps = probs.sum()
-        if ps > 0:
-            probs /= ps
         counts: Dict[str, int] = {}
         memory: List[str] = []
         if shots > 0:
-            indices = np.random.choice(N, size=shots, p=probs)
+            # np.random.choice(N, shots, p=|sv|^2 / sum) draws `shots` uniforms from the
+            # global generator and returns the first index whose cumulative probability
+            # exceeds each; the same draws go to the device sampler.
+            if not nb > 0.0:
+                raise ValueError('probabilities do not sum to 1')   # as np.random.choice
+            u = np.random.random_sample(shots)
+            indices = sv_dev.sample_indices(u)
             for idx in indices:
                 qb = format(int(idx), f'0{num_qubits}b')
                 cb = ['0'] * nc

@@ -75,6 +75,10 @@ _PROTOS = {
                                          ctypes.c_void_p]),
     'qs_sv_probabilities': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                            ctypes.c_void_p]),
+    'qs_sv_sample_workspace_bytes': (ctypes.c_size_t, [ctypes.c_int]),
+    'qs_sv_sample': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_size_t, ctypes.c_void_p]),
     'qs_sv_pauli_inner': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                          _c_int32_p, _c_int32_p, ctypes.c_int, _c_double_p,
                                          ctypes.c_void_p]),

@@ -19,8 +19,8 @@ CSRC = os.path.join(ROOT_DIR, 'csrc')
 LIB_NAME = 'libqsim_hip.so'
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
-SOURCES = ['abi.hip', 'sv_kernels.hip', 'sv_plan.cpp', 'sv_grad.hip', 'mps_kernels.hip',
-           'mps_sample.hip']
+SOURCES = ['abi.hip', 'sv_kernels.hip', 'sv_plan.cpp', 'sv_grad.hip', 'sv_sample.hip',
+           'mps_kernels.hip', 'mps_sample.hip']
 
 HIPCC_FLAGS = [
     '--offload-arch=gfx950',

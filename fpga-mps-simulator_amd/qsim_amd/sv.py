@@ -119,6 +119,27 @@ class StateVector:
                   self._stream())
         return p
 
+    def sample_indices(self, uniforms) -> np.ndarray:
+        """Basis-state indices for the given uniforms in [0, 1): for each, the first index
+        whose cumulative probability |a|^2 / sum |a|^2 exceeds it, which is what
+        np.random.choice(2^n, len(uniforms), p=probs) returns for the uniforms it draws
+        (fpga_simulator_v07.py:592-609).  Computed on the device."""
+        import torch
+        u = np.ascontiguousarray(np.asarray(uniforms, dtype=np.float64).reshape(-1))
+        if u.size == 0:
+            return np.zeros(0, dtype=np.int64)
+        if not (np.all(u >= 0.0) and np.all(u < 1.0)):
+            raise ValueError('uniforms must lie in [0, 1)')
+        dev = self.tensor.device
+        ud = torch.from_numpy(u).to(dev)
+        out = torch.empty(u.size, dtype=torch.int64, device=dev)
+        nbytes = _lib.lib().qs_sv_sample_workspace_bytes(self.nq)
+        ws = torch.empty(max(1, (nbytes + 7) // 8), dtype=torch.float64, device=dev)
+        _lib.call('qs_sv_sample', self.ptr, self.nq, ctypes.c_void_p(ud.data_ptr()), u.size,
+                  ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                  ws.numel() * 8, self._stream())
+        return out.cpu().numpy()
+
 
 def time_passes(enable: bool) -> None:
     """Bracket every fused-pass kernel launched from this thread with HIP events."""

@@ -133,6 +133,21 @@ int qs_sv_reset_qubit(void* state, int nq, int q, void* stream);
 /* probs[i] = |a_i|^2 (float64, device buffer of 2^nq). */
 int qs_sv_probabilities(const void* state, int nq, double* probs, void* stream);
 
+/* Measurement shots, replacing the host-side
+ *   probs = |sv|^2 / sum |sv|^2;  indices = np.random.choice(2^nq, shots, p=probs)
+ * of run_circuit (fpga_simulator_v07.py:592-609).  numpy's choice draws one uniform per
+ * shot (random_sample) and returns, for each, the first index whose cumulative
+ * probability exceeds it; here the caller passes those uniforms (device buffer of
+ * `shots` float64 in [0, 1)) and receives the indices (device buffer of `shots`
+ * int64).  Indices differ from numpy's only when a uniform lies within rounding
+ * distance of a cumulative-probability boundary.  `workspace` is a device buffer of at
+ * least qs_sv_sample_workspace_bytes(nq) bytes.  Fails with QS_E_ARG when sum |a|^2 is
+ * not positive (numpy raises "probabilities do not sum to 1").  Synchronises `stream`
+ * once (normalisation check); the indices are written on `stream`. */
+size_t qs_sv_sample_workspace_bytes(int nq);
+int qs_sv_sample(const void* state, int nq, const double* uniforms, int shots,
+                 int64_t* indices, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------- statevector: gradients */
 /* No counterpart in the reference (it has no differentiation); these serve the adjoint
  * gradient of <psi|H|psi> for circuits of rotations exp(-i theta P / 2). */

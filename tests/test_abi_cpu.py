@@ -137,6 +137,13 @@ def test_argument_errors_are_reported_without_device():
     assert lib.qs_last_error() == b''
     # null state pointer
     assert lib.qs_sv_set_zero(None, 3, None) == _lib.QS_E_ARG
+    # measurement shots: workspace size, null state, negative shot count
+    assert lib.qs_sv_sample_workspace_bytes(12) == 2 * 8
+    assert lib.qs_sv_sample_workspace_bytes(28) == ((1 << 16) + 1) * 8
+    assert lib.qs_sv_sample_workspace_bytes(0) == 0
+    assert lib.qs_sv_sample(None, 3, None, 4, None, None, 0, None) == _lib.QS_E_ARG
+    assert lib.qs_sv_sample(ctypes.c_void_p(16), 3, None, -1, None, None, 0, None) == _lib.QS_E_ARG
+    assert b'negative shot count' in lib.qs_last_error()
     # MPS: chi beyond the limit is an error, never a clamp
     t = (_lib.QsMpsTask * 1)()
     t[0].sites = 1

@@ -202,6 +202,56 @@ def test_measurement_clbit_mapping(fpga):
     assert set(r['counts']) == {'000'}
 
 
+def _host_choice(sv, u):
+    # np.random.choice(N, len(u), p=probs) for the uniforms u it would draw
+    # (numpy's legacy choice: cdf = cumsum(p), cdf /= cdf[-1], searchsorted right)
+    p = sv.real ** 2 + sv.imag ** 2
+    p = p / p.sum()
+    cdf = np.cumsum(p)
+    cdf /= cdf[-1]
+    return np.searchsorted(cdf, u, side='right')
+
+
+@pytest.mark.parametrize('nq,kind', [(1, 'h'), (10, 'random'), (14, 'ghz'), (18, 'random'),
+                                     (20, 'sparse')])
+def test_device_sampler_matches_numpy_choice(nq, kind):
+    from qsim_amd.sv import StateVector, gates_to_records
+    rng = np.random.default_rng(nq)
+    if kind == 'h':
+        gates = [('h', [0], None)]
+    elif kind == 'ghz':
+        gates = [('h', [0], None)] + [('cx', [q, q + 1], None) for q in range(nq - 1)]
+    elif kind == 'sparse':
+        # probability only on the basis states whose top qubits are 0: many whole blocks
+        # and lanes of zero probability
+        gates = [('h', [q], None) for q in range(6)] + [('ry', [q], [0.3 * q]) for q in range(6)]
+    else:
+        gates = random_circuit(rng, nq, 8 * nq)
+    sv = StateVector(nq, 0)
+    sv.apply_records(gates_to_records(gates, nq)[0])
+    host = sv.to_numpy()
+    u = rng.random(5000)
+    u[0] = 0.0                       # the first basis state of nonzero probability
+    got = sv.sample_indices(u)
+    assert np.array_equal(got, _host_choice(host, u))
+    assert np.all(np.abs(host[got]) > 0)
+    if kind == 'ghz':
+        assert set(got.tolist()) <= {0, (1 << nq) - 1}
+
+
+def test_drop_in_shots_follow_the_global_generator(fpga):
+    # the reference's run_circuit samples with np.random.choice on the global generator;
+    # the drop-in draws the same uniforms and samples on the device
+    g = [('h', [0], None), ('ry', [1], [0.7]), ('cx', [1, 2], None), ('rx', [3], [1.1])]
+    np.random.seed(123)
+    r = fpga.run_circuit(4, g, shots=3000)
+    np.random.seed(123)
+    p = np.abs(r['statevector']) ** 2
+    idx = np.random.choice(16, size=3000, p=p / p.sum())
+    exp = [format(int(i), '04b') for i in idx]
+    assert r['memory'] == exp
+
+
 def test_reset_matches_oracle(fpga):
     g = [('h', [0], None), ('cx', [0, 1], None), ('reset', [0], None), ('h', [2], None),
          ('reset', [2], None), ('x', [2], None), ('reset', [2], None), ('ry', [1], [0.3])]
