@@ -15,6 +15,9 @@
 //    Generator.choice(2, p=...) decides it; then v = Av_outcome / ||Av_outcome|| (when the
 //    norm exceeds 1e-15).  The uniforms come from the caller, one per (shot, site) in
 //    shot-major order, which is the order the reference draws them in.
+#include <cmath>
+#include <vector>
+
 #include "qsim_internal.h"
 
 namespace qs {
@@ -200,6 +203,30 @@ mps_sample_kernel(const double* __restrict__ sites, const int32_t* __restrict__ 
     }
 }
 
+// One step of the dense contraction (mps_core.py:433-461): with the state of sites 0..i-1
+// as cur[x][l] (x < 2^i, site q on bit q of x, l < chi_l), nxt[x + s 2^i][r] =
+// f sum_l cur[x][l] A_i[l][s][r].  One thread per output element; threads of one x read
+// the same row of cur.  Step 0 copies site 0 (cur = nullptr).
+__global__ void mps_dense_step_kernel(const cd2* __restrict__ sites, int64_t stride, int i,
+                                      int cl, int cr, const cd2* __restrict__ cur,
+                                      cd2* __restrict__ nxt, double f) {
+    const uint64_t total = (2ull << i) * (uint64_t)cr;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= total) return;
+    const uint64_t xo = tid / cr;
+    const int r = (int)(tid - xo * cr);
+    const int sidx = (int)(xo >> i);
+    const uint64_t x = xo & ((1ull << i) - 1);
+    const cd2* A = sites + (int64_t)i * stride;
+    cd2 acc = mk2(0.0, 0.0);
+    if (cur == nullptr) {
+        acc = A[sidx * cr + r];                      // site 0: chi_l = 1
+    } else {
+        for (int l = 0; l < cl; ++l) acc = cmadd(acc, cur[x * cl + l], A[(l * 2 + sidx) * cr + r]);
+    }
+    nxt[tid] = mk2(acc.x * f, acc.y * f);
+}
+
 int check_chain(const double* sites, const int32_t* bond, int64_t stride, int n, int chi_max) {
     QS_REQUIRE(sites != nullptr && bond != nullptr, QS_E_ARG, "null sites or bond array");
     QS_REQUIRE(n >= 1, QS_E_ARG, "chain of %d sites", n);
@@ -244,6 +271,47 @@ int qs_mps_right_envs(const double* sites, const int32_t* bond, int64_t site_str
     }
     hipLaunchKernelGGL(mps_env_normalise_kernel, dim3(n), dim3(256), 0, hs, bond, n, chi_max, E);
     QS_HIP(hipGetLastError());
+    return QS_OK;
+}
+
+int qs_mps_statevector(const double* sites, const int32_t* bond, int64_t site_stride, int n,
+                       int chi_max, double norm_log, double* out, double* work,
+                       size_t buffer_bytes, void* stream) {
+    clear_error();
+    int st = check_chain(sites, bond, site_stride, n, chi_max);
+    if (st) return st;
+    QS_REQUIRE(n <= 30, QS_E_RANGE, "dense state of %d qubits", n);
+    QS_REQUIRE(out != nullptr && work != nullptr, QS_E_ARG, "null output or work buffer");
+    const size_t need = ((size_t)1 << n) * sizeof(cd2);
+    QS_REQUIRE(buffer_bytes >= need, QS_E_RANGE, "buffers of %zu bytes, %zu needed",
+               buffer_bytes, need);
+    hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
+    std::vector<int32_t> b(n + 1);
+    QS_HIP(hipMemcpyAsync(b.data(), bond, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, hs));
+    QS_HIP(hipStreamSynchronize(hs));
+    QS_REQUIRE(b[0] == 1 && b[n] == 1, QS_E_ARG, "boundary bonds %d, %d (must be 1)", b[0], b[n]);
+    for (int i = 0; i < n; ++i) {
+        QS_REQUIRE(b[i + 1] >= 1 && b[i + 1] <= chi_max, QS_E_CHI, "bond %d = %d", i + 1,
+                   b[i + 1]);
+        // every intermediate fits in 2^n amplitudes when bond i+1 <= 2^(n-i-1), which
+        // SVD truncation guarantees
+        QS_REQUIRE(((uint64_t)2 << i) * (uint64_t)b[i + 1] <= ((uint64_t)1 << n), QS_E_RANGE,
+                   "bond %d = %d exceeds the dimension of the sites to its right", i + 1,
+                   b[i + 1]);
+    }
+    const cd2* S = reinterpret_cast<const cd2*>(sites);
+    cd2* bufs[2] = {reinterpret_cast<cd2*>(out), reinterpret_cast<cd2*>(work)};
+    const double f = std::exp(norm_log);
+    const cd2* cur = nullptr;
+    for (int i = 0; i < n; ++i) {
+        cd2* nxt = bufs[(n - 1 - i) & 1];            // the last step writes `out`
+        const uint64_t total = ((uint64_t)2 << i) * (uint64_t)b[i + 1];
+        const unsigned blocks = (unsigned)((total + 255) / 256);
+        hipLaunchKernelGGL(mps_dense_step_kernel, dim3(blocks), dim3(256), 0, hs, S, site_stride,
+                           i, b[i], b[i + 1], cur, nxt, i == n - 1 ? f : 1.0);
+        QS_HIP(hipGetLastError());
+        cur = nxt;
+    }
     return QS_OK;
 }
 

@@ -236,6 +236,8 @@ def _run_circuit(circuit, shots: int, chi_max: int, svd_cutoff: float, seed: Opt
     chain = run_gate_list(gate_list, nq, chi_max, svd_cutoff, precision, device, pt)
     t0 = time.perf_counter()
     host = chain.to_host()
+    # the dense state the reference builds on the host for n <= 24, contracted on the GPU
+    sv = chain.statevector() if nq <= 24 else None
     pt['hbm_transfer'] += time.perf_counter() - t0
     t0 = time.perf_counter()
     rng = np.random.default_rng(seed)
@@ -246,7 +248,7 @@ def _run_circuit(circuit, shots: int, chi_max: int, svd_cutoff: float, seed: Opt
     pt['host_other'] = max(0.0, total - sum(pt[k] for k in ('fpga_kernel', 'hbm_transfer',
                                                             'cpu_svd', 'cpu_qr',
                                                             'cpu_sampling')))
-    return host, counts, pt
+    return host, counts, pt, sv
 
 
 class FPGAMPSResult:
@@ -326,20 +328,16 @@ class FPGAMPSSimulator:
         for ti, circuit in enumerate(circuits):
             s = (seed + ti) if seed is not None else None
             try:
-                host, counts, pt = _run_circuit(circuit, shots, self.chi_max, self.svd_cutoff, s,
+                host, counts, pt, sv = _run_circuit(circuit, shots, self.chi_max, self.svd_cutoff, s,
                                                 self.precision, self.device)
             except (ValueError, AssertionError) as e:
                 raise RuntimeError(f'Card 0 error: {e}') from e
             except RuntimeError as e:
                 raise RuntimeError(f'Card 0 error: {e}') from e
-            sv = None
-            if circuit.num_qubits <= 24:
-                try:
-                    sv = host.get_statevector()
-                except Exception:
-                    sv = None
+            # 'mps': the reference declares get_mps() -> Optional[MPSState] but its worker
+            # never fills the key (None); here it is the final chain on the host
             results.append({'counts': counts, 'statevector': sv, 'max_bond': host.max_bond(),
-                            'bond_dims': host.bond_dims(), 'phase_times': pt})
+                            'bond_dims': host.bond_dims(), 'phase_times': pt, 'mps': host})
         return FPGAMPSResult(results, shots, self.num_cards, time.time() - t0, 'shots_fpga')
 
     def get_card_stats(self) -> List[dict]:

@@ -165,6 +165,21 @@ class MPSChain:
         row = self.sites.view(self.n, self.stride)[i]
         row[:t.size].copy_(torch.from_numpy(t.reshape(-1)))
 
+    def statevector(self) -> np.ndarray:
+        """Dense state with qubit q <-> index bit q, times exp(norm_log), contracted on
+        the device (mps_core.py:433-461; the reference refuses n > 24)."""
+        import torch
+        if self.n > 24:
+            raise ValueError(f"Statevector reconstruction for n={self.n} > 24 not supported.")
+        dev = self.sites.device
+        out = torch.empty(1 << self.n, dtype=torch.complex128, device=dev)
+        work = torch.empty(1 << self.n, dtype=torch.complex128, device=dev)
+        _lib.call('qs_mps_statevector', ctypes.c_void_p(self.sites.data_ptr()),
+                  ctypes.c_void_p(self.bond.data_ptr()), self.stride, self.n, self.chi_max,
+                  float(self.norm_log), ctypes.c_void_p(out.data_ptr()),
+                  ctypes.c_void_p(work.data_ptr()), out.numel() * 16, self._stream())
+        return out.cpu().numpy()
+
     def to_host(self) -> 'HostMPS':
         return HostMPS(self.site_tensors(), self.norm_log, self.chi_max, self.cutoff)
 

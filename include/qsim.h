@@ -237,6 +237,15 @@ int qs_mps_batch_launch(const void* table_dev, int ntasks, const int32_t counts[
 int qs_mps_time_kernels(int enable);
 int qs_mps_kernel_times(int32_t* kind, int32_t* tasks, double* ms, int cap, int* count);
 
+/* Dense statevector of a device chain (mps_core.py:433-461): out[idx] with bit q of idx
+ * the value of qubit q (site 0 the least significant), times exp(norm_log).  `out` and
+ * `work` are device buffers of at least 2^n complex128 (buffer_bytes each); every bond
+ * must satisfy bond[i+1] <= 2^(n-i-1), as SVD truncation leaves it.  One launch per site;
+ * synchronises `stream` once to read the bond dimensions. */
+int qs_mps_statevector(const double* sites, const int32_t* bond, int64_t site_stride, int n,
+                       int chi_max, double norm_log, double* out, double* work,
+                       size_t buffer_bytes, void* stream);
+
 /* Shot sampling of a chain on the device (mps_core.py:354-420, called by the driver at
  * fpga_mps_simulator.py:639-643, where it runs on the host).
  * qs_mps_right_envs writes the right environments R_0..R_n of the chain (sites, bond,

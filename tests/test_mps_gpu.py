@@ -65,6 +65,9 @@ def test_golden_vectors_fp64(name):
             k += 1
     ch = chain_run(n, chi, cut, ups)
     host = ch.to_host()
+    # the dense contraction on the device equals the host one
+    if n <= 24:
+        assert np.max(np.abs(ch.statevector() - host.get_statevector())) < 1e-12
     assert host.bond_dims() == [int(v) for v in case['bond_dims']]
     assert abs(host.norm_log - float(case['norm_log'])) < 1e-10 * max(1, abs(float(case['norm_log'])))
     sv = host.get_statevector()
@@ -212,6 +215,8 @@ def sim():
 def test_diag_kats(sim, n, ops):
     res = sim.run(_circ(n, ops), shots=256, seed=1)
     sv = res.get_statevector()
+    assert res.get_mps() is not None and res.get_mps().n == sv.size.bit_length() - 1
+    assert np.max(np.abs(res.get_mps().get_statevector() - sv)) < 1e-12
     ref = sv_ref.run_circuit(n, ops)
     fid = abs(np.vdot(sv, ref)) ** 2 / (np.vdot(sv, sv).real * np.vdot(ref, ref).real)
     assert fid > 1 - 1e-12
@@ -311,6 +316,23 @@ def test_host_copy_samples_and_readouts_match_oracle():
     for b in range(n - 1):
         assert abs(host.entanglement_entropy(b) - ref.entanglement_entropy(b)) < 1e-10
         assert np.allclose(host.bond_spectrum(b), ref.bond_spectrum(b), rtol=1e-10, atol=1e-13)
+
+
+@pytest.mark.parametrize('n,chi,depth', [(1, 2, 0), (2, 2, 1), (13, 16, 8), (20, 32, 10)])
+def test_device_statevector_matches_host_contraction_and_oracle(n, chi, depth):
+    from qsim_amd.mps import MPSChain
+    if depth == 0:
+        ch = MPSChain(n, chi, 1e-12)
+        ch.apply_1q(0, np.array([[0.6, 0.8], [0.8, -0.6]], dtype=complex))
+        ref = np.array([0.6, 0.8], dtype=complex)
+    else:
+        ups = workloads.brickwall_gates(n, depth, seed=9, kind='haar')
+        ch = chain_run(n, chi, 1e-12, ups)
+        ref = _oracle_chain(n, chi, 1e-12, ups).get_statevector()
+    sv = ch.statevector()
+    assert sv.shape == (1 << n,)
+    assert np.max(np.abs(sv - ch.to_host().get_statevector())) < 1e-12
+    assert np.max(np.abs(sv - ref)) < 1e-10
 
 
 def test_gpu_sampling_ghz():
