@@ -556,6 +556,52 @@ struct StageScheduler {
     }
 };
 
+// Workgroups are dispatched round-robin over the eight XCDs, so the low three bits of the
+// tile index pick a tile's XCD.  Those bits go to the three highest block qubits: each XCD
+// then streams its own eighth of the state.  Over the tile sets of the 28-qubit layered
+// circuit a load/store probe ran 5 % faster on average (up to 14 %) than with the
+// ascending map, and that circuit's step 2.5 % faster.  The remaining block qubits follow
+// in ascending order.  P.bq lists the
+// block qubits ascending on entry; on exit bit i of the tile index lands on P.bq[i], and
+// the run tables describe the same map.
+constexpr int kXcdBits = 3;
+
+void set_runs(DevPass& P, int& nrun) {
+    nrun = 0;
+    for (int i = 0; i < P.nbq;) {
+        int j = i;
+        while (j + 1 < P.nbq && P.bq[j + 1] == P.bq[j] + 1) ++j;
+        if (nrun < SV_TILE_BITS + 1) {
+            P.run_src[nrun] = i;
+            P.run_dst[nrun] = P.bq[i];
+            P.run_mask[nrun] = (uint32_t)((1ull << (j - i + 1)) - 1);
+        }
+        ++nrun;
+        i = j + 1;
+    }
+    for (int k = nrun; k < SV_TILE_BITS + 1; ++k) {
+        P.run_src[k] = 0;
+        P.run_dst[k] = 0;
+        P.run_mask[k] = 0;
+    }
+}
+
+void set_block_order(DevPass& P) {
+    int nrun = 0;
+    if (P.nbq > kXcdBits) {
+        int ordered[SV_MAX_QUBITS];
+        for (int i = 0; i < kXcdBits; ++i) ordered[i] = P.bq[P.nbq - kXcdBits + i];
+        for (int i = kXcdBits; i < P.nbq; ++i) ordered[i] = P.bq[i - kXcdBits];
+        int asc[SV_MAX_QUBITS];
+        std::copy(P.bq, P.bq + P.nbq, asc);
+        std::copy(ordered, ordered + P.nbq, P.bq);
+        set_runs(P, nrun);
+        if (nrun <= SV_TILE_BITS + 1) return;
+        std::copy(asc, asc + P.nbq, P.bq);    // too many runs: keep the ascending map
+    }
+    set_runs(P, nrun);
+}
+
 void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan) {
     const int K = P.K, R = P.R;
     P.stage0 = (int)plan->stages.size();
@@ -921,18 +967,7 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
         if (!P.tile_is_all)
             for (int q = 0; q < nq; ++q)
                 if (!((tile >> q) & 1)) P.bq[P.nbq++] = q;
-        {
-            int nrun = 0;
-            for (int i = 0; i < P.nbq;) {
-                int j = i;
-                while (j + 1 < P.nbq && P.bq[j + 1] == P.bq[j] + 1) ++j;
-                P.run_src[nrun] = i;
-                P.run_dst[nrun] = P.bq[i];
-                P.run_mask[nrun] = (uint32_t)((1ull << (j - i + 1)) - 1);
-                ++nrun;
-                i = j + 1;
-            }
-        }
+        set_block_order(P);
         build_stages(inc, P, plan);
         plan->passes.push_back(P);
         for (size_t i : inc_idx) done[i] = 1;
