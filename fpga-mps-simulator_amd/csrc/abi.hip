@@ -267,9 +267,9 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
                    << ",\"cmask\":" << G.cmask << ",\"m\":[";
                 // a SIGN op's m holds its SignDesc words (printed under "phase")
                 for (int k = 0; k < 32; ++k)
-                    os << (k ? "," : "") << (G.kind == DG_PHASE_SIGN ? 0.0 : G.m[k]);
+                    os << (k ? "," : "") << (G.kind == DG_PHASE_SIGN || G.kind == DG_SIGN_REG ? 0.0 : G.m[k]);
                 os << "]";
-                if (G.kind == DG_PHASE || G.kind == DG_PHASE_SIGN) {
+                if (G.kind == DG_PHASE || G.kind == DG_PHASE_SIGN || G.kind == DG_SIGN_REG) {
                     const DevPhase& Q = plan.phases[P.phase0 + G.qa];
                     os << ",\"phase\":{\"sign\":" << Q.sign << ",\"slot\":" << Q.slot
                        << ",\"m0\":" << Q.m0 << ",\"mb\":[";

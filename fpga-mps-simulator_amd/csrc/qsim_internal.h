@@ -69,6 +69,9 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //             has only +-1 factors and carries its SignDesc at the start of m; DG_PHASE
 //             carries what the kernel reads of the group: its per-thread table entry
 //             qb, block-factor slot aux and register-pair factors crr in m[0..11]
+//   DG_SIGN_REG  a DG_PHASE_SIGN group whose factors depend on register bits only (no
+//             thread-bit or block terms): amplitude j is negated when bit j of r0 is
+//             set; m and qa as for DG_PHASE_SIGN
 //   DG_LAYOUT transpose through LDS to the DevLayout stored at the start of m
 //   DG_SCALE  every amplitude times (m[0], m[1]); r0 = 1 when the factor is real
 //   DG_ROT + 4 r0 + r1 (16 kinds): scaled real rotation on register bit r0, t = m[0]:
@@ -82,7 +85,8 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //             of them shares one dispatch.
 enum : int32_t {
     DG_U1 = 0, DG_U2 = 1, DG_DIAG1 = 2, DG_DIAG2 = 3, DG_U1X = 4, DG_PHASE = 5,
-    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8, DG_SWAP = 9, DG_SCALE = 10, DG_ROT = 16,
+    DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8, DG_SWAP = 9, DG_SCALE = 10, DG_SIGN_REG = 11,
+    DG_ROT = 16,
     DG_ROTS = 32
 };
 

@@ -428,6 +428,19 @@ __device__ __forceinline__ unsigned sign_mask(const SignDesc& Q, int t) {
     return M & 0xFFFFu;
 }
 
+// DG_SIGN_REG: the negated register indices are the same in every thread, so the
+// factors +-1.0 are formed on the scalar unit and each amplitude costs two FP64
+// products with a scalar operand.
+template <int R>
+__device__ __forceinline__ void apply_sign_reg(cd2 (&a)[1 << R], unsigned M) {
+    const unsigned one_hi = 0x3FF00000u;
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        const double s = __hiloint2double((int)(one_hi | (((M >> j) & 1u) << 31)), 0);
+        a[j] = a[j] * s;
+    }
+}
+
 template <int R>
 __device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], unsigned M,
                                                  const_cd2_ptr bf, int t) {
@@ -605,6 +618,8 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
             apply_rots<R>(a, G, kind, H.m);
         } else if (kind >= DG_ROT) {
             apply_rot<R>(a, G, kind, m0, mk2(m2, m3));
+        } else if (kind == DG_SIGN_REG) {
+            apply_sign_reg<R>(a, (unsigned)r0);
         } else if (kind == DG_U1X) {
             u1_bit<R, 0>(a, G.m);
             if constexpr (R > 1) u1_bit<R, (R > 1 ? 1 : 0)>(a, G.m + 8);

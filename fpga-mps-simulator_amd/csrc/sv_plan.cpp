@@ -337,6 +337,12 @@ void emit_phase_group(const std::vector<const HostGate*>& diag, const int* regs,
         sd.slot = ph.slot;
         static_assert(sizeof(SignDesc) <= sizeof(d.m), "SignDesc must fit in DevGate::m");
         std::memcpy(d.m, &sd, sizeof(sd));
+        bool uniform = ph.slot < 0;
+        for (int b = 0; b < SV_THREAD_BITS; ++b) uniform = uniform && !ph.mb[b] && !ph.pp[b];
+        if (uniform) {
+            d.kind = DG_SIGN_REG;
+            d.r0 = (int32_t)(ph.m0 & ((1u << (1 << P.R)) - 1u));   // 2^R register indices
+        }
     }
     d.qa = (int)plan->phases.size() - P.phase0;
     d.qb = -1;

@@ -11,6 +11,7 @@ import numpy as np
 (DG_U1, DG_U2, DG_DIAG1, DG_DIAG2, DG_U1X, DG_PHASE, DG_LAYOUT, DG_PHASE_SIGN,
  DG_U1S, DG_SWAP) = range(10)
 DG_SCALE = 10
+DG_SIGN_REG = 11     # sign group on register bits only: mask in r0
 DG_ROT = 16          # DG_ROT + 4 * register bit + variant
 DG_ROTS = 32         # DG_ROTS + mask - 1: first-form rotations with factors on mask's bits
 SECOND_FORM_MIN = 1e-3
@@ -126,9 +127,20 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                     i10 = idx[(((idx >> qa) & 1) == 1) & (((idx >> qb) & 1) == 0)]
                     i01 = i10 ^ (1 << qa) ^ (1 << qb)
                     st[i10], st[i01] = st[i01].copy(), st[i10].copy()
-                elif g['kind'] in (DG_PHASE, DG_PHASE_SIGN):
-                    assert (g['kind'] == DG_PHASE_SIGN) == bool(g['phase']['sign'])
-                    st *= phase_factor(g['phase'], tq, regs, idx)
+                elif g['kind'] in (DG_PHASE, DG_PHASE_SIGN, DG_SIGN_REG):
+                    assert (g['kind'] != DG_PHASE) == bool(g['phase']['sign'])
+                    f = phase_factor(g['phase'], tq, regs, idx)
+                    if g['kind'] == DG_SIGN_REG:
+                        ph = g['phase']
+                        assert ph['slot'] < 0 and not any(ph['mb']) and not any(ph['pp'])
+                        # the kernel's reading: amplitude j of the register block is
+                        # negated when bit j of r0 is set
+                        j = np.zeros_like(idx)
+                        for b in range(R):
+                            j |= ((idx >> tq[regs[b]]) & 1) << b
+                        fk = np.where((g['r0'] >> j) & 1, -1.0, 1.0)
+                        assert np.allclose(f[sel], fk[sel], rtol=0, atol=1e-12), "register sign mask"
+                    st *= f
                 elif g['kind'] == DG_DIAG1:
                     d = _cm(g['m'], 2)
                     b = (idx >> g['qa']) & 1
