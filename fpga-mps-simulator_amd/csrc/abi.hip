@@ -264,12 +264,13 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
                 const DevGate& G = plan.gates[P.gate0 + gi];
                 os << (gi > S.g0 ? "," : "") << "{\"kind\":" << G.kind << ",\"r0\":" << G.r0
                    << ",\"r1\":" << G.r1 << ",\"qa\":" << G.qa << ",\"qb\":" << G.qb
-                   << ",\"cmask\":" << G.cmask << ",\"m\":[";
+                   << ",\"cmask\":" << G.cmask << ",\"aux\":" << G.aux << ",\"m\":[";
                 // a SIGN op's m holds its SignDesc words (printed under "phase")
                 for (int k = 0; k < 32; ++k)
                     os << (k ? "," : "") << (G.kind == DG_PHASE_SIGN || G.kind == DG_SIGN_REG ? 0.0 : G.m[k]);
                 os << "]";
-                if (G.kind == DG_PHASE || G.kind == DG_PHASE_SIGN || G.kind == DG_SIGN_REG) {
+                if (G.kind == DG_PHASE || G.kind == DG_PHASE_SIGN || G.kind == DG_SIGN_REG ||
+                    (G.kind >= DG_ROT && G.aux != 0)) {
                     const DevPhase& Q = plan.phases[P.phase0 + G.qa];
                     os << ",\"phase\":{\"sign\":" << Q.sign << ",\"slot\":" << Q.slot
                        << ",\"m0\":" << Q.m0 << ",\"mb\":[";

@@ -78,7 +78,9 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //             r1 bit 0 clear maps a pair (x0, x1) to (x0 - t x1, x1 + t x0), set to
 //             (t x0 - x1, x0 + t x1); r1 bit 1 set multiplies the second output by
 //             k = (m[2], m[3]).  Bit and variant live in the kind so that the kernel
-//             selects the straight-line body with one switch.
+//             selects the straight-line body with one switch.  Nonzero aux (DG_ROT and
+//             DG_ROTS): a DG_SIGN_REG mask applied after the rotations, with its phase
+//             record in qa.
 //   DG_ROTS + mask - 1 (15 kinds): first-form rotations with output factors on every
 //             register bit of mask = r0 (1..15), one op: bit b has t = m[b] and
 //             k = (m[4 + 2b], m[5 + 2b]).  Rotations on distinct bits commute, so a run

@@ -616,8 +616,10 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
                      "s"(H.m[11]));
         if (kind >= DG_ROTS) {
             apply_rots<R>(a, G, kind, H.m);
+            if (H.aux) apply_sign_reg<R>(a, (unsigned)H.aux);
         } else if (kind >= DG_ROT) {
             apply_rot<R>(a, G, kind, m0, mk2(m2, m3));
+            if (H.aux) apply_sign_reg<R>(a, (unsigned)H.aux);
         } else if (kind == DG_SIGN_REG) {
             apply_sign_reg<R>(a, (unsigned)r0);
         } else if (kind == DG_U1X) {

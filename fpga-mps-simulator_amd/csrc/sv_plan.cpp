@@ -685,6 +685,12 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
             if (cur[i] == lp) return i;
         return -1;
     };
+    // Index of the last rotation op that later first-form rotations with an output
+    // factor may join (a DG_ROT of that variant or a DG_ROTS), valid while it is the
+    // last op emitted.
+    int open_rots = -1;
+    // First op of the current stage (after its DG_LAYOUT) in plan->gates.
+    size_t stage_first = plan->gates.size();
     auto emit_diagonals = [&](const std::vector<int>& idx) {
         if (idx.empty()) return;
         close_u1();
@@ -697,7 +703,23 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
             else
                 general.push_back(&inc[j]->g);
         }
-        if (!group.empty()) emit_phase_group(group, cur, P, plan);
+        if (!group.empty()) {
+            emit_phase_group(group, cur, P, plan);
+            // A register-only sign group right after a rotation op of the same stage
+            // rides on that op (its mask in aux, its phase record in qa): one dispatch
+            // fewer.  The op then takes no further rotations, which would have to
+            // precede the signs.
+            const size_t n = plan->gates.size();
+            if (n >= 2 && n - 2 >= stage_first && plan->gates[n - 1].kind == DG_SIGN_REG &&
+                plan->gates[n - 2].kind >= DG_ROT && plan->gates[n - 2].aux == 0 &&
+                plan->gates[n - 1].r0 != 0) {
+                DevGate& rot = plan->gates[n - 2];
+                rot.aux = plan->gates[n - 1].r0;
+                rot.qa = plan->gates[n - 1].qa;
+                plan->gates.pop_back();
+                open_rots = -1;
+            }
+        }
         for (const HostGate* g : general) {   // non-unit entries or several controls
             DevGate d{};
             d.cmask = g->cmask;
@@ -708,10 +730,6 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
             plan->gates.push_back(d);
         }
     };
-    // Index of the last rotation op that later first-form rotations with an output
-    // factor may join (a DG_ROT of that variant or a DG_ROTS), valid while it is the
-    // last op emitted.
-    int open_rots = -1;
     auto emit_gate = [&](int j) {
         const HostGate& g = inc[j]->g;
         const int r0 = reg_index(local_pos(P.tq, K, g.q0));
@@ -733,7 +751,9 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
             const cd k = inc[j]->post;
             const bool post = k != cd(1.0, 0.0);
             const bool joins = form == 1 && post;
-            DevGate* o = (joins && open_rots >= 0 && open_rots == (int)plan->gates.size() - 1)
+            DevGate* o = (joins && open_rots >= 0 &&
+                          open_rots == (int)plan->gates.size() - 1 &&
+                          plan->gates[open_rots].aux == 0)
                              ? &plan->gates[open_rots]
                              : nullptr;
             const int omask = !o ? 0 : o->kind >= DG_ROTS ? o->r0 : 1 << o->r0;
@@ -840,6 +860,7 @@ void build_stages(const std::vector<const Item*>& inc, DevPass& P, SvPlan* plan)
         for (int i = 0; i < R; ++i) stg.regs[i] = cur[i];
         stg.g0 = (int)plan->gates.size() - P.gate0;
         if (masks[si] != curmask) emit_layout(masks[si]);
+        stage_first = plan->gates.size();
         curmask = masks[si];
         S.run_stage(st, masks[si], [&](const std::vector<int>& diags, int j) {
             if (j < 0)

@@ -127,20 +127,11 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                     i10 = idx[(((idx >> qa) & 1) == 1) & (((idx >> qb) & 1) == 0)]
                     i01 = i10 ^ (1 << qa) ^ (1 << qb)
                     st[i10], st[i01] = st[i01].copy(), st[i10].copy()
-                elif g['kind'] in (DG_PHASE, DG_PHASE_SIGN, DG_SIGN_REG):
-                    assert (g['kind'] != DG_PHASE) == bool(g['phase']['sign'])
-                    f = phase_factor(g['phase'], tq, regs, idx)
-                    if g['kind'] == DG_SIGN_REG:
-                        ph = g['phase']
-                        assert ph['slot'] < 0 and not any(ph['mb']) and not any(ph['pp'])
-                        # the kernel's reading: amplitude j of the register block is
-                        # negated when bit j of r0 is set
-                        j = np.zeros_like(idx)
-                        for b in range(R):
-                            j |= ((idx >> tq[regs[b]]) & 1) << b
-                        fk = np.where((g['r0'] >> j) & 1, -1.0, 1.0)
-                        assert np.allclose(f[sel], fk[sel], rtol=0, atol=1e-12), "register sign mask"
-                    st *= f
+                elif g['kind'] == DG_SIGN_REG:
+                    st *= register_signs(g['phase'], g['r0'], tq, regs, idx)
+                elif g['kind'] in (DG_PHASE, DG_PHASE_SIGN):
+                    assert (g['kind'] == DG_PHASE_SIGN) == bool(g['phase']['sign'])
+                    st *= phase_factor(g['phase'], tq, regs, idx)
                 elif g['kind'] == DG_DIAG1:
                     d = _cm(g['m'], 2)
                     b = (idx >> g['qa']) & 1
@@ -152,12 +143,29 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                 else:
                     raise AssertionError(f"unknown kind {g['kind']}")
                 # every non-diagonal target must be a tile qubit
+                # a register-only sign group riding on a rotation op, applied after it
+                if g['kind'] >= DG_ROT and g.get('aux', 0):
+                    st *= register_signs(g['phase'], g['aux'], tq, regs, idx)
                 if g['kind'] in (DG_U1, DG_U2, DG_U1S) or DG_ROT <= g['kind'] < DG_ROTS:
                     assert (tile >> tq[regs[g['r0']]]) & 1
                 if g['kind'] == DG_U1:
                     assert cm != 0, 'uncontrolled one-qubit gates travel as U1X, U1S or ROT'
         assert all(r >= low for r in layout) or layout == default, layout
     return st
+
+
+def register_signs(ph, mask, tq, regs, idx):
+    """Factor of a sign group on register bits only, as the kernel reads it: amplitude j
+    of the register block is negated when bit j of `mask` is set.  Checked against the
+    group's own terms."""
+    assert ph['sign'] and ph['slot'] < 0 and not any(ph['mb']) and not any(ph['pp'])
+    f = phase_factor(ph, tq, regs, idx)
+    j = np.zeros_like(idx)
+    for b in range(len(regs)):
+        j |= ((idx >> tq[regs[b]]) & 1) << b
+    fk = np.where((mask >> j) & 1, -1.0, 1.0)
+    assert np.allclose(f, fk, rtol=0, atol=1e-12), 'register sign mask'
+    return fk
 
 
 def phase_factor(ph, tq, regs, idx):
