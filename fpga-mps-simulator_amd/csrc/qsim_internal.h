@@ -68,7 +68,9 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //   DG_PHASE / DG_PHASE_SIGN  unitary diagonal group, DevPhase record qa; the SIGN form
 //             has only +-1 factors and carries its SignDesc at the start of m; DG_PHASE
 //             carries what the kernel reads of the group: its per-thread table entry
-//             qb, block-factor slot aux and register-pair factors crr in m[0..11]
+//             qb, block-factor slot aux and register-pair factors crr in m[0..11];
+//             r0 bit 0 / bit 1: the group has thread x block / register x block terms
+//             (otherwise those block factors are ones and are not applied)
 //   DG_SIGN_REG  a DG_PHASE_SIGN group whose factors depend on register bits only (no
 //             thread-bit or block terms): amplitude j is negated when bit j of r0 is
 //             set; m and qa as for DG_PHASE_SIGN

@@ -479,19 +479,25 @@ __device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], unsigned M,
 template <int R>
 __device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const double* crr,
                                                  const cd2 (&e)[PHASE_THREAD_VALUES],
-                                                 const_cd2_ptr bf, int t) {
+                                                 const_cd2_ptr bf, int t, int parts) {
     cd2 A = e[0], k0 = e[1], k1 = e[2], k2 = e[3], k3 = e[4];
     if (bf) {
         A = cmul(A, bf[0]);
+        // Uniform branches: a group whose block terms all touch one register qubit (the
+        // QFT's controlled phases after each H) has no thread x block factors.
+        if (parts & 1) {
 #pragma unroll
-        for (int q = 0; q < SV_THREAD_BITS; ++q) {
-            const cd2 Af = cmul(A, bf[1 + q]);
-            A = ((t >> q) & 1) ? Af : A;
+            for (int q = 0; q < SV_THREAD_BITS; ++q) {
+                const cd2 Af = cmul(A, bf[1 + q]);
+                A = ((t >> q) & 1) ? Af : A;
+            }
         }
-        k0 = cmul(k0, bf[PHASE_KAPPA0]);
-        if (R > 1) k1 = cmul(k1, bf[PHASE_KAPPA0 + 1]);
-        if (R > 2) k2 = cmul(k2, bf[PHASE_KAPPA0 + 2]);
-        if (R > 3) k3 = cmul(k3, bf[PHASE_KAPPA0 + 3]);
+        if (parts & 2) {
+            k0 = cmul(k0, bf[PHASE_KAPPA0]);
+            if (R > 1) k1 = cmul(k1, bf[PHASE_KAPPA0 + 1]);
+            if (R > 2) k2 = cmul(k2, bf[PHASE_KAPPA0 + 2]);
+            if (R > 3) k3 = cmul(k3, bf[PHASE_KAPPA0 + 3]);
+        }
     }
     // register-pair factors (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
     const cd2 c01 = mk2(crr[0], crr[1]), c02 = mk2(crr[2], crr[3]);
@@ -657,7 +663,7 @@ __device__ __forceinline__ void run_ops(cd2 (&a)[1 << R], Layout<R>& Lcur,
                           : nullptr;
             const cd2* e = tfactors + (size_t)(tab + t) * PHASE_THREAD_VALUES;
             const cd2 f[PHASE_THREAD_VALUES] = {e[0], e[1], e[2], e[3], e[4]};
-            apply_phase_cplx<R>(a, H.m, f, bf, t);
+            apply_phase_cplx<R>(a, H.m, f, bf, t, r0);
         } else if (kind == DG_LAYOUT) {
 #pragma unroll
             for (int j = 0; j < NR; ++j) lds[lds_slot(Lcur.loc(j))] = a[j];

@@ -349,6 +349,12 @@ void emit_phase_group(const std::vector<const HostGate*>& diag, const int* regs,
     if (!sign) {
         d.qb = ph.tab;
         d.aux = ph.slot;
+        // which parts of the block-factor record are not all ones: bit 0 the thread-bit
+        // factors (thread x block terms), bit 1 the register factors (register x block)
+        for (const DevTerm& x : bt) {
+            if (x.cls == PT_BT) d.r0 |= 1;
+            if (x.cls == PT_BR) d.r0 |= 2;
+        }
         for (int i = 0; i < 6; ++i) {
             d.m[2 * i] = ph.crr[i][0];
             d.m[2 * i + 1] = ph.crr[i][1];

@@ -131,6 +131,11 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                     st *= register_signs(g['phase'], g['r0'], tq, regs, idx)
                 elif g['kind'] in (DG_PHASE, DG_PHASE_SIGN):
                     assert (g['kind'] == DG_PHASE_SIGN) == bool(g['phase']['sign'])
+                    if g['kind'] == DG_PHASE:
+                        # the kernel skips the block factors whose flag is clear
+                        classes = {c for c, *_ in g['phase']['terms']}
+                        assert bool(g['r0'] & 1) == (PT_BT in classes), 'thread x block flag'
+                        assert bool(g['r0'] & 2) == (PT_BR in classes), 'register x block flag'
                     st *= phase_factor(g['phase'], tq, regs, idx)
                 elif g['kind'] == DG_DIAG1:
                     d = _cm(g['m'], 2)
