@@ -254,6 +254,11 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
            << P.load.regs[0] << "," << P.load.regs[1] << "," << P.load.regs[2] << ","
            << P.load.regs[3] << "],\"tq\":[";
         for (int i = 0; i < P.K; ++i) os << (i ? "," : "") << P.tq[i];
+        // the tile-index map: [src, dst, mask] per run of consecutive block qubits
+        os << "],\"runs\":[";
+        for (int k = 0; k < SV_TILE_BITS + 1; ++k)
+            os << (k ? "," : "") << "[" << P.run_src[k] << "," << P.run_dst[k] << ","
+               << P.run_mask[k] << "]";
         os << "],\"stages\":[";
         for (int s = 0; s < P.nstage; ++s) {
             const DevStage& S = plan.stages[P.stage0 + s];

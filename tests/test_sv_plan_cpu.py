@@ -87,6 +87,32 @@ def test_plan_replay_benchmark_structure():
     assert any(k >= 32 for k in kinds)                        # fused rotation runs
 
 
+@pytest.mark.parametrize('nq', [13, 15, 18, 24])
+def test_tile_index_map_is_a_bijection(nq):
+    # Each pass's run tables send the tile index bits one to one onto the qubits outside
+    # the tile; with more than three of them, index bits 0..2 (which pick the XCD a
+    # workgroup runs on) land on the three highest, the rest follow in ascending order.
+    _lib_or_skip()
+    from qsim_amd import workloads
+    gates = workloads.u3_cz_layers(nq, 4, seed=nq) + workloads.qft_gates(nq)
+    recs, _ = gates_to_records(gates, nq)
+    plan = plan_json(nq, recs)
+    for p in plan['passes']:
+        block = [q for q in range(nq) if q not in p['tq']]
+        bits = {}
+        for src, dst, mask in p['runs']:
+            assert mask & (mask + 1) == 0, 'a run covers consecutive index bits'
+            for i in range(mask.bit_length()):
+                assert src + i not in bits
+                bits[src + i] = dst + i
+        m = [bits[i] for i in range(len(bits))]
+        assert sorted(m) == block
+        if len(block) > 3:
+            assert m == block[-3:] + block[:-3]
+        else:
+            assert m == block
+
+
 def test_plan_u3_cz_layers_pass_count():
     _lib_or_skip()
     nq = 20
