@@ -105,7 +105,9 @@ int qs_sv_apply_words(void* state, int nq, const int32_t* gate_sequence, int num
 int qs_sv_plan_passes(int nq, const qs_gate* gates, int ng, int* npasses);
 
 /* Diagnostics (host only, no device needed): the fused-pass plan as JSON
- * {"passes":[{"K","R","tq":[...],"stages":[{"regs":[...],"gates":[...]}]}]}.
+ * {"passes":[{"K","R","load_regs":[...],"tq":[...],"runs":[[src,dst,mask],...],
+ *              "stages":[{"regs":[...],"gates":[...]}]}]}: the tile qubits, the map from
+ * tile index bits to the other qubits as runs, and per stage the register bits and ops.
  * Writes at most `cap` bytes into `buf` (NUL-terminated when it fits) and the size
  * required into *needed.  No counterpart in the reference; it lets the planner be
  * tested without a GPU. */
