@@ -17,7 +17,16 @@ Deviations (each deliberate):
     enforces CHI_MAX_SVD_HW = 64);
   * xclbin_path / FPGA_MPS_XCLBIN are accepted and ignored;
   * the keyword `precision` ('fp64' default, or 'fp32' for the complex-FP32
-    MFMA contraction) has no counterpart in the reference.
+    MFMA contraction) has no counterpart in the reference;
+  * the keyword `fix_ccx_orientation` (default False) has no counterpart in the
+    reference.  By default compile_circuit emits what the reference emits: an
+    adjacent CX of the Toffoli decomposition goes out with the unreversed matrix
+    (fpga_mps_simulator.py:395-402), so when its control site is above its
+    target site control and target are exchanged.  With the keyword set, that
+    CX gets the [0,2,1,3] reordering every other reversed two-qubit gate gets.
+
+Matching the reference, 'cnot' compiles to an error (mps_core._gate_2q has no
+'cnot' matrix) and gate names the compiler does not know are skipped.
 """
 
 from __future__ import annotations
@@ -79,16 +88,17 @@ def _decompose_ccx(c0: int, c1: int, tgt: int) -> list:
             ('cx', [c0, c1], None)]
 
 
-def _mat_2q(name: str, params) -> np.ndarray:
-    return matrix_2q('cx' if name == 'cnot' else name, params)
-
-
-def compile_circuit(circuit, n_qubits: int) -> Tuple[list, list]:
+def compile_circuit(circuit, n_qubits: int,
+                    fix_ccx_orientation: bool = False) -> Tuple[list, list]:
     """Circuit -> nearest-neighbour gate list + measured sites (fpga_mps_simulator.py:285-410).
 
     Gate dicts carry 'type' ('1q' | '2q' | 'reset'), 'gate_name', 'sites', 'params',
     'opcode' and 'gate_mat'.  Non-adjacent two-qubit gates are routed by SWAP chains
-    that walk the far qubit next to the near one and back again.
+    that walk the far qubit next to the near one and back again.  'cnot' raises
+    ValueError (matrix_2q has no 'cnot', as mps_core._gate_2q has none) and unknown
+    names are skipped, as in the reference (:338-405 fall through).  An adjacent CX
+    of a Toffoli decomposition keeps the reference's unreversed matrix (:395-402)
+    unless `fix_ccx_orientation` is set.
     """
     perm = list(range(n_qubits))
     out: list = []
@@ -105,7 +115,7 @@ def compile_circuit(circuit, n_qubits: int) -> Tuple[list, list]:
                 perm[k] = lo
 
     def emit_2q(pa: int, pb: int, nm: str, ps):
-        g = _mat_2q(nm, ps)
+        g = matrix_2q(nm, ps)
         if pa > pb:
             g = swap_qubit_order(g)
         lo, hi = min(pa, pb), max(pa, pb)
@@ -145,10 +155,15 @@ def compile_circuit(circuit, n_qubits: int) -> Tuple[list, list]:
                     out.append({'type': '1q', 'gate_name': snm, 'sites': [sqs[0]],
                                 'params': sps, 'opcode': _OP1Q.get(snm, 0),
                                 'gate_mat': matrix_1q(snm, sps)})
+                elif abs(sqs[0] - sqs[1]) == 1 and not fix_ccx_orientation:
+                    out.append({'type': '2q', 'gate_name': snm,
+                                'sites': [min(sqs), max(sqs)], 'params': sps,
+                                'opcode': _OP2Q.get(snm, 17),
+                                'gate_mat': matrix_2q(snm, sps)})
                 else:
                     emit_2q(sqs[0], sqs[1], snm, sps)
             continue
-        raise ValueError(f"Unknown gate for the MPS path: {nm}")
+        logger.debug('compile_circuit: gate %r not supported on the MPS path, skipped', nm)
     if not measured:
         measured = list(range(n_qubits))
     return out, sorted(measured)
@@ -226,13 +241,13 @@ def run_gate_list(gate_list, n: int, chi_max: int, svd_cutoff: float, precision:
 
 
 def _run_circuit(circuit, shots: int, chi_max: int, svd_cutoff: float, seed: Optional[int],
-                 precision: str, device: int):
+                 precision: str, device: int, fix_ccx_orientation: bool = False):
     """One circuit: compile, run on the GPU, sample (fpga_mps_simulator.py:417-654)."""
     t_total0 = time.perf_counter()
     pt = {'fpga_kernel': 0.0, 'hbm_transfer': 0.0, 'cpu_svd': 0.0, 'cpu_qr': 0.0,
           'cpu_sampling': 0.0}
     nq = circuit.num_qubits
-    gate_list, measured = compile_circuit(circuit, nq)
+    gate_list, measured = compile_circuit(circuit, nq, fix_ccx_orientation)
     chain = run_gate_list(gate_list, nq, chi_max, svd_cutoff, precision, device, pt)
     t0 = time.perf_counter()
     host = chain.to_host()
@@ -291,7 +306,8 @@ class FPGAMPSSimulator:
 
     def __init__(self, num_cards: int = NUM_CARDS, chi_max: int = CHI_MAX_SVD_HW,
                  svd_cutoff: float = 1e-12, engine: str = 'fpga', mode: str = 'shots',
-                 xclbin_path: Optional[str] = None, precision: str = 'fp64', device: int = 0):
+                 xclbin_path: Optional[str] = None, precision: str = 'fp64', device: int = 0,
+                 fix_ccx_orientation: bool = False):
         if num_cards != 1:
             raise ValueError(f'Single-card verification build: num_cards must be 1, got {num_cards}. '
                              'Multi-card operation is not permitted in this environment.')
@@ -317,6 +333,7 @@ class FPGAMPSSimulator:
         self.mode = mode
         self.precision = precision
         self.device = device
+        self.fix_ccx_orientation = bool(fix_ccx_orientation)
         self._engine = 'fpga'
         self._xclbin = xclbin_path or os.environ.get('FPGA_MPS_XCLBIN')
 
@@ -328,8 +345,9 @@ class FPGAMPSSimulator:
         for ti, circuit in enumerate(circuits):
             s = (seed + ti) if seed is not None else None
             try:
-                host, counts, pt, sv = _run_circuit(circuit, shots, self.chi_max, self.svd_cutoff, s,
-                                                self.precision, self.device)
+                host, counts, pt, sv = _run_circuit(circuit, shots, self.chi_max,
+                                                    self.svd_cutoff, s, self.precision,
+                                                    self.device, self.fix_ccx_orientation)
             except (ValueError, AssertionError) as e:
                 raise RuntimeError(f'Card 0 error: {e}') from e
             except RuntimeError as e:

@@ -202,10 +202,11 @@ class MPSChain:
                max_shots_per_launch: int = 65536) -> List[str]:
         """Sequential conditional sampling on the device (mps_core.py:371-420).
 
-        Bitstrings as the reference returns them (qubit n-1 first; with `measured`, the
-        measured qubits in descending order).  The uniforms are drawn from `rng` in the
-        reference's order (one per site per shot, shot-major), so a seeded generator
-        gives the reference's samples.
+        Bitstrings as the reference returns them: qubit n-1 first; with a measured
+        subset, one character per measured site q in descending order holding the
+        outcome of site n-1-q, the mirrored site (mps_core.py:415-417).  The uniforms
+        are drawn from `rng` in the reference's order (one per site per shot,
+        shot-major), so a seeded generator gives the reference's samples.
         """
         import torch
         rng = rng if rng is not None else np.random.default_rng()
@@ -223,7 +224,7 @@ class MPSChain:
                   ctypes.c_void_p(self.bond.data_ptr()), self.stride, n, self.chi_max,
                   ctypes.c_void_p(envs.data_ptr()), nbytes, stream)
         cols = (list(range(n - 1, -1, -1)) if set(measured) == set(range(n))
-                else sorted(measured, reverse=True))
+                else [n - 1 - q for q in sorted(measured, reverse=True)])
         out: List[str] = []
         for b0 in range(0, shots, max_shots_per_launch):
             k = min(max_shots_per_launch, shots - b0)

@@ -275,7 +275,12 @@ class MPS:
 
     def sample(self, shots: int, measured: Optional[List[int]] = None,
                rng: Optional[np.random.Generator] = None) -> List[str]:
-        """Sequential conditional sampling; bitstring[0] is qubit n-1."""
+        """Sequential conditional sampling (mps_core.py:371-420); bitstring[0] is qubit n-1.
+
+        With a measured subset the key is built as mps_core.py:415-417 builds it: for q
+        in the measured sites in descending order, the outcome of site n-1-q (the
+        mirrored site, not site q itself).
+        """
         rng = rng if rng is not None else np.random.default_rng()
         measured = list(range(self.n)) if measured is None else list(measured)
         envs = self.right_environments()
@@ -303,7 +308,8 @@ class MPS:
             if full:
                 out.append(''.join(str(bits[q]) for q in range(self.n - 1, -1, -1)))
             else:
-                out.append(''.join(str(bits[q]) for q in sorted(measured, reverse=True)))
+                out.append(''.join(str(bits[self.n - 1 - q])
+                                   for q in sorted(measured, reverse=True)))
         return out
 
     def get_counts(self, shots, measured=None, rng=None) -> Dict[str, int]:

@@ -358,3 +358,40 @@ def test_gpu_sampling_200_qubits_chi32_matches_oracle():
     got = ch.sample(48, None, np.random.default_rng(2))
     exp = ref.sample(48, None, np.random.default_rng(2))
     assert got == exp
+
+
+# ------------------------------------------- fixtures written by the reference itself
+def test_gpu_samples_equal_mps_core_fixture():
+    """Seeded shots of mps_core.MPSState.sample (tests/golden/mps_core_samples_10q.npz,
+    written by the reference) for the full register and three measured subsets; the
+    subset key takes the mirrored site's outcome (mps_core.py:415-417)."""
+    from qsim_amd.mps import MPSChain
+    case = np.load(os.path.join(HERE, 'golden', 'mps_core_samples_10q.npz'))
+    n = int(case['n'])
+    ch = MPSChain(n, int(case['chi_max']), float(case['cutoff']))
+    for g, i in zip(case['gates'], case['sites']):
+        ch.apply_2q(int(i), g)
+    ch.check_status()
+    for name in ('full', 'sub_a', 'sub_b', 'sub_c'):
+        meas = [int(q) for q in case[f'{name}_measured']] or None
+        exp = [str(s) for s in case[f'{name}_samples']]
+        got = ch.sample(len(exp), meas, np.random.default_rng(int(case[f'{name}_rng_seed'])))
+        assert got == exp, name
+
+
+def test_dropin_run_equals_reference_compiler_fixture(sim):
+    """FPGAMPSSimulator.run on a circuit with Toffolis whose adjacent sub-CX gates have
+    the control site above the target, a routed gate, a reset, an unknown gate name and
+    a measured subset: statevector, bond dims and seeded counts equal what the
+    reference's compile_circuit + gate loop + sampling produce
+    (tests/golden/mps_compile_ccx_6q.npz)."""
+    from collections import Counter
+    from golden.fixture_circuits import ccx_circuit
+    case = np.load(os.path.join(HERE, 'golden', 'mps_compile_ccx_6q.npz'))
+    qc = ccx_circuit()
+    res = sim.run(qc, shots=int(case['shots']), seed=int(case['rng_seed']))
+    sv = res.get_statevector()
+    ref = case['statevector']
+    assert np.max(np.abs(sv - ref)) < 1e-10 * max(1.0, float(np.max(np.abs(ref))))
+    assert res.get_bond_dims() == [int(b) for b in case['bond_dims']]
+    assert res.get_counts() == dict(Counter(str(s) for s in case['samples']))
