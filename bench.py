@@ -4,7 +4,8 @@
 Headline (`value`): 28-qubit complex128 statevector sweep, gates/s, on the
 workload "20 layers of U3 on every qubit + CZ ladder, then QFT-28" (1520
 logical gates) with the state resident in HBM.  A step applies the whole
-circuit once.  The line also carries:
+circuit once: one launch of a plan built before the timed region
+(qs_sv_plan_create; its host time is reported as `config.plan_ms`).  The line also carries:
   roofline     : algorithmic HBM bytes of the dominant kernel (the fused pass:
                  read + write of the 4 GiB state) per launch over its average
                  duration, timed with HIP events around every pass launch of the
@@ -60,6 +61,9 @@ def parse():
     ap.add_argument('--multi', type=int, default=0, metavar='NCIRC',
                     help='instead of the headline line, print one JSON line for NCIRC '
                          'independent MPS circuits (chi_max 16..128) replayed as one graph')
+    ap.add_argument('--pass-table', metavar='PATH', default=None,
+                    help='also write a per-pass table (mean kernel time over the timed steps, '
+                         'tile qubits, stages and op counts by kind) as markdown to PATH')
     ap.add_argument('--calibration-copy', action='store_true',
                     help='after the SV leg, copy the 4 GiB state once (a copy kernel of '
                          'known size for scaling the HBM byte counters of a profile)')
@@ -69,15 +73,19 @@ def parse():
 def sv_leg(steps: int, warmup: int, calibration_copy: bool = False):
     import torch
     from qsim_amd import workloads
-    from qsim_amd.sv import StateVector, gates_to_records, pass_times, plan_passes, time_passes
+    from qsim_amd.sv import StateVector, gates_to_records, pass_times, time_passes
 
     gates = workloads.sv_benchmark_circuit(SV_QUBITS, SV_DEPTH, SV_SEED)
     recs, _ = gates_to_records(gates, SV_QUBITS)
-    npass = plan_passes(SV_QUBITS, recs)
     sv = StateVector(SV_QUBITS, 0)
     stream = torch.cuda.current_stream()
+    # plan once (host planner + upload), outside the timed region and reported apart
+    t0 = time.perf_counter()
+    plan = sv.plan(recs)
+    plan_s = time.perf_counter() - t0
+    npass = plan.npasses
     for _ in range(warmup):
-        sv.apply_records(recs)
+        sv.launch(plan)
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     # the library brackets every pass-kernel launch with HIP events on this stream
@@ -85,7 +93,7 @@ def sv_leg(steps: int, warmup: int, calibration_copy: bool = False):
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(steps):
-        sv.apply_records(recs)
+        sv.launch(plan)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -109,11 +117,53 @@ def sv_leg(steps: int, warmup: int, calibration_copy: bool = False):
     if len(kernel_ms) != steps * npass:
         raise RuntimeError(f'expected {steps * npass} pass timings, got {len(kernel_ms)}')
     per_launch_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+    per_pass_ms = [sum(kernel_ms[k * npass + p] for k in range(steps)) / steps
+                   for p in range(npass)]
     return {
+        'per_pass_ms': per_pass_ms, 'records': recs, 'plan_s': plan_s,
+        'plan_device_bytes': plan.device_bytes,
         'wall_s': wall, 'gpu_ms': gpu_ms, 'npass': npass, 'ngates': len(gates),
         'per_launch_s': per_launch_s, 'bytes_per_pass': bytes_per_pass,
         'norm_sq': norm, 'inplace_rmw_gbps': rmw_gbps,
     }
+
+
+_OP_NAMES = {0: 'U1', 1: 'U2', 2: 'DIAG1', 3: 'DIAG2', 4: 'U1X', 5: 'PHASE', 6: 'LAYOUT',
+             7: 'SIGN', 8: 'U1S', 9: 'SWAP', 10: 'SCALE', 11: 'SIGN_REG'}
+
+
+def write_pass_table(path: str, sv: dict, cmd: str) -> None:
+    """Markdown table of the fused passes of one step: mean kernel time over the timed
+    steps, bandwidth, tile qubits, stages and ops by kind (from the plan JSON)."""
+    from qsim_amd.sv import plan_json
+    plan = plan_json(SV_QUBITS, sv['records'])['passes']
+    bpp = sv['bytes_per_pass']
+    lines = [f'# Fused passes of the 28-qubit statevector step ({len(plan)} passes)', '',
+             f'command: `{cmd}`', '',
+             'Per pass: mean duration of the pass kernel over the timed steps (HIP events on '
+             'the launch stream), algorithmic bandwidth (2 x 16 x 2^28 B per pass), tile '
+             'qubits, stages and op counts by kind (ROT/ROTS = rotations, SIGN = +-1 phase '
+             'groups, PHASE = complex phase groups).', '',
+             '| pass | ms | TB/s | frac of 8 TB/s | tile qubits | stages | ops |',
+             '|---:|---:|---:|---:|---|---:|---|']
+    total = 0.0
+    for i, (ms, P) in enumerate(zip(sv['per_pass_ms'], plan)):
+        total += ms
+        kinds = {}
+        for S in P['stages']:
+            for G in S['gates']:
+                k = G['kind']
+                nm = 'ROTS' if k >= 32 else 'ROT' if k >= 16 else _OP_NAMES.get(k, str(k))
+                kinds[nm] = kinds.get(nm, 0) + 1
+        ops = ' '.join(f'{k}:{v}' for k, v in sorted(kinds.items()))
+        tq = ','.join(str(q) for q in P['tq'])
+        tbs = bpp / (ms / 1e3) / 1e12
+        lines.append(f'| {i} | {ms:.3f} | {tbs:.2f} | {tbs / 8.0:.3f} | {tq} | '
+                     f'{len(P["stages"])} | {ops} |')
+    lines += ['', f'sum of pass means: {total:.2f} ms per step; mean per pass '
+              f'{total / len(plan):.3f} ms = {bpp / (total / len(plan) / 1e3) / 1e12:.2f} TB/s']
+    with open(path, 'w') as f:
+        f.write('\n'.join(lines) + '\n')
 
 
 def cpu_leg(ngates: int):
@@ -212,6 +262,8 @@ def main():
         print(json.dumps(mps_bench.run_multi(args.multi)))
         return
     sv = sv_leg(args.steps, args.warmup, args.calibration_copy)
+    if args.pass_table:
+        write_pass_table(args.pass_table, sv, 'python bench.py ' + ' '.join(sys.argv[1:]))
     value = sv['ngates'] * args.steps / (sv['gpu_ms'] / 1e3)
     achieved = sv['bytes_per_pass'] / sv['per_launch_s'] / 1e9
     out = {
@@ -230,6 +282,7 @@ def main():
         'config': {'workload': '28q statevector: 20x[U3 all qubits + CZ ladder] + QFT-28 '
                                '(1520 gates), complex128, state resident in HBM',
                    'qubits': SV_QUBITS, 'gates': sv['ngates'], 'passes_per_step': sv['npass'],
+                   'plan_ms': sv['plan_s'] * 1e3, 'plan_device_bytes': sv['plan_device_bytes'],
                    'hbm_gbps_whole_step': sv['npass'] * sv['bytes_per_pass'] * args.steps /
                    (sv['gpu_ms'] / 1e3) / 1e9},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBPS,

@@ -25,6 +25,31 @@ void set_error(const char* fmt, ...) {
 void clear_error() { g_last_error.clear(); }
 
 namespace {
+// One event per host thread marks the completion of that thread's last upload.
+struct CopyEvent {
+    hipEvent_t e = nullptr;
+    ~CopyEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+thread_local CopyEvent g_copy_event;
+}  // namespace
+
+int h2d_copy(void* dst, const void* src, size_t bytes, hipStream_t stream, hipEvent_t* done) {
+    *done = nullptr;
+    if (!g_copy_event.e) QS_HIP(hipEventCreateWithFlags(&g_copy_event.e, hipEventDisableTiming));
+    QS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    QS_HIP(hipEventRecord(g_copy_event.e, stream));
+    *done = g_copy_event.e;
+    return QS_OK;
+}
+
+int h2d_copy_wait(hipEvent_t done) {
+    QS_HIP(hipEventSynchronize(done));
+    return QS_OK;
+}
+
+namespace {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
@@ -94,9 +119,9 @@ double f32_word(int32_t w) {
     return (double)f;
 }
 
-// Uploads the plan's tables with a stream-ordered allocation and launches it.
-int run_plan(double2* state, int nq, const SvPlan& plan, hipStream_t stream) {
-    if (plan.passes.empty()) return QS_OK;
+// The plan's tables as one host blob laid out as sv_blob_layout says (the block-factor
+// table at the end is written on the device and is not part of the upload).
+std::vector<char> plan_blob(const SvPlan& plan, int nq, SvBlobLayout* layout) {
     const SvBlobLayout L = sv_blob_layout(plan, nq);
     std::vector<char> host(L.factors, 0);
     auto put = [&](size_t off, const void* src, size_t bytes) {
@@ -108,13 +133,28 @@ int run_plan(double2* state, int nq, const SvPlan& plan, hipStream_t stream) {
     put(L.phases, plan.phases.data(), plan.phases.size() * sizeof(DevPhase));
     put(L.terms, plan.terms.data(), plan.terms.size() * sizeof(DevTerm));
     put(L.tfactors, plan.thread_factors.data(), plan.thread_factors.size() * sizeof(double));
+    *layout = L;
+    return host;
+}
+
+// Uploads the plan's tables into a stream-ordered allocation and launches it.  The host
+// blob lives until the upload has completed (h2d_copy_wait), whatever the runtime does
+// with pageable sources; the launches are queued before that wait, so the device does
+// not idle while the host waits.
+int run_plan(double2* state, int nq, const SvPlan& plan, hipStream_t stream) {
+    if (plan.passes.empty()) return QS_OK;
+    SvBlobLayout L;
+    const std::vector<char> host = plan_blob(plan, nq, &L);
     void* dev = nullptr;
     QS_HIP(hipMallocAsync(&dev, L.total, stream));
-    // Pageable source: the runtime stages the bytes before returning.
-    QS_HIP(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, stream));
-    int st = sv_launch_plan(state, nq, plan, dev, stream);
-    QS_HIP(hipFreeAsync(dev, stream));
-    return st;
+    hipEvent_t copied = nullptr;
+    int st = h2d_copy(dev, host.data(), host.size(), stream, &copied);
+    if (st == QS_OK) st = sv_launch_plan(state, nq, plan, dev, stream);
+    const hipError_t fe = hipFreeAsync(dev, stream);
+    const int wt = copied ? h2d_copy_wait(copied) : QS_OK;
+    if (st) return st;
+    QS_REQUIRE(fe == hipSuccess, QS_E_HIP, "hipFreeAsync failed: %s", hipGetErrorString(fe));
+    return wt;
 }
 
 }  // namespace
@@ -221,6 +261,83 @@ int qs_sv_apply_words(void* state, int nq, const int32_t* words, int num_gates, 
     st = sv_build_plan(nq, gates.data(), (int)gates.size(), &plan);
     if (st) return st;
     return run_plan(static_cast<double2*>(state), nq, plan, as_stream(stream));
+}
+
+}  // extern "C"
+
+// A plan built once on the host and kept on the device (qs_sv_plan_create).
+struct qs_sv_plan {
+    int nq = 0;
+    int device = 0;
+    qs::SvPlan plan;
+    void* dev = nullptr;       // uploaded tables + block-factor table
+    size_t bytes = 0;
+};
+
+extern "C" {
+
+int qs_sv_plan_create(int nq, const qs_gate* gates, int ng, qs_sv_plan** plan_out) {
+    clear_error();
+    QS_REQUIRE(plan_out != nullptr, QS_E_ARG, "plan_out is null");
+    *plan_out = nullptr;
+    QS_REQUIRE(nq >= 1 && nq <= SV_MAX_QUBITS, QS_E_RANGE, "nq=%d outside [1, %d]", nq,
+               SV_MAX_QUBITS);
+    QS_REQUIRE(ng >= 0 && (ng == 0 || gates), QS_E_ARG, "bad gate array");
+    qs_sv_plan* p = new qs_sv_plan();
+    p->nq = nq;
+    int st = sv_build_plan(nq, gates, ng, &p->plan);
+    if (st == QS_OK && !p->plan.passes.empty()) {
+        SvBlobLayout L;
+        const std::vector<char> host = plan_blob(p->plan, nq, &L);
+        hipError_t e = hipGetDevice(&p->device);
+        if (e == hipSuccess) e = hipMalloc(&p->dev, L.total);
+        if (e == hipSuccess) e = hipMemcpy(p->dev, host.data(), host.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_error("plan upload failed: %s", hipGetErrorString(e));
+            st = QS_E_HIP;
+        }
+        p->bytes = L.total;
+    }
+    if (st != QS_OK) {
+        if (p->dev) (void)hipFree(p->dev);
+        delete p;
+        return st;
+    }
+    *plan_out = p;
+    return QS_OK;
+}
+
+int qs_sv_plan_launch(const qs_sv_plan* plan, void* state, void* stream) {
+    clear_error();
+    QS_REQUIRE(plan != nullptr, QS_E_ARG, "plan is null");
+    int st = check_state(state, plan->nq);
+    if (st) return st;
+    if (plan->plan.passes.empty()) return QS_OK;
+    int dev = -1;
+    QS_HIP(hipGetDevice(&dev));
+    QS_REQUIRE(dev == plan->device, QS_E_ARG, "plan lives on device %d, current device is %d",
+               plan->device, dev);
+    return sv_launch_plan(static_cast<double2*>(state), plan->nq, plan->plan, plan->dev,
+                          as_stream(stream));
+}
+
+int qs_sv_plan_info(const qs_sv_plan* plan, int* nq, int* npasses, size_t* device_bytes) {
+    clear_error();
+    QS_REQUIRE(plan != nullptr, QS_E_ARG, "plan is null");
+    if (nq) *nq = plan->nq;
+    if (npasses) *npasses = (int)plan->plan.passes.size();
+    if (device_bytes) *device_bytes = plan->bytes;
+    return QS_OK;
+}
+
+int qs_sv_plan_destroy(qs_sv_plan* plan) {
+    clear_error();
+    if (plan == nullptr) return QS_OK;
+    hipError_t e = hipSuccess;
+    if (plan->dev) e = hipFree(plan->dev);
+    delete plan;
+    QS_REQUIRE(e == hipSuccess, QS_E_HIP, "hipFree failed: %s", hipGetErrorString(e));
+    return QS_OK;
 }
 
 int qs_sv_plan_passes(int nq, const qs_gate* gates, int ng, int* npasses) {

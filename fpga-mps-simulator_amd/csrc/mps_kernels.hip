@@ -1124,7 +1124,10 @@ int plan_batch(const qs_mps_task* tasks, int ntasks, DevTask* out, int32_t* coun
                    kMaxChi);
         QS_REQUIRE(t.site_stride >= (int64_t)2 * t.chi_max * t.chi_max, QS_E_ARG,
                    "task %d: site_stride %lld < 2*chi_max^2", i, (long long)t.site_stride);
-        QS_REQUIRE(t.site >= 0, QS_E_ARG, "task %d: negative site", i);
+        QS_REQUIRE(t.n_sites >= 2, QS_E_ARG, "task %d: n_sites %d < 2", i, t.n_sites);
+        QS_REQUIRE(t.site >= 0 && t.site + 1 < t.n_sites, QS_E_ARG,
+                   "task %d: bond (%d, %d) outside the chain of %d sites", i, t.site,
+                   t.site + 1, t.n_sites);
         QS_REQUIRE(std::isfinite(t.cutoff) && t.cutoff >= 0, QS_E_ARG, "task %d: bad cutoff", i);
         if (t.chi_max > chi) chi = t.chi_max;
     }
@@ -1218,9 +1221,14 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
     hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
     char* ws = static_cast<char*>(workspace);
     DevTask* dtasks = reinterpret_cast<DevTask*>(ws);
-    QS_HIP(hipMemcpyAsync(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask),
-                          hipMemcpyHostToDevice, hs));
-    return launch_batch(dtasks, ntasks, counts, chi, precision, ws, status_dev, hs);
+    // the table is pageable host memory: it stays alive until the copy has read it, and
+    // the launches are queued before that wait
+    hipEvent_t copied = nullptr;
+    st = h2d_copy(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask), hs, &copied);
+    if (st) return st;
+    st = launch_batch(dtasks, ntasks, counts, chi, precision, ws, status_dev, hs);
+    const int wt = h2d_copy_wait(copied);
+    return st ? st : wt;
 }
 
 int qs_mps_time_kernels(int enable) {

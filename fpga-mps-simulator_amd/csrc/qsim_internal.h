@@ -46,6 +46,13 @@ void clear_error();
         }                                                                         \
     } while (0)
 
+// Host-to-device copy of `bytes` from host memory that may be pageable, on `stream`.
+// *done receives an event recorded after the copy (one per host thread, reused by the
+// next call); h2d_copy_wait(*done) returns once the copy has read the source, so the
+// caller keeps the source alive until then and may queue more work first.
+int h2d_copy(void* dst, const void* src, size_t bytes, hipStream_t stream, hipEvent_t* done);
+int h2d_copy_wait(hipEvent_t done);
+
 // ----------------------------------------------------------------- statevector
 
 constexpr int SV_MAX_QUBITS = 34;      // 2^34 * 16 B = 256 GiB, inside 288 GB of HBM

@@ -45,7 +45,8 @@ class QsMpsTask(ctypes.Structure):
     _fields_ = [('sites', ctypes.c_void_p), ('bond', ctypes.c_void_p),
                 ('dlog', ctypes.c_void_p), ('site_stride', ctypes.c_int64),
                 ('site', ctypes.c_int32), ('chi_max', ctypes.c_int32),
-                ('cutoff', ctypes.c_double), ('gate', ctypes.c_double * 32)]
+                ('cutoff', ctypes.c_double), ('n_sites', ctypes.c_int32),
+                ('reserved', ctypes.c_int32), ('gate', ctypes.c_double * 32)]
 
 
 # name -> (restype, argtypes); every function in include/qsim.h
@@ -59,6 +60,13 @@ _PROTOS = {
                                    ctypes.c_int, ctypes.c_void_p]),
     'qs_sv_apply_words': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int32_p,
                                          ctypes.c_int, ctypes.c_void_p]),
+    'qs_sv_plan_create': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(QsGate), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    'qs_sv_plan_launch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    'qs_sv_plan_info': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_size_t)]),
+    'qs_sv_plan_destroy': (ctypes.c_int, [ctypes.c_void_p]),
     'qs_sv_plan_passes': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(QsGate), ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int)]),
     'qs_sv_plan_json': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(QsGate), ctypes.c_int,

@@ -93,6 +93,7 @@ class MPSChain:
             t.dlog = dl.data_ptr() + 8 * j
             t.site_stride = self.stride
             t.site = int(i)
+            t.n_sites = self.n
             t.chi_max = self.chi_max
             t.cutoff = self.cutoff
             gf = np.asarray(g, dtype=np.complex128).reshape(16)

@@ -27,7 +27,8 @@ from .mps import MPSChain, PRECISIONS
 
 TASK_DTYPE = np.dtype([('sites', '<u8'), ('bond', '<u8'), ('dlog', '<u8'),
                        ('site_stride', '<i8'), ('site', '<i4'), ('chi_max', '<i4'),
-                       ('cutoff', '<f8'), ('gate', '<f8', (32,))])
+                       ('cutoff', '<f8'), ('n_sites', '<i4'), ('reserved', '<i4'),
+                       ('gate', '<f8', (32,))])
 assert TASK_DTYPE.itemsize == ctypes.sizeof(_lib.QsMpsTask)
 
 
@@ -103,6 +104,7 @@ class MultiRun:
             t['dlog'] = dbase + 8 * (self._next + np.arange(k, dtype=np.uint64))
             t['site_stride'] = ch.stride
             t['site'] = sites
+            t['n_sites'] = s.n
             t['chi_max'] = s.chi_max
             t['cutoff'] = self.cutoff
             gf = g.reshape(k, 16)

@@ -41,6 +41,41 @@ def gates_to_records(gates: Iterable[tuple], nq: int) -> Tuple[list, complex]:
     return recs, phase
 
 
+class SvPlan:
+    """A fused-pass plan built once on the host and kept in device memory
+    (qs_sv_plan_create); StateVector.launch applies it with no host planning or upload."""
+
+    def __init__(self, nq: int, records: Sequence[tuple], device: int = 0):
+        _lib.ensure_device(device)
+        arr = records_to_array(records)
+        h = ctypes.c_void_p()
+        _lib.call('qs_sv_plan_create', nq, arr, len(records), ctypes.byref(h))
+        self._h = h
+        self.nq = nq
+        self.device = device
+        n, p, b = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_size_t(0)
+        _lib.call('qs_sv_plan_info', h, ctypes.byref(n), ctypes.byref(p), ctypes.byref(b))
+        self.npasses = p.value
+        self.device_bytes = b.value
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise RuntimeError('plan already destroyed')
+        return self._h
+
+    def destroy(self) -> None:
+        if getattr(self, '_h', None) is not None:
+            _lib.call('qs_sv_plan_destroy', self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class StateVector:
     """A 2^nq complex128 state on one GPU."""
 
@@ -84,6 +119,16 @@ class StateVector:
             return
         arr = records_to_array(records)
         _lib.call('qs_sv_apply', self.ptr, self.nq, arr, len(records), self._stream())
+
+    def plan(self, records: Sequence[tuple]) -> SvPlan:
+        """Plan `records` once for this state's qubit count (see SvPlan)."""
+        return SvPlan(self.nq, records, self.device)
+
+    def launch(self, plan: SvPlan) -> None:
+        """Apply a prebuilt plan on the current stream (no planning, no upload)."""
+        if plan.nq != self.nq:
+            raise ValueError(f'plan is for {plan.nq} qubits, state has {self.nq}')
+        _lib.call('qs_sv_plan_launch', plan.handle, self.ptr, self._stream())
 
     def apply(self, gates: Iterable[tuple]) -> complex:
         """Apply logical gates; returns the reference-lowering global phase."""

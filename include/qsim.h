@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define QS_ABI_VERSION 1
+#define QS_ABI_VERSION 2
 
 #define QS_OK         0
 #define QS_E_ARG     -1   /* bad argument (null pointer, qubit out of range, ...) */
@@ -100,6 +100,23 @@ int qs_sv_apply(void* state, int nq, const qs_gate* gates, int ng, void* stream)
  * kernel, num_gates is not clamped to 512: any count is applied. */
 int qs_sv_apply_words(void* state, int nq, const int32_t* gate_sequence, int num_gates,
                       void* stream);
+
+/* Plan once, launch many (no counterpart in the reference, whose driver re-packs and
+ * re-uploads the gate words for every run, fpga_simulator_v07.py:456-500).
+ * qs_sv_plan_create builds the fused-pass plan of `gates` for an nq-qubit state (the
+ * plan qs_sv_apply would build) and uploads it, synchronously, into device memory of the
+ * current device owned by the plan.  qs_sv_plan_launch applies the planned gates to
+ * `state` on `stream`: it plans nothing, copies nothing from the host and allocates
+ * nothing, so it can be recorded into a HIP graph; its result equals qs_sv_apply's bit for
+ * bit.  A plan keeps a block-factor table that its launches rewrite, so one plan must not
+ * be launched on two streams at once.  qs_sv_plan_info reports the qubit count, the
+ * passes and the device bytes the plan holds.  qs_sv_plan_destroy frees it (hipFree: it
+ * waits for the device, so launches in flight finish first); NULL is accepted. */
+typedef struct qs_sv_plan qs_sv_plan;
+int qs_sv_plan_create(int nq, const qs_gate* gates, int ng, qs_sv_plan** plan_out);
+int qs_sv_plan_launch(const qs_sv_plan* plan, void* state, void* stream);
+int qs_sv_plan_info(const qs_sv_plan* plan, int* nq, int* npasses, size_t* device_bytes);
+int qs_sv_plan_destroy(qs_sv_plan* plan);
 
 /* Number of fused passes qs_sv_apply would launch for this gate list (host only). */
 int qs_sv_plan_passes(int nq, const qs_gate* gates, int ng, int* npasses);
@@ -188,9 +205,11 @@ typedef struct qs_mps_task {
     int32_t* bond;         /* device, int32[n+1]                               */
     double*  dlog;         /* device, receives the norm_log increment           */
     int64_t  site_stride;  /* complex elements per slot, >= chi_max*2*chi_max  */
-    int32_t  site;         /* left site i                                      */
+    int32_t  site;         /* left site i, 0 <= i < n_sites - 1                */
     int32_t  chi_max;
     double   cutoff;
+    int32_t  n_sites;      /* n, the chain's site count (ABI version 2)        */
+    int32_t  reserved;     /* 0                                                */
     double   gate[32];     /* 4x4, row index = 2*sigma_i + sigma_{i+1}         */
 } qs_mps_task;
 

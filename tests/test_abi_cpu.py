@@ -31,7 +31,7 @@ def test_library_exports_every_declared_function():
 
 def test_abi_version_and_limits():
     lib = _lib.load_library()
-    assert lib.qs_abi_version() == 1
+    assert lib.qs_abi_version() == 2
     assert lib.qs_sv_max_qubits() == 34
     assert lib.qs_mps_max_chi() == 128
     ntasks, chi = 5, 64
@@ -43,7 +43,7 @@ def test_abi_version_and_limits():
     assert ntasks * 8 * 128 * 128 * 16 < ws128 < ntasks * 8 * 128 * 128 * 16 + ntasks * 1024 + 256
     assert lib.qs_mps_workspace_bytes(0, chi) == 0
     assert ctypes.sizeof(_lib.QsGate) == 16 + 8 + 256
-    assert ctypes.sizeof(_lib.QsMpsTask) == 8 * 3 + 8 + 4 + 4 + 8 + 256
+    assert ctypes.sizeof(_lib.QsMpsTask) == 8 * 3 + 8 + 4 + 4 + 8 + 4 + 4 + 256
 
 
 def test_svd_team_width_table():
@@ -80,6 +80,7 @@ def test_batch_plan_groups_by_svd_class_without_device():
         t.sites, t.bond, t.dlog = 0x10000, 0x20000, 0x30000 + 8 * k
         t.site_stride = 2 * 128 * 128
         t.site = 3 * k
+        t.n_sites = 3 * len(chis)
         t.chi_max = c
         t.cutoff = 1e-10
     nb = lib.qs_mps_batch_table_bytes(len(chis))
@@ -101,6 +102,15 @@ def test_batch_plan_groups_by_svd_class_without_device():
         assert buf[k * rec:k * rec + size].tobytes() == bytes(tasks[o])
     assert lib.qs_mps_batch_plan(tasks, len(chis), ctypes.c_void_p(buf.ctypes.data), nb - 1,
                                  counts, ctypes.byref(chi)) == _lib.QS_E_RANGE
+    tasks[6].site = tasks[6].n_sites - 1                    # bond (n-1, n) past the chain
+    assert lib.qs_mps_batch_plan(tasks, len(chis), ctypes.c_void_p(buf.ctypes.data), nb, counts,
+                                 ctypes.byref(chi)) == _lib.QS_E_ARG
+    assert b'outside the chain' in lib.qs_last_error()
+    tasks[6].site = 18
+    tasks[6].n_sites = 0                                    # chain length missing
+    assert lib.qs_mps_batch_plan(tasks, len(chis), ctypes.c_void_p(buf.ctypes.data), nb, counts,
+                                 ctypes.byref(chi)) == _lib.QS_E_ARG
+    tasks[6].n_sites = 3 * len(chis)
     tasks[1].site = tasks[0].site + 1                       # overlapping sites
     assert lib.qs_mps_batch_plan(tasks, len(chis), ctypes.c_void_p(buf.ctypes.data), nb, counts,
                                  ctypes.byref(chi)) == _lib.QS_E_ARG
@@ -137,6 +147,14 @@ def test_argument_errors_are_reported_without_device():
     assert lib.qs_last_error() == b''
     # null state pointer
     assert lib.qs_sv_set_zero(None, 3, None) == _lib.QS_E_ARG
+    # plan objects: argument errors before any device work
+    h = ctypes.c_void_p()
+    assert lib.qs_sv_plan_create(40, g, 1, ctypes.byref(h)) == _lib.QS_E_RANGE and not h.value
+    assert lib.qs_sv_plan_create(3, None, 1, ctypes.byref(h)) == _lib.QS_E_ARG
+    assert lib.qs_sv_plan_create(3, g, 1, None) == _lib.QS_E_ARG
+    assert lib.qs_sv_plan_launch(None, ctypes.c_void_p(16), None) == _lib.QS_E_ARG
+    assert lib.qs_sv_plan_info(None, None, None, None) == _lib.QS_E_ARG
+    assert lib.qs_sv_plan_destroy(None) == _lib.QS_OK
     # measurement shots: workspace size, null state, negative shot count
     assert lib.qs_sv_sample_workspace_bytes(12) == 2 * 8
     assert lib.qs_sv_sample_workspace_bytes(28) == ((1 << 16) + 1) * 8
@@ -149,6 +167,7 @@ def test_argument_errors_are_reported_without_device():
     t[0].sites = 1
     t[0].bond = 1
     t[0].dlog = 1
+    t[0].n_sites = 4
     t[0].chi_max = 129
     t[0].site_stride = 2 * 129 * 129
     assert lib.qs_mps_update_batch(t, 1, 0, ctypes.c_void_p(1), 1 << 30, None, None) == _lib.QS_E_CHI
@@ -159,6 +178,7 @@ def test_argument_errors_are_reported_without_device():
     for k in range(2):
         t2[k].sites, t2[k].bond, t2[k].dlog = 1, 1, 1
         t2[k].chi_max, t2[k].site_stride, t2[k].site = 8, 128, 3 + k
+        t2[k].n_sites = 8
     assert lib.qs_mps_update_batch(t2, 2, 0, ctypes.c_void_p(1), 1 << 30, None, None) == _lib.QS_E_ARG
     assert b'share a site' in lib.qs_last_error()
 

@@ -359,3 +359,85 @@ def test_28q_properties():
     sv.apply(g)
     assert torch.equal(sv.tensor, snap)          # bitwise deterministic
     del snap
+
+
+# ------------------------------------------------------- plan once, launch many
+def test_plan_launch_equals_apply_bitwise_and_replays_in_a_graph():
+    """qs_sv_plan_create + qs_sv_plan_launch give qs_sv_apply's result bit for bit, on
+    every launch of the same plan, eagerly and replayed from a HIP graph (three replays,
+    eager work and synchronisations between them)."""
+    import torch
+    from qsim_amd.sv import StateVector, gates_to_records
+    nq = 16
+    recs, _ = gates_to_records(workloads.u3_cz_layers(nq, 6, seed=3) + workloads.qft_gates(nq),
+                               nq)
+    ref = StateVector(nq, 0)
+    got = StateVector(nq, 0)
+    plan = got.plan(recs)
+    assert plan.npasses >= 2 and plan.device_bytes > 0
+    for _ in range(2):
+        ref.apply_records(recs)
+        got.launch(plan)
+        torch.cuda.synchronize()
+        assert torch.equal(ref.tensor, got.tensor)
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        got.launch(plan)                 # warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(s)
+    ref.apply_records(recs)
+    torch.cuda.synchronize()
+    assert torch.equal(ref.tensor, got.tensor)
+    with torch.cuda.graph(g):
+        got.launch(plan)
+    for _ in range(3):
+        ref.apply_records(recs)
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(ref.tensor, got.tensor)
+    plan.destroy()
+
+
+def test_28q_depth20_benchmark_circuit_inverse_roundtrip():
+    """The full 28-qubit benchmark circuit (20 x [U3 + CZ ladder] + QFT-28, 1520 gates)
+    through a prebuilt plan, then its inverse: back to |0..0> and unit norm."""
+    import torch
+    from qsim_amd.sv import StateVector, gates_to_records
+    nq = 28
+    g = workloads.sv_benchmark_circuit(nq, 20, 1234)
+    recs, _ = gates_to_records(g, nq)
+    inv, _ = gates_to_records(workloads.inverse_circuit(g), nq)
+    sv = StateVector(nq, 0)
+    fwd = sv.plan(recs)
+    sv.launch(fwd)
+    assert abs(sv.norm_sq() - 1.0) < 1e-10
+    sv.apply_records(inv)
+    torch.cuda.synchronize()
+    assert abs(abs(sv.tensor[0].item()) - 1.0) < 1e-9
+    assert abs(sv.norm_sq() - 1.0) < 1e-10
+    fwd.destroy()
+
+
+def test_32q_indices_beyond_2_pow_31():
+    """A 64 GiB state (32 qubits): amplitude indices and byte offsets past 2^31 land where
+    the basis-state index says, and a layered circuit followed by its inverse returns to
+    |0..0>."""
+    import torch
+    from qsim_amd.sv import StateVector
+    nq = 32
+    sv = StateVector(nq, 0)
+    sv.apply([('h', [31], None), ('cx', [31, 0], None), ('x', [30], None)])
+    r = 1 / math.sqrt(2)
+    i0, i1 = 1 << 30, (1 << 31) | (1 << 30) | 1
+    assert abs(sv.tensor[i0].item() - r) < 1e-15
+    assert abs(sv.tensor[i1].item() - r) < 1e-15
+    assert abs(sv.norm_sq() - 1.0) < 1e-12
+    sv.set_zero()
+    g = workloads.u3_cz_layers(nq, 2, seed=13)
+    sv.apply(g)
+    sv.apply(workloads.inverse_circuit(g))
+    torch.cuda.synchronize()
+    assert abs(abs(sv.tensor[0].item()) - 1.0) < 1e-10
+    del sv
+    torch.cuda.empty_cache()
