@@ -184,7 +184,7 @@ int qs_init(int device) {
     QS_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, QS_E_NODEV,
                "device %d is %s; this library is built for gfx950 only", device,
                prop.gcnArchName);
-    return QS_OK;
+    return mps_scratch_setup(device);
 }
 
 int qs_sv_set_zero(void* state, int nq, void* stream) {

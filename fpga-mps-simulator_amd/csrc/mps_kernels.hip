@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstring>
 #include <atomic>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -973,6 +974,60 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
     }
 }
 
+// Device scratch of qs_mps_update_batch_static: one batch's task table and theta slots
+// (the workspace the other entry points take as an argument).  Each device the library
+// runs on holds its own copy until the process ends.  In host code `scratch` is only a
+// symbol: its device address on the current device comes from hipGetSymbolAddress and is
+// kept per device in g_scratch_ptr.
+static __device__ __align__(256) char scratch[QS_MPS_STATIC_SCRATCH_BYTES];
+
+namespace qs {
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};
+}  // namespace
+
+int mps_scratch_setup(int device) {
+    QS_REQUIRE(device >= 0 && device < kScratchDevices, QS_E_NODEV,
+               "device %d beyond the %d devices the scratch table holds", device,
+               kScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return QS_OK;
+    int cur = -1;
+    QS_HIP(hipGetDevice(&cur));
+    QS_REQUIRE(cur == device, QS_E_ARG, "scratch set-up for device %d while device %d is current",
+               device, cur);
+    void* ptr = nullptr;
+    QS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    QS_REQUIRE(ptr != nullptr, QS_E_HIP, "no device address for the scratch array");
+    g_scratch_ptr[device] = ptr;
+    return QS_OK;
+}
+
+namespace {
+// Device address of the current device's scratch array (set up on first use).
+int scratch_ptr(char** out) {
+    int dev = -1;
+    QS_HIP(hipGetDevice(&dev));
+    QS_REQUIRE(dev >= 0 && dev < kScratchDevices, QS_E_NODEV, "device %d out of range", dev);
+    void* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        p = g_scratch_ptr[dev];
+    }
+    if (!p) {
+        const int st = mps_scratch_setup(dev);
+        if (st) return st;
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        p = g_scratch_ptr[dev];
+    }
+    *out = static_cast<char*>(p);
+    return QS_OK;
+}
+}  // namespace
+}  // namespace qs
+
 extern "C" {
 
 int qs_mps_max_chi(void) { return kMaxChi; }
@@ -1223,6 +1278,38 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
     DevTask* dtasks = reinterpret_cast<DevTask*>(ws);
     // the table is pageable host memory: it stays alive until the copy has read it, and
     // the launches are queued before that wait
+    hipEvent_t copied = nullptr;
+    st = h2d_copy(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask), hs, &copied);
+    if (st) return st;
+    st = launch_batch(dtasks, ntasks, counts, chi, precision, ws, status_dev, hs);
+    const int wt = h2d_copy_wait(copied);
+    return st ? st : wt;
+}
+
+int qs_mps_update_batch_static(const qs_mps_task* tasks, int ntasks, int precision,
+                               int32_t* status_dev, void* stream) {
+    clear_error();
+    QS_REQUIRE(ntasks >= 0, QS_E_ARG, "negative task count");
+    QS_REQUIRE(precision == QS_PREC_FP64 || precision == QS_PREC_FP32_MFMA, QS_E_ARG,
+               "unknown precision %d", precision);
+    if (ntasks == 0) return QS_OK;
+    // the host table is reused by the calls of one thread (no allocation once it has grown)
+    thread_local std::vector<DevTask> dt;
+    if (dt.size() < (size_t)ntasks) dt.resize(ntasks);
+    int32_t counts[5];
+    int chi = 1;
+    int st = plan_batch(tasks, ntasks, dt.data(), counts, &chi);
+    if (st) return st;
+    const size_t need = qs_mps_workspace_bytes(ntasks, chi);
+    QS_REQUIRE(need <= (size_t)QS_MPS_STATIC_SCRATCH_BYTES, QS_E_RANGE,
+               "batch of %d tasks at chi_max %d needs %zu bytes of scratch; the static scratch "
+               "holds %zu (use qs_mps_update_batch with a workspace)", ntasks, chi, need,
+               (size_t)QS_MPS_STATIC_SCRATCH_BYTES);
+    char* ws = nullptr;
+    st = scratch_ptr(&ws);
+    if (st) return st;
+    hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
+    DevTask* dtasks = reinterpret_cast<DevTask*>(ws);
     hipEvent_t copied = nullptr;
     st = h2d_copy(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask), hs, &copied);
     if (st) return st;

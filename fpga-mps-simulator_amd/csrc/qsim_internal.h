@@ -53,6 +53,10 @@ void clear_error();
 int h2d_copy(void* dst, const void* src, size_t bytes, hipStream_t stream, hipEvent_t* done);
 int h2d_copy_wait(hipEvent_t done);
 
+// Resolves the device address of the MPS static scratch array on `device`, which must be
+// the current device (qs_init calls it); later calls for the same device do nothing.
+int mps_scratch_setup(int device);
+
 // ----------------------------------------------------------------- statevector
 
 constexpr int SV_MAX_QUBITS = 34;      // 2^34 * 16 B = 256 GiB, inside 288 GB of HBM

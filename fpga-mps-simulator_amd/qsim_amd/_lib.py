@@ -31,6 +31,8 @@ QS_GATE_DIAG2 = 3
 QS_PREC_FP64 = 0
 QS_PREC_FP32_MFMA = 1
 
+QS_MPS_STATIC_SCRATCH_BYTES = 32 << 20
+
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int32_p = ctypes.POINTER(ctypes.c_int32)
 
@@ -127,6 +129,9 @@ _PROTOS = {
     'qs_mps_update_batch': (ctypes.c_int, [ctypes.POINTER(QsMpsTask), ctypes.c_int,
                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
                                            ctypes.c_void_p, ctypes.c_void_p]),
+    'qs_mps_update_batch_static': (ctypes.c_int, [ctypes.POINTER(QsMpsTask), ctypes.c_int,
+                                                  ctypes.c_int, ctypes.c_void_p,
+                                                  ctypes.c_void_p]),
     'qs_mps_apply_1q': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                        ctypes.c_int, _c_double_p, ctypes.c_void_p]),
 }

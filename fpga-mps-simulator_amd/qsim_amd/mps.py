@@ -72,10 +72,22 @@ class MPSChain:
             self._status = torch.zeros(ntasks, dtype=torch.int32, device=self.sites.device)
         return self._ws, self._status
 
+    def _status_buffer(self, ntasks: int):
+        import torch
+        if self._status is None or self._status.numel() < ntasks:
+            self._status = torch.zeros(ntasks, dtype=torch.int32, device=self.sites.device)
+            self._ws = None
+            self._ws_tasks = 0
+        return self._status
+
     # -- gates ---------------------------------------------------------------
     def apply_2q_batch(self, updates: Sequence[Tuple[int, np.ndarray]],
-                       precision: Optional[str] = None) -> None:
-        """Two-site updates on bonds (i, i+1), all disjoint, in one launch sequence."""
+                       precision: Optional[str] = None, allocation_free: bool = False) -> None:
+        """Two-site updates on bonds (i, i+1), all disjoint, in one launch sequence.
+
+        allocation_free: run through qs_mps_update_batch_static, whose workspace is the
+        library's static device scratch (no workspace allocated here; batches beyond its
+        size are refused)."""
         import torch
         k = len(updates)
         if k == 0:
@@ -101,9 +113,14 @@ class MPSChain:
             vals[0::2] = gf.real
             vals[1::2] = gf.imag
             t.gate[:] = vals.tolist()
-        ws, status = self._workspace(k)
-        _lib.call('qs_mps_update_batch', tasks, k, prec, ctypes.c_void_p(ws.data_ptr()),
-                  ws.numel(), ctypes.c_void_p(status.data_ptr()), self._stream())
+        if allocation_free:
+            status = self._status_buffer(k)
+            _lib.call('qs_mps_update_batch_static', tasks, k, prec,
+                      ctypes.c_void_p(status.data_ptr()), self._stream())
+        else:
+            ws, status = self._workspace(k)
+            _lib.call('qs_mps_update_batch', tasks, k, prec, ctypes.c_void_p(ws.data_ptr()),
+                      ws.numel(), ctypes.c_void_p(status.data_ptr()), self._stream())
         self._dlogs.append(dl)
         self._last_batch = k
         # running minimum of the statuses (negative = failure), kept on the device

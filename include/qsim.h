@@ -228,6 +228,21 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision,
                         void* workspace, size_t ws_bytes, int32_t* status_dev,
                         void* stream);
 
+/* Allocation-free form of qs_mps_update_batch.  It has no counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The workspace is a static
+ * device array of QS_MPS_STATIC_SCRATCH_BYTES (32 MiB) that the library holds on every
+ * device it runs on until the process ends; the call allocates nothing (no hipMalloc,
+ * hipFree or allocating library routine) and refuses, with QS_E_RANGE, a batch whose
+ * qs_mps_workspace_bytes(ntasks, max chi_max) exceeds that size (e.g. more than 127 tasks
+ * at chi_max 64).  It runs on the current device; tasks, statuses and results are as for
+ * qs_mps_update_batch, bit for bit.  Every call on one device shares that device's
+ * array: calls that overlap in time on one device (two host threads, or two streams
+ * whose work overlaps) race and must be serialised by the caller. */
+#define QS_MPS_STATIC_SCRATCH_BYTES (32u << 20)
+int qs_mps_update_batch_static(const qs_mps_task* tasks, int ntasks, int precision,
+                               int32_t* status_dev, void* stream);
+
 /* Graph-capturable form of qs_mps_update_batch, in two steps (no counterpart in the
  * reference, whose host loop launches one update at a time and waits for it,
  * fpga_mps_simulator.py:566-634).

@@ -181,6 +181,19 @@ def test_argument_errors_are_reported_without_device():
         t2[k].n_sites = 8
     assert lib.qs_mps_update_batch(t2, 2, 0, ctypes.c_void_p(1), 1 << 30, None, None) == _lib.QS_E_ARG
     assert b'share a site' in lib.qs_last_error()
+    # the allocation-free batch: argument checks and the scratch limit come first
+    assert lib.qs_mps_update_batch_static(t, 1, 7, None, None) == _lib.QS_E_ARG
+    assert lib.qs_mps_update_batch_static(t2, 2, 0, None, None) == _lib.QS_E_ARG
+    n = 130
+    big = (_lib.QsMpsTask * n)()
+    for k in range(n):
+        big[k].sites, big[k].bond, big[k].dlog = 1, 1, 1
+        big[k].chi_max, big[k].site_stride, big[k].site = 64, 2 * 64 * 64, 2 * k
+        big[k].n_sites = 2 * n + 1
+    assert lib.qs_mps_update_batch_static(big, n, 1, None, None) == _lib.QS_E_RANGE
+    assert b'static scratch' in lib.qs_last_error()
+    need = lib.qs_mps_workspace_bytes(128, 64)
+    assert need > _lib.QS_MPS_STATIC_SCRATCH_BYTES >= lib.qs_mps_workspace_bytes(127, 64)
 
 
 def test_engines_fail_loudly_without_gpu():
