@@ -651,6 +651,71 @@ __global__ void mps_apply_1q_kernel(cd2* __restrict__ sites, const int32_t* __re
     }
 }
 
+// Many one-site gates, one workgroup per gate; the gates of one launch act on distinct
+// sites (the caller composes gates that share a site), so workgroups never overlap.
+struct OneSiteGate {
+    int32_t site, pad[3];
+    double m[8];
+};
+constexpr int kMax1qPerLaunch = 32;
+struct OneSiteBatch {
+    OneSiteGate g[kMax1qPerLaunch];
+};
+
+__global__ void __launch_bounds__(256)
+mps_apply_1q_batch_kernel(cd2* __restrict__ sites, const int32_t* __restrict__ bond,
+                          int64_t stride, OneSiteBatch batch) {
+    const OneSiteGate& G = batch.g[blockIdx.x];
+    const int site = G.site;
+    const int cl = bond[site], cr = bond[site + 1];
+    const cd2 m00 = mk2(G.m[0], G.m[1]), m01 = mk2(G.m[2], G.m[3]);
+    const cd2 m10 = mk2(G.m[4], G.m[5]), m11 = mk2(G.m[6], G.m[7]);
+    cd2* A = sites + (size_t)site * stride;
+    for (int idx = threadIdx.x; idx < cl * cr; idx += blockDim.x) {
+        const int a = idx / cr, b = idx - a * cr;
+        cd2* p0 = A + (size_t)(2 * a) * cr + b;
+        cd2* p1 = p0 + cr;
+        const cd2 x0 = *p0, x1 = *p1;
+        *p0 = cmadd(cmul(m00, x0), m01, x1);
+        *p1 = cmadd(cmul(m10, x0), m11, x1);
+    }
+}
+
+// Reset of one site (fpga_mps_simulator.py:530-552): zero the sigma = 1 half, divide the
+// site by its norm when that exceeds 1e-15, otherwise make it the |0> tensor t[0,0,0] = 1.
+// One workgroup; the squared norm is summed in index order per thread and folded by a
+// fixed tree.
+__global__ void __launch_bounds__(256)
+mps_reset_site_kernel(cd2* __restrict__ sites, const int32_t* __restrict__ bond,
+                      int64_t stride, int site) {
+    __shared__ double red[256];
+    const int cl = bond[site], cr = bond[site + 1];
+    cd2* A = sites + (size_t)site * stride;
+    const int t = threadIdx.x;
+    double part = 0.0;
+    for (int idx = t; idx < cl * cr; idx += 256) {
+        const int a = idx / cr, b = idx - a * cr;
+        const cd2 x = A[(size_t)(2 * a) * cr + b];
+        part = fma(x.x, x.x, fma(x.y, x.y, part));
+        A[(size_t)(2 * a + 1) * cr + b] = mk2(0.0, 0.0);
+    }
+    red[t] = part;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    const double nrm = sqrt(red[0]);
+    const bool keep = nrm > 1e-15;
+    const double inv = keep ? 1.0 / nrm : 0.0;
+    for (int idx = t; idx < cl * cr; idx += 256) {
+        const int a = idx / cr, b = idx - a * cr;
+        cd2* p = A + (size_t)(2 * a) * cr + b;
+        const cd2 x = *p;
+        *p = keep ? mk2(x.x * inv, x.y * inv) : mk2(idx == 0 ? 1.0 : 0.0, 0.0);
+    }
+}
+
 }  // namespace
 }  // namespace qs
 
@@ -1392,6 +1457,54 @@ int qs_mps_batch_launch(const void* table_dev, int ntasks, const int32_t* counts
     return launch_batch(static_cast<const DevTask*>(table_dev), ntasks, counts, chi_max,
                         precision, static_cast<char*>(workspace), status_dev,
                         reinterpret_cast<hipStream_t>(stream));
+}
+
+int qs_mps_apply_1q_batch(double* sites, const int32_t* bond, int64_t site_stride, int n,
+                          const int32_t* site_list, const double* mats, int count,
+                          void* stream) {
+    clear_error();
+    QS_REQUIRE(sites && bond, QS_E_ARG, "null sites or bond array");
+    QS_REQUIRE(count >= 0 && (count == 0 || (site_list && mats)), QS_E_ARG,
+               "bad gate list (count %d)", count);
+    QS_REQUIRE(n >= 1 && site_stride > 0, QS_E_ARG, "bad chain (n %d, stride %lld)", n,
+               (long long)site_stride);
+    {
+        std::vector<int32_t> sorted(site_list, site_list + count);
+        std::sort(sorted.begin(), sorted.end());
+        for (int i = 0; i < count; ++i) {
+            QS_REQUIRE(sorted[i] >= 0 && sorted[i] < n, QS_E_ARG, "site %d outside the chain of %d",
+                       sorted[i], n);
+            QS_REQUIRE(i == 0 || sorted[i] != sorted[i - 1], QS_E_ARG,
+                       "site %d appears twice (compose gates of one site first)", sorted[i]);
+        }
+    }
+    hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
+    for (int g0 = 0; g0 < count; g0 += kMax1qPerLaunch) {
+        const int k = std::min(kMax1qPerLaunch, count - g0);
+        OneSiteBatch b;
+        std::memset(&b, 0, sizeof(b));
+        for (int j = 0; j < k; ++j) {
+            b.g[j].site = site_list[g0 + j];
+            std::memcpy(b.g[j].m, mats + 8 * (size_t)(g0 + j), 8 * sizeof(double));
+        }
+        hipLaunchKernelGGL(mps_apply_1q_batch_kernel, dim3(k), dim3(256), 0, hs,
+                           reinterpret_cast<cd2*>(sites), bond, site_stride, b);
+        QS_HIP(hipGetLastError());
+    }
+    return QS_OK;
+}
+
+int qs_mps_reset_site(double* sites, const int32_t* bond, int64_t site_stride, int n, int site,
+                      void* stream) {
+    clear_error();
+    QS_REQUIRE(sites && bond, QS_E_ARG, "null sites or bond array");
+    QS_REQUIRE(site >= 0 && site < n && site_stride > 0, QS_E_ARG,
+               "site %d outside the chain of %d", site, n);
+    hipLaunchKernelGGL(mps_reset_site_kernel, dim3(1), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), reinterpret_cast<cd2*>(sites), bond,
+                       site_stride, site);
+    QS_HIP(hipGetLastError());
+    return QS_OK;
 }
 
 int qs_mps_apply_1q(double* sites, const int32_t* bond, int64_t site_stride, int site,

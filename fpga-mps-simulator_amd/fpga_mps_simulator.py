@@ -169,34 +169,26 @@ def compile_circuit(circuit, n_qubits: int,
     return out, sorted(measured)
 
 
-def _host_reset(t: np.ndarray) -> np.ndarray:
-    """Reset of one site tensor (fpga_mps_simulator.py:530-552)."""
-    t = t.copy()
-    t[:, 1, :] = 0.0
-    nm = np.linalg.norm(t)
-    if nm > 1e-15:
-        return t / nm
-    t = np.zeros_like(t)
-    t[0, 0, 0] = 1.0
-    return t
-
-
 def run_gate_list(gate_list, n: int, chi_max: int, svd_cutoff: float, precision: str = 'fp64',
                   device: int = 0, phase_times: Optional[Dict[str, float]] = None) -> MPSChain:
     """Execute a compiled gate list on the GPU (the gate loop of _run_circuit_fpga).
 
-    Two-site gates accumulate into a batch while they touch pairwise-disjoint
-    bonds; a gate that touches a batched site flushes the batch first, so the
-    order of non-commuting gates is the order of the list.
+    Two-site gates accumulate into a batch while they touch pairwise-disjoint bonds; a
+    gate that touches a batched site flushes the batch first, so the order of
+    non-commuting gates is the order of the list.  One-site gates accumulate per site
+    (gates on one site multiplied together in order) and go out in one launch sequence
+    when a two-site gate or a reset needs their site, as the reference flushes its
+    one-qubit instruction buffer (fpga_mps_simulator.py:498-526); resets run on the device
+    (:530-552).
     """
     import torch
     chain = MPSChain(n, chi_max, svd_cutoff, device, precision)
     batch: List[Tuple[int, np.ndarray]] = []
     busy = set()
+    pend: Dict[int, np.ndarray] = {}
     t_kernel = 0.0
-    t_xfer = 0.0
 
-    def flush():
+    def flush2():
         nonlocal batch, busy, t_kernel
         if batch:
             t0 = time.perf_counter()
@@ -204,6 +196,14 @@ def run_gate_list(gate_list, n: int, chi_max: int, svd_cutoff: float, precision:
             t_kernel += time.perf_counter() - t0
         batch = []
         busy = set()
+
+    def flush1():
+        nonlocal pend, t_kernel
+        if pend:
+            t0 = time.perf_counter()
+            chain.apply_1q_batch(list(pend.items()))
+            t_kernel += time.perf_counter() - t0
+        pend = {}
 
     for g in gate_list:
         typ = g['type']
@@ -213,30 +213,32 @@ def run_gate_list(gate_list, n: int, chi_max: int, svd_cutoff: float, precision:
             if b != a + 1:
                 raise AssertionError(f"Non-adjacent 2q gate after routing: sites {a},{b}")
             if a in busy or b in busy:
-                flush()
+                flush2()
+            if a in pend or b in pend:
+                flush1()
             batch.append((a, g['gate_mat']))
             busy.update((a, b))
         elif typ == '1q':
             if s in busy:
-                flush()
-            t0 = time.perf_counter()
-            chain.apply_1q(s, g['gate_mat'])
-            t_kernel += time.perf_counter() - t0
+                flush2()
+            u = np.asarray(g['gate_mat'], dtype=np.complex128)
+            pend[s] = u @ pend[s] if s in pend else u
         elif typ == 'reset':
             if s in busy:
-                flush()
+                flush2()
+            if s in pend:
+                flush1()
             t0 = time.perf_counter()
-            host = chain.site_tensors()[s]
-            chain.set_site(s, _host_reset(host))
-            t_xfer += time.perf_counter() - t0
-    flush()
+            chain.reset_site(s)
+            t_kernel += time.perf_counter() - t0
+    flush1()
+    flush2()
     t0 = time.perf_counter()
     torch.cuda.synchronize(device)
     t_kernel += time.perf_counter() - t0
     chain.check_status()
     if phase_times is not None:
         phase_times['fpga_kernel'] += t_kernel
-        phase_times['hbm_transfer'] += t_xfer
     return chain
 
 

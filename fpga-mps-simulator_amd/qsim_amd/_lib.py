@@ -105,6 +105,20 @@ _PROTOS = {
     'qs_mps_right_envs': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_size_t, ctypes.c_void_p]),
+    'qs_mps_left_envs': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_size_t, ctypes.c_void_p]),
+    'qs_mps_expect_work_bytes': (ctypes.c_size_t, [ctypes.c_int]),
+    'qs_mps_expect': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_void_p, _c_double_p, _c_double_p, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                     ctypes.c_void_p]),
+    'qs_mps_overlap_work_bytes': (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
+    'qs_mps_overlap': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_size_t, _c_double_p, ctypes.c_void_p]),
     'qs_mps_statevector': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
@@ -132,6 +146,11 @@ _PROTOS = {
     'qs_mps_update_batch_static': (ctypes.c_int, [ctypes.POINTER(QsMpsTask), ctypes.c_int,
                                                   ctypes.c_int, ctypes.c_void_p,
                                                   ctypes.c_void_p]),
+    'qs_mps_apply_1q_batch': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                             ctypes.c_int, _c_int32_p, _c_double_p,
+                                             ctypes.c_int, ctypes.c_void_p]),
+    'qs_mps_reset_site': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     'qs_mps_apply_1q': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                        ctypes.c_int, _c_double_p, ctypes.c_void_p]),
 }

@@ -140,6 +140,29 @@ class MPSChain:
                   ctypes.c_void_p(self.bond.data_ptr()), self.stride, int(site), arr,
                   self._stream())
 
+    def apply_1q_batch(self, gates: Sequence[Tuple[int, np.ndarray]]) -> None:
+        """One-site gates on distinct sites, 32 per launch (qs_mps_apply_1q_batch)."""
+        k = len(gates)
+        if k == 0:
+            return
+        sites = (ctypes.c_int32 * k)(*[int(s) for s, _ in gates])
+        vals = np.empty((k, 8))
+        for j, (_, u) in enumerate(gates):
+            m = np.asarray(u, dtype=np.complex128).reshape(4)
+            vals[j, 0::2] = m.real
+            vals[j, 1::2] = m.imag
+        vals = np.ascontiguousarray(vals)
+        _lib.call('qs_mps_apply_1q_batch', ctypes.c_void_p(self.sites.data_ptr()),
+                  ctypes.c_void_p(self.bond.data_ptr()), self.stride, self.n, sites,
+                  vals.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), k, self._stream())
+
+    def reset_site(self, site: int) -> None:
+        """Zero the |1> half of a site and renormalise it, or set it to |0>
+        (fpga_mps_simulator.py:530-552), on the device."""
+        _lib.call('qs_mps_reset_site', ctypes.c_void_p(self.sites.data_ptr()),
+                  ctypes.c_void_p(self.bond.data_ptr()), self.stride, self.n, int(site),
+                  self._stream())
+
     def check_status(self) -> None:
         """Raise if any two-site update so far failed (status < 0)."""
         if self._err is not None:
@@ -262,6 +285,83 @@ class MPSChain:
         for s in self.sample(shots, measured, rng):
             c[s] = c.get(s, 0) + 1
         return c
+
+    # -- transfer-matrix read-outs on the device ---------------------------------
+    def environments(self):
+        """(left, right) environments of <psi|psi> on the device, each trace-normalised
+        (qs_mps_left_envs / qs_mps_right_envs)."""
+        import torch
+        lib = _lib.lib()
+        nbytes = lib.qs_mps_env_bytes(self.n, self.chi_max)
+        dev = self.sites.device
+        left = torch.empty(nbytes // 16, dtype=torch.complex128, device=dev)
+        right = torch.empty(nbytes // 16, dtype=torch.complex128, device=dev)
+        args = (ctypes.c_void_p(self.sites.data_ptr()), ctypes.c_void_p(self.bond.data_ptr()),
+                self.stride, self.n, self.chi_max)
+        _lib.call('qs_mps_left_envs', *args, ctypes.c_void_p(left.data_ptr()), nbytes,
+                  self._stream())
+        _lib.call('qs_mps_right_envs', *args, ctypes.c_void_p(right.data_ptr()), nbytes,
+                  self._stream())
+        return left, right
+
+    def _expect(self, op_a, op_b, span: int) -> np.ndarray:
+        import torch
+        dev = self.sites.device
+        left, right = self.environments()
+        cnt = self.n - span + 1
+        out = torch.zeros(max(cnt, 1), dtype=torch.float64, device=dev)
+        work = torch.empty(_lib.lib().qs_mps_expect_work_bytes(self.chi_max), dtype=torch.uint8,
+                           device=dev)
+
+        def mat(o):
+            m = np.asarray(o, dtype=np.complex128).reshape(4)
+            return (ctypes.c_double * 8)(*[float(v) for z in m for v in (z.real, z.imag)])
+        _lib.call('qs_mps_expect', ctypes.c_void_p(self.sites.data_ptr()),
+                  ctypes.c_void_p(self.bond.data_ptr()), self.stride, self.n, self.chi_max,
+                  ctypes.c_void_p(left.data_ptr()), ctypes.c_void_p(right.data_ptr()),
+                  mat(op_a), mat(op_b) if op_b is not None else None, span,
+                  ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(work.data_ptr()),
+                  work.numel(), self._stream())
+        return out.cpu().numpy()[:max(cnt, 0)]
+
+    def expectations(self, op) -> np.ndarray:
+        """<O_i> / <psi|psi> for a 2x2 operator on every site, on the device."""
+        return self._expect(op, None, 1)
+
+    def pair_expectations(self, op_a, op_b) -> np.ndarray:
+        """<O_i O'_{i+1}> / <psi|psi> for i = 0..n-2, on the device."""
+        return self._expect(op_a, op_b, 2)
+
+    def z_expectations(self) -> np.ndarray:
+        return self.expectations(np.diag([1.0, -1.0]))
+
+    def zz_expectations(self) -> np.ndarray:
+        z = np.diag([1.0, -1.0])
+        return self.pair_expectations(z, z)
+
+    def overlap_log(self, other: 'MPSChain'):
+        """(log |<self|other>|, arg <self|other>) with both norm_log factors, by transfer
+        matrices on the device (qs_mps_overlap)."""
+        import torch
+        if other.n != self.n or other.chi_max != self.chi_max:
+            raise ValueError('overlap needs chains of equal length and chi_max')
+        chi = self.chi_max
+        nbytes = _lib.lib().qs_mps_overlap_work_bytes(self.n, chi)
+        work = torch.empty(nbytes, dtype=torch.uint8, device=self.sites.device)
+        out = (ctypes.c_double * 2)()
+        _lib.call('qs_mps_overlap', ctypes.c_void_p(self.sites.data_ptr()),
+                  ctypes.c_void_p(self.bond.data_ptr()), self.stride,
+                  ctypes.c_void_p(other.sites.data_ptr()), ctypes.c_void_p(other.bond.data_ptr()),
+                  other.stride, self.n, chi, ctypes.c_void_p(work.data_ptr()), nbytes, out,
+                  self._stream())
+        return out[0] + self.norm_log + other.norm_log, out[1]
+
+    def fidelity(self, other: 'MPSChain') -> float:
+        """|<a|b>|^2 / (<a|a> <b|b>), on the device."""
+        ab, _ = self.overlap_log(other)
+        aa, _ = self.overlap_log(self)
+        bb, _ = other.overlap_log(other)
+        return float(math.exp(2 * ab - aa - bb))
 
 
 class HostMPS:

@@ -33,11 +33,21 @@ CUTOFF = 1e-10
 SEED = 42
 SHOTS = 1024
 MFMA_FP32_PEAK_TFLOPS = 157.3        # MI355X dense FP32 matrix
+FP64_PEAK_TFLOPS = 78.6              # MI355X FP64 (vector and matrix)
 
 
 def contraction_flops(cl: int, cm: int, cr: int) -> float:
     """Algorithmic real flops of one update's contraction: 32 chi_l chi_r (chi_m + 4)."""
     return 32.0 * cl * cr * (cm + 4)
+
+
+def svd_sweep_flops(m: int, ncol: int) -> float:
+    """Algorithmic real flops of one one-sided Jacobi sweep over an m x ncol complex
+    matrix: every column pair forms a^H b (8 m) and is rotated (two columns, 10 m each:
+    real cosine times a column, complex sine times the other, and their sum), and every
+    column's squared norm is formed once (4 m)."""
+    pairs = ncol * (ncol - 1) / 2
+    return pairs * 28.0 * m + 4.0 * m * ncol
 
 
 def _layers(gates):
@@ -77,13 +87,17 @@ def run_contraction_timed(layers):
     from .mps import MPSChain
     ch = MPSChain(N_SITES, CHI, CUTOFF, precision='fp32')
     flops = 0.0
+    svd_flops, sweeps = 0.0, []
     _lib.call('qs_mps_time_kernels', 1)
     try:
         for batch in layers:
             b = ch.bond.cpu().numpy().astype(np.float64)
-            for i, _ in batch:
-                flops += contraction_flops(b[i], b[i + 1], b[i + 2])
             ch.apply_2q_batch(batch)
+            sw = ch.last_sweeps()
+            for (i, _), nsw in zip(batch, sw):
+                flops += contraction_flops(b[i], b[i + 1], b[i + 2])
+                svd_flops += float(nsw) * svd_sweep_flops(int(2 * b[i]), int(2 * b[i + 2]))
+            sweeps.extend(int(v) for v in sw)
         torch.cuda.synchronize()
         cap = 8 * len(layers)
         kind = (ctypes.c_int32 * cap)()
@@ -97,7 +111,7 @@ def run_contraction_timed(layers):
     n = min(cnt.value, cap)
     t_con = sum(ms[k] for k in range(n) if kind[k] == 0) / 1e3
     t_svd = sum(ms[k] for k in range(n) if kind[k] == 2) / 1e3
-    return flops, t_con, t_svd
+    return flops, t_con, t_svd, svd_flops, sweeps
 
 
 def run_plain_sample(layers, sample_layers: List[int]):
@@ -126,7 +140,6 @@ def run_plain_sample(layers, sample_layers: List[int]):
 def run(plain_layers=(30, 31)) -> dict:
     import torch
     from . import workloads
-    from .mps import overlap
     gates = workloads.brickwall_gates(N_SITES, DEPTH, SEED)
     layers = _layers(gates)
     nupd = sum(len(b) for b in layers)
@@ -141,7 +154,7 @@ def run(plain_layers=(30, 31)) -> dict:
     ch32, t32, wall32 = run_batched(layers, 'fp32')
     ch64, t64, wall64 = run_batched(layers, 'fp64')
     n_plain, t_plain = run_plain_sample(layers, list(plain_layers))
-    flops, t_con, t_svd = run_contraction_timed(layers)
+    flops, t_con, t_svd, svd_flops, sweeps = run_contraction_timed(layers)
     # shot sampling of the final 1000-qubit state (right environments + 1024 shots)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -149,10 +162,9 @@ def run(plain_layers=(30, 31)) -> dict:
     torch.cuda.synchronize()
     t_samp = time.perf_counter() - t0
     bonds = ch64.bond_dims()
-    h32, h64 = ch32.to_host(), ch64.to_host()
-    fid = abs(overlap(h32, h64)) ** 2 / (overlap(h32, h32).real * overlap(h64, h64).real)
-    from .mps import z_expectations
-    dz = float(np.max(np.abs(z_expectations(h32) - z_expectations(h64))))
+    # FP32-contraction state against the FP64 one, by transfer matrices on the device
+    fid = ch32.fidelity(ch64)
+    dz = float(np.max(np.abs(ch32.z_expectations() - ch64.z_expectations())))
     s32 = sum(t32)
     s64 = sum(t64)
     samp32 = sum(t32[k] for k in plain_layers)
@@ -184,6 +196,18 @@ def run(plain_layers=(30, 31)) -> dict:
             'mfma_peak_tflops': MFMA_FP32_PEAK_TFLOPS,
             'mfma_frac': flops / t_con / 1e12 / MFMA_FP32_PEAK_TFLOPS,
             'svd_seconds_same_run': t_svd,
+        },
+        'svd': {
+            'kernel': 'mps_svd_kernel<64,*>',
+            'model': 'per update: sweeps x (ncol(ncol-1)/2 x 28 m + 4 m ncol) real flops, '
+                     'm = 2 chi_l rows, ncol = 2 chi_r columns',
+            'mean_sweeps_per_update': float(np.mean(sweeps)),
+            'algorithmic_tflop': svd_flops / 1e12,
+            'seconds': t_svd,
+            'tflops': svd_flops / t_svd / 1e12,
+            'fp64_peak_tflops': FP64_PEAK_TFLOPS,
+            'fp64_frac': svd_flops / t_svd / 1e12 / FP64_PEAK_TFLOPS,
+            'share_of_update_kernel_time': t_svd / (t_svd + t_con),
         },
         'svd_team_width': {str(c): r['lanes'] for c, r in tuned.items()},
         'sampling_shots_per_s': SHOTS / t_samp,

@@ -302,6 +302,34 @@ int qs_mps_sample(const double* sites, const int32_t* bond, int64_t site_stride,
                   int chi_max, const double* envs, const double* uniforms, int shots,
                   uint8_t* bits, void* stream);
 
+/* Transfer-matrix read-outs of device chains (no counterpart in the reference, whose
+ * observables need the dense statevector, mps_core.py:433-461, or host copies).  Chains are
+ * given as for qs_mps_right_envs (sites, bond, site_stride, n, chi_max <= 128).
+ * qs_mps_left_envs writes L_0 = [[1]] and L_{i+1}[a', b'] = sum_s sum_{a,b}
+ * conj(A_i[a, s, a']) L_i[a, b] A_i[b, s, b'], each divided by its trace, (bond[i] x bond[i])
+ * at envs + i * chi_max^2 complex elements (row stride chi_max; qs_mps_env_bytes bytes).
+ * qs_mps_expect writes, for span 1, out[i] = <psi|O_i|psi> / <psi|psi> for i < n with O =
+ * op_a, and for span 2 out[i] = <psi|O_i O'_{i+1}|psi> / <psi|psi> for i < n - 1 with
+ * O' = op_b (2x2 complex row-major, host memory; out is device float64), from the left
+ * environments and the right environments of qs_mps_right_envs; `work` is device memory of
+ * qs_mps_expect_work_bytes(chi_max).  qs_mps_overlap computes <bra|ket> of two chains of
+ * n sites (norm_log factors excluded) into out_host = {log |<bra|ket>|, arg <bra|ket>};
+ * `work` is device memory of qs_mps_overlap_work_bytes(n, chi_max); synchronous.
+ * Every transfer is renormalised on the way (trace or entry sum), so chains of thousands
+ * of sites neither overflow nor underflow. */
+int qs_mps_left_envs(const double* sites, const int32_t* bond, int64_t site_stride, int n,
+                     int chi_max, double* envs, size_t env_bytes, void* stream);
+size_t qs_mps_expect_work_bytes(int chi_max);
+int qs_mps_expect(const double* sites, const int32_t* bond, int64_t site_stride, int n,
+                  int chi_max, const double* lenvs, const double* renvs, const double op_a[8],
+                  const double op_b[8], int span, double* out, void* work, size_t work_bytes,
+                  void* stream);
+size_t qs_mps_overlap_work_bytes(int n, int chi_max);
+int qs_mps_overlap(const double* bra_sites, const int32_t* bra_bond, int64_t bra_stride,
+                   const double* ket_sites, const int32_t* ket_bond, int64_t ket_stride, int n,
+                   int chi_max, void* work, size_t work_bytes, double out_host[2],
+                   void* stream);
+
 /* SVD tile shape, the autotuner's knob (no counterpart in the reference, whose SVD is
  * LAPACK gesdd on the host).  The register SVD of size class 8, 16, 32 or 64 runs one
  * team of `lanes` lanes per column pair (lanes * class threads per task, 2 class / lanes
@@ -313,6 +341,21 @@ int qs_mps_sample(const double* sites, const int32_t* bond, int64_t site_stride,
 int qs_mps_svd_lane_choices(int chi_class, int* out, int cap, int* count);
 int qs_mps_svd_lanes(int chi_class);
 int qs_mps_set_svd_lanes(int chi_class, int lanes);
+
+/* Many one-site gates in one launch sequence (the driver's batched flush of up to 508
+ * one-qubit instructions per kernel call, fpga_mps_simulator.py:498-526, 562): gate j is the
+ * 2x2 complex row-major matrix mats[8j .. 8j+7] (host memory) on site site_list[j] (host)
+ * of a chain of n sites.  Sites must be distinct (compose the gates of one site first);
+ * gates travel in the launch arguments, 32 per launch. */
+int qs_mps_apply_1q_batch(double* sites, const int32_t* bond, int64_t site_stride, int n,
+                          const int32_t* site_list, const double* mats, int count,
+                          void* stream);
+
+/* Reset of one site on the device (fpga_mps_simulator.py:530-552): the sigma = 1 half is
+ * zeroed and the site divided by its norm, or set to the |0> tensor (t[0,0,0] = 1) when
+ * that norm is at most 1e-15. */
+int qs_mps_reset_site(double* sites, const int32_t* bond, int64_t site_stride, int n, int site,
+                      void* stream);
 
 /* A'[a, s', b] = sum_s m[s', s] A[a, s, b] on one site, dims read from bond[site],
  * bond[site+1] on the device (fpga_mps_simulator.cpp:232-276, mps_core.py:267-273).

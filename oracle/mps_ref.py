@@ -384,24 +384,72 @@ def overlap(bra: MPS, ket: MPS) -> complex:
     return complex(env[0, 0]) * math.exp(bra.norm_log + ket.norm_log)
 
 
-def local_expectations(m: MPS, op: np.ndarray) -> np.ndarray:
-    """<O_i>/<psi|psi> for a one-site operator O on every site."""
+def _environments(m: MPS):
+    """Left and right transfer environments of <psi|psi>, each divided by its trace (the
+    ratios below are unchanged, and long chains neither overflow nor underflow)."""
     n = m.n
     left = [np.ones((1, 1), dtype=np.complex128)]
     for a in m.sites:
-        left.append(np.einsum('xy,xsa,ysb->ab', left[-1], a.conj(), a))
+        e = np.einsum('xy,xsa,ysb->ab', left[-1], a.conj(), a)
+        left.append(e / max(abs(np.trace(e)), 1e-300))
     right = [None] * (n + 1)
     right[n] = np.ones((1, 1), dtype=np.complex128)
     for i in range(n - 1, -1, -1):
         a = m.sites[i]
-        right[i] = np.einsum('xsa,ysb,ab->xy', a.conj(), a, right[i + 1])
-    nrm = np.real(left[n][0, 0])
-    out = np.empty(n)
-    for i in range(n):
+        e = np.einsum('xsa,ysb,ab->xy', a.conj(), a, right[i + 1])
+        right[i] = e / max(abs(np.trace(e)), 1e-300)
+    return left, right
+
+
+def local_expectations(m: MPS, op: np.ndarray) -> np.ndarray:
+    """<O_i>/<psi|psi> for a one-site operator O on every site."""
+    left, right = _environments(m)
+    out = np.empty(m.n)
+    eye = np.eye(2, dtype=np.complex128)
+    for i in range(m.n):
         a = m.sites[i]
         v = np.einsum('xy,xsa,st,ytb,ab->', left[i], a.conj(), op, a, right[i + 1])
-        out[i] = np.real(v) / nrm
+        d = np.einsum('xy,xsa,st,ytb,ab->', left[i], a.conj(), eye, a, right[i + 1])
+        out[i] = np.real(v) / np.real(d)
     return out
+
+
+def zz_neighbours(m: MPS) -> np.ndarray:
+    """<Z_i Z_i+1>/<psi|psi> for i = 0..n-2."""
+    left, right = _environments(m)
+    z = FIXED_GATES['z']
+    eye = np.eye(2, dtype=np.complex128)
+    out = np.empty(m.n - 1)
+    for i in range(m.n - 1):
+        a, b = m.sites[i], m.sites[i + 1]
+        num = np.einsum('xy,xsa,st,ytb->ab', left[i], a.conj(), z, a)
+        num = np.einsum('xy,xsa,st,ytb->ab', num, b.conj(), z, b)
+        den = np.einsum('xy,xsa,ysb->ab', left[i], a.conj(), a)
+        den = np.einsum('xy,xsa,ysb->ab', den, b.conj(), b)
+        out[i] = np.real(np.sum(num * right[i + 2])) / np.real(np.sum(den * right[i + 2]))
+    return out
+
+
+def random_product_state(n: int, seed: int) -> np.ndarray:
+    """n normalised one-qubit states (n x 2 complex) from a seeded generator."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def product_overlap_log(m: MPS, phi: np.ndarray):
+    """<phi|psi> for the product state phi (n x 2) as (log |<phi|psi>|, arg <phi|psi>),
+    norm_log included; the running boundary vector is renormalised at every site."""
+    v = np.ones(1, dtype=np.complex128)
+    logabs = m.norm_log
+    for i, a in enumerate(m.sites):
+        v = np.einsum('s,x,xsb->b', phi[i].conj(), v, a)
+        nv = float(np.linalg.norm(v))
+        if nv == 0.0:
+            return -math.inf, 0.0
+        logabs += math.log(nv)
+        v = v / nv
+    return logabs + math.log(abs(v[0])), float(np.angle(v[0]))
 
 
 def tfim_energy(m: MPS, j_coupling: float = 1.0, h_field: float = 1.0) -> float:

@@ -181,6 +181,18 @@ def test_argument_errors_are_reported_without_device():
         t2[k].n_sites = 8
     assert lib.qs_mps_update_batch(t2, 2, 0, ctypes.c_void_p(1), 1 << 30, None, None) == _lib.QS_E_ARG
     assert b'share a site' in lib.qs_last_error()
+    # batched one-site gates and site resets: argument checks before any launch
+    sl = (ctypes.c_int32 * 3)(1, 4, 1)
+    mats = (ctypes.c_double * 24)()
+    one = ctypes.c_void_p(16)
+    assert lib.qs_mps_apply_1q_batch(one, one, 128, 8, sl, mats, 3, None) == _lib.QS_E_ARG
+    assert b'appears twice' in lib.qs_last_error()
+    sl[2] = 8
+    assert lib.qs_mps_apply_1q_batch(one, one, 128, 8, sl, mats, 3, None) == _lib.QS_E_ARG
+    assert b'outside the chain' in lib.qs_last_error()
+    assert lib.qs_mps_apply_1q_batch(one, one, 128, 8, None, None, 0, None) == _lib.QS_OK
+    assert lib.qs_mps_reset_site(one, one, 128, 8, 8, None) == _lib.QS_E_ARG
+    assert lib.qs_mps_reset_site(None, one, 128, 8, 0, None) == _lib.QS_E_ARG
     # the allocation-free batch: argument checks and the scratch limit come first
     assert lib.qs_mps_update_batch_static(t, 1, 7, None, None) == _lib.QS_E_ARG
     assert lib.qs_mps_update_batch_static(t2, 2, 0, None, None) == _lib.QS_E_ARG

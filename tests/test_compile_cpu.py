@@ -120,3 +120,23 @@ def test_oracle_samples_equal_mps_core_fixture(name):
             bits = s_full[::-1]
             own.append(''.join(bits[q] for q in sorted(meas, reverse=True)))
         assert Counter(own) != Counter(exp)
+
+
+def test_dropin_pauli_expectation_matches_dense_kronecker():
+    """The drop-in's _pauli_expectation_sv (fpga_mps_simulator.py:1091-1108: character k
+    acts on qubit n-1-k) against the explicit Kronecker product and the oracle."""
+    import fpga_mps_simulator as fms
+    rng = np.random.default_rng(3)
+    n = 5
+    sv = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    sv /= np.linalg.norm(sv)
+    mats = {'I': np.eye(2), 'X': np.array([[0, 1], [1, 0]]),
+            'Y': np.array([[0, -1j], [1j, 0]]), 'Z': np.diag([1, -1])}
+    for _ in range(20):
+        p = ''.join(rng.choice(list('IXYZ'), n))
+        full = np.array([[1.0]])
+        for ch in p:                      # first character = most significant = qubit n-1
+            full = np.kron(full, mats[ch])
+        want = float(np.real(np.vdot(sv, full @ sv)))
+        assert abs(fms._pauli_expectation_sv(sv, p) - want) < 1e-12
+        assert abs(mps_ref.pauli_expectation_sv(sv, p) - want) < 1e-12

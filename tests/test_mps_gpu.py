@@ -395,3 +395,38 @@ def test_dropin_run_equals_reference_compiler_fixture(sim):
     assert np.max(np.abs(sv - ref)) < 1e-10 * max(1.0, float(np.max(np.abs(ref))))
     assert res.get_bond_dims() == [int(b) for b in case['bond_dims']]
     assert res.get_counts() == dict(Counter(str(s) for s in case['samples']))
+
+
+# ------------------------------------------------ one-site gates and resets on the device
+def test_batched_one_site_gates_and_device_reset_match_oracle():
+    """qs_mps_apply_1q_batch over more sites than one launch carries (32) and
+    qs_mps_reset_site, including the |0> fallback of a site polarised into |1>
+    (fpga_mps_simulator.py:530-552), against the oracle's gate loop."""
+    from qsim_amd.mps import MPSChain
+    n, chi = 40, 16
+    rng = np.random.default_rng(8)
+    ch = MPSChain(n, chi, 1e-12)
+    gl = [{'type': '2q', 'sites': [i, i + 1], 'mat': g}
+          for _, i, g in workloads.brickwall_gates(n, 4, seed=8, kind='haar')]
+    us = [mps_ref.gate_1q('u3', list(rng.uniform(0, 6.28, 3))) for _ in range(n)]
+    for g in gl:
+        ch.apply_2q(g['sites'][0], g['mat'])
+    ch.apply_1q_batch([(s, us[s]) for s in range(n)])
+    ch.reset_site(5)
+    ch.reset_site(17)
+    ch.check_status()
+    ref_gl = gl + [{'type': '1q', 'sites': [s], 'mat': us[s]} for s in range(n)] + \
+        [{'type': 'reset', 'sites': [5]}, {'type': 'reset', 'sites': [17]}]
+    ref = mps_ref.run_gate_list(n, ref_gl, chi, 1e-12)
+    got = as_oracle(ch.to_host())
+    assert got.bond_dims() == ref.bond_dims()
+    gg, rr = mps_ref.overlap(got, got).real, mps_ref.overlap(ref, ref).real
+    gr = mps_ref.overlap(got, ref)
+    assert abs(gg / rr - 1) < 1e-10                 # same norm: the resets scale alike
+    assert abs(gr / rr - 1) < 1e-10                 # same state, phase included
+    # a product-state site entirely in |1>: reset falls back to |0>
+    one = MPSChain(3, 4, 1e-12)
+    one.apply_1q_batch([(1, np.array([[0, 1], [1, 0]], dtype=complex))])
+    one.reset_site(1)
+    t = one.to_host().sites[1]
+    assert t.shape == (1, 2, 1) and t[0, 0, 0] == 1.0 and t[0, 1, 0] == 0.0
