@@ -61,6 +61,8 @@ def parse():
     ap.add_argument('--multi', type=int, default=0, metavar='NCIRC',
                     help='instead of the headline line, print one JSON line for NCIRC '
                          'independent MPS circuits (chi_max 16..128) replayed as one graph')
+    ap.add_argument('--multi-sample', type=int, default=24, metavar='NCIRC',
+                    help='circuits of the many-circuit MPS workload run inside the default line')
     ap.add_argument('--pass-table', metavar='PATH', default=None,
                     help='also write a per-pass table (mean kernel time over the timed steps, '
                          'tile qubits, stages and op counts by kind) as markdown to PATH')
@@ -297,6 +299,11 @@ def main():
         m = mps_leg()
         if m is not None:
             out['mps'] = m
+            from qsim_amd import mps_bench
+            out['mps']['multi'] = mps_bench.run_multi(args.multi_sample, quiet=True)
+            out['mps']['multi']['sample'] = (
+                f'the first {args.multi_sample} circuits of the seeded 256-circuit list; the '
+                f'full list: python bench.py --multi 256')
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_leg(args.cpu_gates)
         if not args.no_mps and 'mps' in out:

@@ -259,6 +259,131 @@ mps_contract_f32_mfma_kernel(const DevTask* __restrict__ tasks, cd2* __restrict_
     }
 }
 
+// The same complex-FP32 MFMA contraction for the chi 65..128 class, where theta (up to
+// 256 x 256) is larger than one workgroup's LDS: workgroup (task, tile) owns the rows of
+// a in [64 ta, 64 ta + 64) (both p) and the columns of d in [64 td, 64 td + 64) (both q),
+// a 128 x 128 block of T, so the gate mixing of each (a, d) finds its four (p, q) entries
+// in the same workgroup.  K = chi_m (up to 128) runs in chunks of 64 staged through LDS
+// as in the kernel above; the accumulators stay in registers across chunks.
+__global__ void __launch_bounds__(kMfmaThreads)
+mps_contract_f32_mfma_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws,
+                                 size_t theta_slot, int32_t* __restrict__ status) {
+    __shared__ float lds[2 * kMaxM * kMaxK * 2];
+    float* Are = lds;                           // [128 local rows][64 k]
+    float* Aim = Are + kMaxM * kMaxK;
+    float* Bre = Aim + kMaxM * kMaxK;           // [64 k][128 local columns]
+    float* Bim = Bre + kMaxK * kMaxM;
+
+    const qs_mps_task& T = tasks[blockIdx.x].t;
+    Dims d;
+    if (!load_dims(T, d, theta_slot)) {
+        if (blockIdx.y == 0 && threadIdx.x == 0 && status)
+            status[tasks[blockIdx.x].orig] = QS_E_CHI;
+        return;
+    }
+    const int a0 = 64 * (blockIdx.y >> 1), d0 = 64 * (blockIdx.y & 1);
+    if (a0 >= d.cl || d0 >= d.cr) return;        // uniform per workgroup
+    const int K = d.cm, N = 2 * d.cr;
+    const cd2* A = site_ptr(T, T.site);          // rows 2a + p, K columns
+    const cd2* B = site_ptr(T, T.site + 1);      // K rows, columns q cr + d
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int qm = (wave >> 1) * 64, qn = (wave & 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+    floatx4 cre[4][4], cim[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cre[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+            cim[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+    for (int kc = 0; kc < K; kc += kMaxK) {
+        const int kn = min(kMaxK, K - kc);
+        const int Kp = (kn + 3) & ~3;
+        for (int idx = tid; idx < kMaxM * Kp; idx += kMfmaThreads) {
+            const int r = idx / Kp, k = idx - r * Kp;
+            const int a = a0 + (r >> 1);
+            cd2 v = mk2(0, 0);
+            if (a < d.cl && k < kn) v = A[(size_t)(2 * a0 + r) * K + kc + k];
+            Are[r * kMaxK + k] = (float)v.x;
+            Aim[r * kMaxK + k] = (float)v.y;
+        }
+        for (int idx = tid; idx < Kp * kMaxM; idx += kMfmaThreads) {
+            const int k = idx / kMaxM, c = idx - k * kMaxM;
+            const int dd = d0 + (c & 63);
+            cd2 v = mk2(0, 0);
+            if (k < kn && dd < d.cr) v = B[(size_t)(kc + k) * N + (c >> 6) * d.cr + dd];
+            Bre[k * kMaxM + c] = (float)v.x;
+            Bim[k * kMaxM + c] = (float)v.y;
+        }
+        __syncthreads();
+        for (int k0 = 0; k0 < Kp; k0 += 4) {
+            float ar[4], ai[4], nai[4], br[4], bi[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = qm + 16 * i + lr;
+                ar[i] = Are[row * kMaxK + k0 + lk];
+                ai[i] = Aim[row * kMaxK + k0 + lk];
+                nai[i] = -ai[i];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int col = qn + 16 * j + lr;
+                br[j] = Bre[(k0 + lk) * kMaxM + col];
+                bi[j] = Bim[(k0 + lk) * kMaxM + col];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], br[j], cre[i][j], 0, 0, 0);
+                    cre[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[i], bi[j], cre[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bi[j], cim[i][j], 0, 0, 0);
+                    cim[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], br[j], cim[i][j], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    float* Tre = lds;                             // [128 local rows][128 local columns]
+    float* Tim = lds + kMaxM * kMaxM;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = qm + 16 * i + 4 * lk + e;
+                const int col = qn + 16 * j + lr;
+                Tre[row * kMaxM + col] = cre[i][j][e];
+                Tim[row * kMaxM + col] = cim[i][j][e];
+            }
+    __syncthreads();
+    cd2 g[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) g[k] = mk2(T.gate[2 * k], T.gate[2 * k + 1]);
+    cd2* th = ws + (size_t)blockIdx.x * theta_slot;
+    const int cr = d.cr, ldt = 2 * cr;
+    const int na = min(64, d.cl - a0), nd = min(64, cr - d0);
+    for (int pr = tid; pr < na * nd; pr += kMfmaThreads) {
+        const int al = pr / nd, dl = pr - al * nd;
+        const int r0 = 2 * al, r1 = r0 + 1, c0 = dl, c1 = 64 + dl;
+        const cd2 t00 = mk2(Tre[r0 * kMaxM + c0], Tim[r0 * kMaxM + c0]);
+        const cd2 t01 = mk2(Tre[r0 * kMaxM + c1], Tim[r0 * kMaxM + c1]);
+        const cd2 t10 = mk2(Tre[r1 * kMaxM + c0], Tim[r1 * kMaxM + c0]);
+        const cd2 t11 = mk2(Tre[r1 * kMaxM + c1], Tim[r1 * kMaxM + c1]);
+        const int a = a0 + al, e = d0 + dl;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            cd2 v = cmul(g[4 * s + 0], t00);
+            v = cmadd(v, g[4 * s + 1], t01);
+            v = cmadd(v, g[4 * s + 2], t10);
+            v = cmadd(v, g[4 * s + 3], t11);
+            th[(size_t)(2 * a + (s >> 1)) * ldt + (s & 1) * cr + e] = v;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ Jacobi SVD
 // FP64 reciprocal and reciprocal square root: the hardware estimate refined by two
 // Newton steps (relative error at rounding level).
@@ -1294,7 +1419,14 @@ int launch_batch(const DevTask* dtasks, int ntasks, const int32_t* counts, int c
                            dtasks, theta, slot, status_dev);
         if ((st = timer_end(hs, e1, 0, nreg))) return st;
     }
-    const int f64_first = precision == QS_PREC_FP32_MFMA ? nreg : 0;
+    if (precision == QS_PREC_FP32_MFMA && counts[4] > 0) {
+        if ((st = timer_begin(hs, &e1))) return st;
+        hipLaunchKernelGGL(mps_contract_f32_mfma_big_kernel, dim3(counts[4], 4),
+                           dim3(kMfmaThreads), 0, hs, dtasks + nreg,
+                           theta + (size_t)nreg * slot, slot, status_dev);
+        if ((st = timer_end(hs, e1, 0, counts[4]))) return st;
+    }
+    const int f64_first = precision == QS_PREC_FP32_MFMA ? ntasks : 0;
     if (ntasks > f64_first) {
         if ((st = timer_begin(hs, &e1))) return st;
         hipLaunchKernelGGL(mps_contract_fp64_kernel, dim3(ntasks - f64_first), dim3(256), 0, hs,

@@ -217,30 +217,46 @@ def run(plain_layers=(30, 31)) -> dict:
 
 
 def run_multi(ncirc: int = 256, seed: int = 7, precision: str = 'fp32',
-              max_tasks: int = 4096) -> dict:
-    """The batched multi-circuit workload, planned and recorded as one graph."""
+              max_tasks: int = 4096, quiet: bool = False) -> dict:
+    """The batched multi-circuit workload (the first `ncirc` circuits of the seeded
+    256-circuit list), planned and recorded as graphs, one per round of updates."""
     import torch
     from . import mps_tune
     from .mps_multi import MultiRun, random_specs
-    specs = random_specs(ncirc, seed=seed, chis=(16, 32, 64, 128))
+    specs = random_specs(256, seed=seed, chis=(16, 32, 64, 128))[:ncirc]
     tuned = mps_tune.autotune(precision=precision)
+    import sys
     run = MultiRun(specs, cutoff=1e-10, precision=precision, max_tasks=max_tasks)
     t0 = time.perf_counter()
-    run.build_graph()
+
+    def planned(r, nr):
+        if not quiet and (r % 10 == 9 or r == nr - 1):
+            print(f'multi: planned round {r + 1}/{nr}, {time.perf_counter() - t0:.0f} s',
+                  file=sys.stderr, flush=True)
+    run.build_graph(progress=planned)
     plan_s = time.perf_counter() - t0
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
+    t1 = time.perf_counter()
+
+    def replayed(k, nk):
+        # one synchronisation per ten rounds, outside the replayed graphs: a progress line
+        if not quiet and k % 10 == 9:
+            torch.cuda.synchronize()
+            print(f'multi: replayed round {k + 1}/{nk}, {time.perf_counter() - t1:.0f} s',
+                  file=sys.stderr, flush=True)
     e0.record(stream)
-    run.replay()
+    run.replay(progress=replayed)
     e1.record(stream)
     torch.cuda.synchronize()
     run.check_graph()
     secs = e0.elapsed_time(e1) / 1e3
     per_class = {}
+    upd = np.bincount(run.owner[:run._next], minlength=len(specs))
     for c in (16, 32, 64, 128):
         idx = [i for i, s in enumerate(specs) if s.chi_max == c]
-        per_class[str(c)] = {'circuits': len(idx),
+        per_class[str(c)] = {'circuits': len(idx), 'updates': int(sum(upd[i] for i in idx)),
                              'max_bond': max((run.chains[i].max_bond() for i in idx), default=0)}
     return {
         'workload': f'{ncirc} independent circuits, n ~ U{{50..2000}}, chi_max in '
@@ -250,6 +266,7 @@ def run_multi(ncirc: int = 256, seed: int = 7, precision: str = 'fp32',
         'updates_per_s': run._next / secs,
         'seconds': secs,
         'graph_chunks': len(run.g_chunks),
+        'graphs': len(run.graphs),
         'plan_and_record_seconds': plan_s,
         'classes': per_class,
         'svd_team_width': {str(c): r['lanes'] for c, r in tuned.items()},

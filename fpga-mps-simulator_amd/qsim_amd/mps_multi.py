@@ -131,19 +131,23 @@ class MultiRun:
             self._submit(self._round_tasks(r))
 
     # -- graph mode --------------------------------------------------------------
-    def build_graph(self, rounds: Optional[int] = None):
-        """Plan every round, upload the launch tables and record the run as a graph.
+    def build_graph(self, rounds: Optional[int] = None, progress=None):
+        """Plan every round, upload the launch tables and record the run as graphs, one
+        per round of updates.
 
-        The graph covers exactly what run() would do for the same rounds (same gates,
-        same dlog slots); replay() executes it.  Returns the torch CUDAGraph.
+        The graphs cover exactly what run() would do for the same rounds (same gates,
+        same dlog slots); replay() executes them in order.  `progress(r, rounds)` is
+        called after each round is planned.  Returns the list of torch CUDAGraphs.
         """
         import torch
         lib = _lib.lib()
         chunks = []                  # (table byte offset, ntasks, counts, chi, status offset)
         tables = []
+        round_of_chunk = []
         off = 0
         soff = 0
-        for r in range(self.rounds if rounds is None else rounds):
+        nround = self.rounds if rounds is None else rounds
+        for r in range(nround):
             tasks = self._round_tasks(r)
             for b in range(0, tasks.size, self.max_tasks):
                 chunk = np.ascontiguousarray(tasks[b:b + self.max_tasks])
@@ -155,31 +159,43 @@ class MultiRun:
                           ctypes.c_void_p(buf.ctypes.data), nb, counts, ctypes.byref(chi))
                 tables.append(buf)
                 chunks.append((off, chunk.size, counts, chi.value, soff))
+                round_of_chunk.append(r)
                 off += nb
                 soff += chunk.size
+            if progress is not None:
+                progress(r, nround)
         dev = f'cuda:{self.device}'
         host = np.concatenate(tables) if tables else np.zeros(1, dtype=np.uint8)
         self.g_tables = torch.from_numpy(host).to(dev)
         self.g_status = torch.zeros(max(soff, 1), dtype=torch.int32, device=dev)
         self.g_chunks = chunks
         torch.cuda.synchronize(self.device)
-        g = torch.cuda.CUDAGraph()
         side = torch.cuda.Stream(self.device)
         side.wait_stream(torch.cuda.current_stream(self.device))
-        with torch.cuda.graph(g, stream=side):
-            stream = _lib.current_stream(self.device)
-            tbase = self.g_tables.data_ptr()
-            sbase = self.g_status.data_ptr()
-            for toff, n, counts, chi, so in chunks:
-                _lib.call('qs_mps_batch_launch', ctypes.c_void_p(tbase + toff), n, counts, chi,
-                          self.prec, ctypes.c_void_p(self.ws.data_ptr()), self.ws.numel(),
-                          ctypes.c_void_p(sbase + 4 * so), stream)
+        tbase = self.g_tables.data_ptr()
+        sbase = self.g_status.data_ptr()
+        graphs = []
+        for r in sorted(set(round_of_chunk)):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=side):
+                stream = _lib.current_stream(self.device)
+                for (toff, n, counts, chi, so), rc in zip(chunks, round_of_chunk):
+                    if rc != r:
+                        continue
+                    _lib.call('qs_mps_batch_launch', ctypes.c_void_p(tbase + toff), n, counts,
+                              chi, self.prec, ctypes.c_void_p(self.ws.data_ptr()),
+                              self.ws.numel(), ctypes.c_void_p(sbase + 4 * so), stream)
+            graphs.append(g)
         torch.cuda.current_stream(self.device).wait_stream(side)
-        self.graph = g
-        return g
+        self.graphs = graphs
+        return graphs
 
-    def replay(self) -> None:
-        self.graph.replay()
+    def replay(self, progress=None) -> None:
+        """Replay the recorded rounds in order; `progress(k, count)` after each launch."""
+        for k, g in enumerate(self.graphs):
+            g.replay()
+            if progress is not None:
+                progress(k, len(self.graphs))
 
     def check_graph(self) -> None:
         worst = int(self.g_status.min().item())

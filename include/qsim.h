@@ -188,8 +188,9 @@ int qs_sv_apply_tfim(const void* psi, void* out, int nq, double J, double h, voi
 #define QS_PREC_FP32_MFMA  1   /* contraction complex FP32 on MFMA, SVD FP64        */
 
 /* Largest chi_max the two-site update supports (128).  Up to 64 the SVD keeps theta in
- * registers; the 65..128 class keeps the columns in a working copy in the workspace, and
- * its contraction runs in FP64 whatever the requested precision. */
+ * registers; the 65..128 class keeps the columns in a working copy in the workspace.  Both
+ * classes take the requested contraction precision (the 65..128 class contracts in
+ * 128 x 128 blocks of theta, four workgroups per update). */
 int qs_mps_max_chi(void);
 
 /* One two-site update of a chain: sites i and i+1 of the chain whose slots start at

@@ -390,13 +390,13 @@ def _environments(m: MPS):
     n = m.n
     left = [np.ones((1, 1), dtype=np.complex128)]
     for a in m.sites:
-        e = np.einsum('xy,xsa,ysb->ab', left[-1], a.conj(), a)
+        e = np.einsum('xy,xsa,ysb->ab', left[-1], a.conj(), a, optimize=True)
         left.append(e / max(abs(np.trace(e)), 1e-300))
     right = [None] * (n + 1)
     right[n] = np.ones((1, 1), dtype=np.complex128)
     for i in range(n - 1, -1, -1):
         a = m.sites[i]
-        e = np.einsum('xsa,ysb,ab->xy', a.conj(), a, right[i + 1])
+        e = np.einsum('xsa,ysb,ab->xy', a.conj(), a, right[i + 1], optimize=True)
         right[i] = e / max(abs(np.trace(e)), 1e-300)
     return left, right
 
@@ -408,8 +408,10 @@ def local_expectations(m: MPS, op: np.ndarray) -> np.ndarray:
     eye = np.eye(2, dtype=np.complex128)
     for i in range(m.n):
         a = m.sites[i]
-        v = np.einsum('xy,xsa,st,ytb,ab->', left[i], a.conj(), op, a, right[i + 1])
-        d = np.einsum('xy,xsa,st,ytb,ab->', left[i], a.conj(), eye, a, right[i + 1])
+        v = np.einsum('xy,xsa,st,ytb,ab->', left[i], a.conj(), op, a, right[i + 1],
+                      optimize=True)
+        d = np.einsum('xy,xsa,st,ytb,ab->', left[i], a.conj(), eye, a, right[i + 1],
+                      optimize=True)
         out[i] = np.real(v) / np.real(d)
     return out
 
@@ -422,10 +424,10 @@ def zz_neighbours(m: MPS) -> np.ndarray:
     out = np.empty(m.n - 1)
     for i in range(m.n - 1):
         a, b = m.sites[i], m.sites[i + 1]
-        num = np.einsum('xy,xsa,st,ytb->ab', left[i], a.conj(), z, a)
-        num = np.einsum('xy,xsa,st,ytb->ab', num, b.conj(), z, b)
-        den = np.einsum('xy,xsa,ysb->ab', left[i], a.conj(), a)
-        den = np.einsum('xy,xsa,ysb->ab', den, b.conj(), b)
+        num = np.einsum('xy,xsa,st,ytb->ab', left[i], a.conj(), z, a, optimize=True)
+        num = np.einsum('xy,xsa,st,ytb->ab', num, b.conj(), z, b, optimize=True)
+        den = np.einsum('xy,xsa,ysb->ab', left[i], a.conj(), a, optimize=True)
+        den = np.einsum('xy,xsa,ysb->ab', den, b.conj(), b, optimize=True)
         out[i] = np.real(np.sum(num * right[i + 2])) / np.real(np.sum(den * right[i + 2]))
     return out
 

@@ -25,12 +25,16 @@ def oracle_run(spec, cutoff):
 @pytest.mark.parametrize('precision', ['fp64', 'fp32'])
 def test_multi_circuit_batch_matches_oracle(precision):
     from qsim_amd.mps_multi import CircuitSpec, MultiRun
+    # the chi-100 and chi-128 circuits reach their chi_max inside the batch (bonds 100
+    # and 128 in the middle), so both SVD classes and, for fp32, both MFMA contractions run
     specs = [CircuitSpec(9, 4, 7, 1), CircuitSpec(14, 8, 12, 2), CircuitSpec(30, 16, 9, 3),
-             CircuitSpec(21, 32, 14, 4), CircuitSpec(12, 64, 16, 5), CircuitSpec(2, 2, 3, 6)]
+             CircuitSpec(21, 32, 14, 4), CircuitSpec(12, 64, 16, 5), CircuitSpec(2, 2, 3, 6),
+             CircuitSpec(14, 100, 14, 7), CircuitSpec(16, 128, 16, 8)]
     cut = 1e-10
     run = MultiRun(specs, cutoff=cut, precision=precision, max_tasks=40)
     run.run()
     run.check()
+    assert run.chains[6].max_bond() == 100 and run.chains[7].max_bond() == 128
     logs = run.norm_logs()
     for c, s in enumerate(specs):
         ref = oracle_run(s, cut)
@@ -46,8 +50,9 @@ def test_multi_circuit_batch_matches_oracle(precision):
 
 
 def test_graph_replay_matches_eager_run():
-    """The whole multi-circuit run recorded as one graph: every replay (from the same
-    start, restored by eager copies between replays) equals the eager run bit for bit."""
+    """The whole multi-circuit run recorded as graphs (one per round of updates): every
+    replay (from the same start, restored by eager copies between replays) equals the
+    eager run bit for bit."""
     import torch
     from qsim_amd.mps_multi import CircuitSpec, MultiRun
     specs = [CircuitSpec(9, 4, 7, 1), CircuitSpec(14, 8, 12, 2), CircuitSpec(30, 16, 9, 3),
