@@ -9,15 +9,18 @@ Recorded (all gauge-invariant): the bond dimensions and norm_log after every lay
 <Z_i> and <Z_i Z_i+1> of the final state, and the overlaps <phi_k|psi> with four seeded
 random product states (log-magnitude and phase, norm_log included) as a fidelity anchor.
 
-usage: python tests/golden/make_c3_fixture.py      (several minutes on 8 cores)
+usage: python tests/golden/make_c3_fixture.py      (a few minutes, one BLAS thread)
 """
 
-import math
 import os
 import sys
 import time
 
-import numpy as np
+# 128 x 128 SVDs run fastest on one thread; more threads only contend
+os.environ.setdefault('OPENBLAS_NUM_THREADS', '1')
+os.environ.setdefault('OMP_NUM_THREADS', '1')
+
+import numpy as np  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))

@@ -125,9 +125,15 @@ def test_haar_chi128_class_matches_oracle(chi, precision):
     host = ch.to_host()
     assert host.bond_dims() == ref.bond_dims()
     assert max(host.bond_dims()) == chi
-    assert abs(host.norm_log - ref.norm_log) < 1e-10 * max(1.0, abs(ref.norm_log))
     sv, rv = host.get_statevector(), ref.get_statevector()
-    assert np.max(np.abs(sv - rv)) < 1e-10 * np.max(np.abs(rv))
+    if precision == 'fp64':
+        assert abs(host.norm_log - ref.norm_log) < 1e-10 * max(1.0, abs(ref.norm_log))
+        assert np.max(np.abs(sv - rv)) < 1e-10 * np.max(np.abs(rv))
+    else:
+        # complex-FP32 contraction on MFMA (the chi 65..128 class's block kernel)
+        fid = abs(np.vdot(sv, rv)) ** 2 / (np.vdot(sv, sv).real * np.vdot(rv, rv).real)
+        assert 1 - fid < 1e-6
+        assert abs(host.norm_log - ref.norm_log) < 1e-6 * max(1.0, abs(ref.norm_log))
 
 
 @pytest.mark.parametrize('chi', [8, 16, 32, 64])

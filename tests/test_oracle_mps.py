@@ -242,3 +242,39 @@ def test_product_z_expectations_match_oracle():
         m.sites = [s.copy() for s in sites]
         got = z_expectations(HostMPS(sites, 0.0))
         assert np.max(np.abs(got - m.z_expectations())) < 1e-12
+
+
+def test_oracle_zz_and_product_overlaps_match_dense_statevector():
+    """zz_neighbours, local_expectations and product_overlap_log (the quantities the
+    1000-qubit fixture records) against the dense statevector at 10 qubits."""
+    from qsim_amd import workloads
+    n = 10
+    m = mps_ref.MPS(n, 16, 1e-12)
+    for _, i, g in workloads.brickwall_gates(n, 6, 3, 'haar'):
+        m.apply_adjacent_2q(i, g)
+    sv = m.get_statevector()
+    idx = np.arange(sv.size)
+    p = np.abs(sv) ** 2
+    zq = lambda q: 1 - 2 * ((idx >> q) & 1)
+    zz = [np.sum(p * zq(q) * zq(q + 1)) / p.sum() for q in range(n - 1)]
+    z = [np.sum(p * zq(q)) / p.sum() for q in range(n)]
+    assert np.max(np.abs(mps_ref.zz_neighbours(m) - zz)) < 1e-13
+    assert np.max(np.abs(mps_ref.local_expectations(m, mps_ref.FIXED_GATES['z']) - z)) < 1e-13
+    for seed in (1, 2):
+        phi = mps_ref.random_product_state(n, seed)
+        full = phi[0]
+        for q in range(1, n):
+            full = np.kron(phi[q], full)              # qubit q on index bit q
+        ov = np.vdot(full, sv)
+        lg, ph = mps_ref.product_overlap_log(m, phi)
+        assert abs(lg - np.log(abs(ov))) < 1e-12 and abs(np.angle(np.exp(1j * (ph - np.angle(ov))))) < 1e-12
+
+
+def test_c3_fixture_is_consistent():
+    import os
+    case = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden',
+                                'c3_1000q_chi64.npz'))
+    b = case['bond_dims_per_layer']
+    assert b.shape == (40, 999) and b.max() == 64 and b.min() >= 1 and b[-1].min() >= 2
+    assert np.all(np.abs(case['z']) <= 1) and np.all(np.abs(case['zz']) <= 1)
+    assert np.all(np.diff(case['norm_log_per_layer']) > 0)     # truncation only adds weight
