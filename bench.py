@@ -168,6 +168,41 @@ def write_pass_table(path: str, sv: dict, cmd: str) -> None:
         f.write('\n'.join(lines) + '\n')
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (first `model name` line of /proc/cpuinfo)."""
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_omp_leg(nlayer_gates: int = 60):
+    """The oracle's C/OpenMP pair updates (oracle/sv_kernel_omp.c) on a bounded sample of
+    the 28-qubit workload: the first `nlayer_gates` gates of the layered part (U3s and the
+    CZ ladder), plus four QFT controlled phases, lowered as the reference lowers them."""
+    import numpy as np
+    from oracle import sv_omp, sv_ref
+    from qsim_amd import workloads
+    gates = workloads.sv_benchmark_circuit(SV_QUBITS, SV_DEPTH, SV_SEED)
+    sample = gates[:nlayer_gates] + [g for g in gates if g[0] == 'cp'][:4]
+    ops = sv_ref.lower(sample)
+    re = np.zeros(1 << SV_QUBITS)
+    im = np.zeros(1 << SV_QUBITS)
+    re[0] = 1.0
+    t0 = time.perf_counter()
+    sv_omp.apply_kernel_ops(re, im, SV_QUBITS, ops)
+    dt = time.perf_counter() - t0
+    nthr = sv_omp.threads()
+    return {'value': len(sample) / dt, 'unit': 'gates/s', 'cores': nthr, 'kind': 'port',
+            'sample': f'{len(sample)} logical gates ({len(ops)} kernel ops after the reference '
+                      f'lowering) of the 28q workload, C/OpenMP pair updates on split re/im '
+                      f'arrays (oracle/sv_kernel_omp.c), {nthr} threads, {dt:.1f} s'}
+
+
 def cpu_leg(ngates: int):
     """NumPy pair updates of the oracle on the first `ngates` logical gates at 28 qubits."""
     import numpy as np
@@ -189,7 +224,7 @@ def cpu_leg(ngates: int):
                       f'{dt:.1f} s'}
 
 
-def cpu_mps_leg(nsample: int = 200, layer: int = 30):
+def cpu_mps_leg(nsample: int = 500, layer: int = 30):
     """The MPS oracle (NumPy/SciPy restatement of mps_core) on the host: `nsample`
     two-site updates of layer `layer` of the chi=64 workload, starting from the state the
     GPU's FP64 path reaches after the earlier layers (copied to the host)."""
@@ -208,23 +243,50 @@ def cpu_mps_leg(nsample: int = 200, layer: int = 30):
     ref = mps_ref.MPS(mps_bench.N_SITES, mps_bench.CHI, mps_bench.CUTOFF)
     ref.sites = host.sites
     sample = layers[layer][:nsample]
-    t0 = time.perf_counter()
-    for i, g in sample:
-        ref.apply_adjacent_2q(i, g)
-    dt = time.perf_counter() - t0
+    import multiprocessing as mp
+    from threadpoolctl import threadpool_limits
+    # 128 x 128 SVDs run fastest with one BLAS thread each; the updates of one layer are
+    # independent (disjoint bonds), so the parallel row hands one update at a time to a
+    # pool of worker processes (one BLAS thread each), the two sites travelling with it
+    nproc = min(16, os.cpu_count() or 1)
+    items = [(ref.sites[i], ref.sites[i + 1], g, mps_bench.CHI, mps_bench.CUTOFF)
+             for i, g in sample]
+    saved = {k: os.environ.get(k) for k in ('OPENBLAS_NUM_THREADS', 'OMP_NUM_THREADS')}
+    os.environ.update({'OPENBLAS_NUM_THREADS': '1', 'OMP_NUM_THREADS': '1'})
+    try:
+        with mp.get_context('spawn').Pool(nproc) as pool:
+            pool.map(mps_ref.update_pair, items[:nproc], chunksize=1)     # warm-up
+            t0 = time.perf_counter()
+            pool.map(mps_ref.update_pair, items, chunksize=1)
+            dtp = time.perf_counter() - t0
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        for i, g in sample:
+            ref.apply_adjacent_2q(i, g)
+        dt = time.perf_counter() - t0
+    nthr = nproc
     # host sampling of the same chain (right environments + `shots` shots)
     shots = 4
     t1 = time.perf_counter()
     ref.sample(shots, None, np.random.default_rng(mps_bench.SEED))
     ts = time.perf_counter() - t1
-    threads = os.environ.get('OMP_NUM_THREADS') or os.environ.get('OPENBLAS_NUM_THREADS')
-    return {'value': len(sample) / dt, 'unit': 'updates/s',
+    return {'value': len(sample) / dtp, 'unit': 'updates/s',
+            'cores': nthr, 'kind': 'port',
+            'sample': f'{len(sample)} two-site updates (chi 64, layer {layer}) of the '
+                      f'1000-qubit workload, NumPy/SciPy (LAPACK gesdd), one update per '
+                      f'task over {nthr} worker processes, one BLAS thread each, {dtp:.2f} s',
+            'single_thread': {'value': len(sample) / dt, 'unit': 'updates/s', 'cores': 1,
+                              'sample': f'the same updates in order on one thread, {dt:.1f} s'},
             'sampling_shots_per_s': shots / ts,
             'sampling_sample': f'{shots} shots of the 1000-qubit chi 64 chain, right '
-                               f'environments included, {ts:.1f} s',
-            'cores': int(threads) if threads else os.cpu_count(), 'kind': 'port',
-            'sample': f'{len(sample)} two-site updates (chi 64, layer {layer}) of the '
-                      f'1000-qubit workload, NumPy/SciPy (LAPACK gesdd), {dt:.1f} s'}
+                               f'environments included (the reference einsum formulation), '
+                               f'{ts:.1f} s'}
 
 
 def traffic_from_profiles():
@@ -306,6 +368,9 @@ def main():
                 f'full list: python bench.py --multi 256')
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_leg(args.cpu_gates)
+        out['cpu_baseline']['cpu_model'] = cpu_model()
+        out['cpu_baseline']['host_cpus'] = os.cpu_count()
+        out['cpu_baseline']['openmp'] = cpu_omp_leg()
         if not args.no_mps and 'mps' in out:
             out['cpu_baseline']['mps'] = cpu_mps_leg()
     print(json.dumps(out))

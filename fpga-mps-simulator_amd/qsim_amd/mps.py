@@ -509,31 +509,3 @@ def overlap(bra, ket) -> complex:
         t = np.tensordot(env, b, axes=([1], [0]))              # (x, s, q)
         env = np.tensordot(a.conj(), t, axes=([0, 1], [0, 1]))  # (p, q)
     return complex(env[0, 0]) * math.exp(bra.norm_log + ket.norm_log)
-
-
-def smoke_check() -> None:
-    """One small brick-wall run on cuda:0 against the MPS oracle (used by smoke())."""
-    from oracle import mps_ref
-    from . import workloads
-    n, depth, chi = 10, 6, 16
-    gates = workloads.brickwall_gates(n, depth, seed=3, kind='haar')
-    ch = MPSChain(n, chi, 1e-12, precision='fp64')
-    ref = mps_ref.MPS(n, chi, 1e-12)
-    layer = None
-    batch = []
-    for lay, i, g in gates:
-        if lay != layer and batch:
-            ch.apply_2q_batch(batch)
-            batch = []
-        layer = lay
-        batch.append((i, g))
-        ref.apply_adjacent_2q(i, g)
-    ch.apply_2q_batch(batch)
-    ch.check_status()
-    got = ch.to_host()
-    sv_g, sv_r = got.get_statevector(), ref.get_statevector()
-    err = float(np.max(np.abs(sv_g - sv_r)))
-    print(f'smoke: MPS {n}q depth {depth} chi {chi}: bonds {got.bond_dims()} '
-          f'max |diff| of statevector vs oracle = {err:.3e}')
-    assert got.bond_dims() == ref.bond_dims()
-    assert err < 1e-10 * max(1.0, float(np.max(np.abs(sv_r))))

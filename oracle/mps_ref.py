@@ -629,6 +629,18 @@ def run_gate_list(n: int, gates, chi_max: int, cutoff: float) -> MPS:
     return m
 
 
+def update_pair(args):
+    """One two-site update of an adjacent pair given on its own: (A, B, gate, chi_max,
+    cutoff) -> (A', B', norm_log increment), as MPS.apply_adjacent_2q does it in a chain.
+    The work item of the process-parallel CPU baseline (the updates of one brick-wall
+    layer are independent)."""
+    a, b, g, chi_max, cutoff = args
+    two = MPS(2, chi_max, cutoff)
+    two.sites = [a, b]
+    two.apply_adjacent_2q(0, g)
+    return two.sites[0], two.sites[1], two.norm_log
+
+
 # ----------------------------------------------------------------------------
 # Workload generators (inputs only; shared by tests and bench)
 # ----------------------------------------------------------------------------

@@ -133,3 +133,24 @@ def test_reset_fallback_and_clbit_keys():
     assert abs(st[0] - 1) < 1e-15
     assert sv_ref.clbit_key(1, 1, 3, [(0, 0), (0, 1)]) == '011'
     assert sv_ref.clbit_key(1, 1, 3, [(0, 2)]) == '100'
+
+
+def test_openmp_pair_updates_match_numpy_restatement():
+    """oracle/sv_kernel_omp.c (the C/OpenMP CPU baseline) against the NumPy pair updates
+    on the lowered benchmark structure and random gates."""
+    from oracle import sv_omp
+    from qsim_amd import workloads
+    nq = 12
+    rng = np.random.default_rng(4)
+    gates = workloads.u3_cz_layers(nq, 3, seed=2) + workloads.qft_gates(nq)
+    gates += [('cx', [int(a) for a in rng.choice(nq, 2, replace=False)]) for _ in range(20)]
+    ops = sv_ref.lower(gates)
+    ref = np.zeros(1 << nq, dtype=np.complex128)
+    ref[0] = 1.0
+    sv_ref.apply_kernel_ops(ref, nq, ops)
+    re = np.zeros(1 << nq)
+    im = np.zeros(1 << nq)
+    re[0] = 1.0
+    sv_omp.apply_kernel_ops(re, im, nq, ops)
+    assert np.max(np.abs(re + 1j * im - ref)) < 1e-13
+    assert sv_omp.threads() >= 1
