@@ -289,6 +289,92 @@ def cpu_mps_leg(nsample: int = 500, layer: int = 30):
                                f'{ts:.1f} s'}
 
 
+def gradients_leg():
+    """C5 gradients on the GPU: seconds per full gradient of the TFIM energy (J = h = 1,
+    open chain) of the hardware-efficient ansatz (4 x [Ry all, Rz all, CZ ladder]), params
+    ~ U[-pi, pi]: the statevector adjoint at 24 qubits (192 parameters) and the MPS reverse
+    sweep at 200 qubits, chi_max 32 (1600 parameters)."""
+    import numpy as np
+    import torch
+    from qsim_amd.adjoint import TFIMEnergy, hardware_efficient_ansatz
+    from qsim_amd.mps_grad import MPSEnergyGrad
+    rng = np.random.default_rng(2024)
+    out = {}
+    c = hardware_efficient_ansatz(24, 4)
+    th = rng.uniform(-math.pi, math.pi, c.nparams)
+    mod = TFIMEnergy(c)
+    mod.energy_and_grad(th)                       # warm-up (allocation, kernel load)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        e, g = mod.energy_and_grad(th)
+    torch.cuda.synchronize()
+    out['sv_24q'] = {'params': c.nparams, 'seconds_per_gradient': (time.perf_counter() - t0) / reps,
+                     'energy': e, 'method': 'adjoint reverse sweep (one forward, two reverse '
+                                             'passes, one Pauli reduction per rotation block)'}
+    del mod
+    c2 = hardware_efficient_ansatz(200, 4)
+    th2 = rng.uniform(-math.pi, math.pi, c2.nparams)
+    eng = MPSEnergyGrad(c2, chi_max=32, cutoff=1e-12)
+    eng.energy_and_grad(th2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e2, g2 = eng.energy_and_grad(th2)
+    torch.cuda.synchronize()
+    out['mps_200q_chi32'] = {'params': c2.nparams,
+                             'seconds_per_gradient': time.perf_counter() - t0, 'energy': e2,
+                             'method': 'GPU forward with tape + reverse sweep through the '
+                                       'contractions and truncated SVDs'}
+    return out
+
+
+def cpu_gradients_leg():
+    """The oracle's finite-difference gradient cost for the same C5 circuits, extrapolated
+    from a bounded sample: central differences need 2 P energy evaluations; one energy is
+    timed on a sample of the circuit (the first gates) and scaled to the whole circuit."""
+    import numpy as np
+    from oracle import mps_ref, sv_ref
+    rng = np.random.default_rng(2024)
+    out = {}
+    n, layers = 24, 4
+    th = rng.uniform(-math.pi, math.pi, 2 * n * layers)
+    gates = sv_ref.hea_gates(n, th, layers)
+    ns = 30
+    state = np.zeros(1 << n, dtype=np.complex128)
+    state[0] = 1.0
+    t0 = time.perf_counter()
+    sv_ref.apply_kernel_ops(state, n, sv_ref.lower(gates[:ns]))
+    t_gates = (time.perf_counter() - t0) * len(gates) / ns
+    t0 = time.perf_counter()
+    sv_ref.tfim_energy_sv(state)
+    t_energy = t_gates + time.perf_counter() - t0
+    P = th.size
+    out['sv_24q'] = {'seconds_per_gradient': 2 * P * t_energy, 'cores': 1, 'kind': 'port',
+                     'sample': f'{ns} of {len(gates)} gates at 24 qubits (NumPy pair updates) '
+                               f'and one TFIM energy timed, {t_energy:.1f} s per energy '
+                               f'extrapolated to 2 x {P} central-difference evaluations'}
+    n2 = 200
+    th2 = rng.uniform(-math.pi, math.pi, 2 * n2 * layers)
+    m = mps_ref.MPS(n2, 32, 1e-12)
+    t0 = time.perf_counter()
+    for name, qs, ps in sv_ref.hea_gates(n2, th2, layers):
+        if len(qs) == 1:
+            m.apply_1q(qs[0], mps_ref.gate_1q(name, ps))
+        else:
+            m.apply_adjacent_2q(qs[0], mps_ref.gate_2q(name, ps))
+    mps_ref.tfim_energy(m)
+    t_energy2 = time.perf_counter() - t0
+    P2 = th2.size
+    out['mps_200q_chi32'] = {'seconds_per_gradient': 2 * P2 * t_energy2, 'cores': 1,
+                             'kind': 'port',
+                             'sample': f'one energy of the 200-qubit chi_max 32 ansatz on the '
+                                       f'MPS oracle (forward + TFIM energy), {t_energy2:.1f} s, '
+                                       f'extrapolated to 2 x {P2} central-difference '
+                                       f'evaluations'}
+    return out
+
+
 def traffic_from_profiles():
     """HBM bytes per pass launch from the committed rocprofv3 PMC summary, if present
     (the line `hbm_bytes_per_launch: N` of profiles/sv_pass_28q_traffic.md)."""
@@ -366,6 +452,8 @@ def main():
             out['mps']['multi']['sample'] = (
                 f'the first {args.multi_sample} circuits of the seeded 256-circuit list; the '
                 f'full list: python bench.py --multi 256')
+    if not args.no_mps:
+        out['gradients'] = gradients_leg()
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_leg(args.cpu_gates)
         out['cpu_baseline']['cpu_model'] = cpu_model()
@@ -373,6 +461,9 @@ def main():
         out['cpu_baseline']['openmp'] = cpu_omp_leg()
         if not args.no_mps and 'mps' in out:
             out['cpu_baseline']['mps'] = cpu_mps_leg()
+            from threadpoolctl import threadpool_limits
+            with threadpool_limits(limits=1):
+                out['cpu_baseline']['gradients'] = cpu_gradients_leg()
     print(json.dumps(out))
 
 

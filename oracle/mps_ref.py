@@ -460,22 +460,22 @@ def tfim_energy(m: MPS, j_coupling: float = 1.0, h_field: float = 1.0) -> float:
     z, x = FIXED_GATES['z'], FIXED_GATES['x']
     left = [np.ones((1, 1), dtype=np.complex128)]
     for a in m.sites:
-        left.append(np.einsum('xy,xsa,ysb->ab', left[-1], a.conj(), a))
+        left.append(np.einsum('xy,xsa,ysb->ab', left[-1], a.conj(), a, optimize=True))
     right = [None] * (n + 1)
     right[n] = np.ones((1, 1), dtype=np.complex128)
     for i in range(n - 1, -1, -1):
         a = m.sites[i]
-        right[i] = np.einsum('xsa,ysb,ab->xy', a.conj(), a, right[i + 1])
+        right[i] = np.einsum('xsa,ysb,ab->xy', a.conj(), a, right[i + 1], optimize=True)
     nrm = np.real(left[n][0, 0])
     e = 0.0
     for i in range(n):
         a = m.sites[i]
-        e -= h_field * np.real(np.einsum('xy,xsa,st,ytb,ab->', left[i], a.conj(), x, a, right[i + 1]))
+        e -= h_field * np.real(np.einsum('xy,xsa,st,ytb,ab->', left[i], a.conj(), x, a, right[i + 1], optimize=True))
     for i in range(n - 1):
         a, b = m.sites[i], m.sites[i + 1]
-        env = np.einsum('xy,xsa,st,ytb->ab', left[i], a.conj(), z, a)
-        env = np.einsum('xy,xsa,st,ytb->ab', env, b.conj(), z, b)
-        e -= j_coupling * np.real(np.einsum('ab,ab->', env, right[i + 2]))
+        env = np.einsum('xy,xsa,st,ytb->ab', left[i], a.conj(), z, a, optimize=True)
+        env = np.einsum('xy,xsa,st,ytb->ab', env, b.conj(), z, b, optimize=True)
+        e -= j_coupling * np.real(np.einsum('ab,ab->', env, right[i + 2], optimize=True))
     return float(e / nrm)
 
 
