@@ -201,6 +201,135 @@ __global__ void mps_ratio_kernel(const cd2* __restrict__ num, const cd2* __restr
     if (i < count) out[i] = den[i].x != 0.0 ? num[i].x / den[i].x : 0.0;
 }
 
+// Eigenvalues of the Hermitian (dim x dim) matrix H (row stride ld, in global memory,
+// overwritten), dim <= kMaxChi, by cyclic two-sided Jacobi: each round rotates dim/2
+// disjoint index pairs of a round-robin tournament (a dummy index pads odd dims).  Pair
+// (p, q) with H[p][q] = |h| e^{i phi} uses J = diag(1, e^{-i phi}) R, R the real rotation
+// [[c, s], [-s, c]] that diagonalises [[a, |h|], [|h|, b]] (t = sign(tau) / (|tau| +
+// sqrt(1 + tau^2)), tau = (b - a) / (2|h|)); disjoint rotations commute, so a round
+// applies every column rotation and then every row rotation.  A pair is skipped when
+// |h| <= 1e-15 sqrt(|a b|) or |h| <= 1e-300; sweeps end when a sweep skips every pair.
+__device__ void wg_hermitian_eigenvalues(cd2* H, int dim, int ld, double* evals,
+                                         double* sh_c, double* sh_s, cd2* sh_e, int* sh_p,
+                                         int* sh_q, int* sh_any) {
+    const int tid = threadIdx.x;
+    const int np = (dim + 1) & ~1;                    // players incl. the dummy
+    const int half = np / 2;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        if (tid == 0) *sh_any = 0;
+        __syncthreads();
+        for (int r = 0; r < np - 1; ++r) {
+            if (tid < half) {
+                // round-robin: player 0 fixed, the others rotate by r
+                auto player = [&](int k) { return k == 0 ? 0 : 1 + (k - 1 + r) % (np - 1); };
+                int p = player(tid), q = player(np - 1 - tid);
+                if (p > q) { const int x = p; p = q; q = x; }
+                double c = 1.0, sn = 0.0;
+                cd2 e = mk2(1.0, 0.0);
+                if (q < dim) {
+                    const double a = H[(size_t)p * ld + p].x, b = H[(size_t)q * ld + q].x;
+                    const cd2 h = H[(size_t)p * ld + q];
+                    const double g = hypot(h.x, h.y);
+                    if (g > 1e-300 && g > 1e-15 * sqrt(fabs(a * b))) {
+                        const double tau = (b - a) / (2.0 * g);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        sn = t * c;
+                        e = mk2(h.x / g, -h.y / g);          // e^{-i phi}
+                        *sh_any = 1;
+                    }
+                }
+                sh_c[tid] = c;
+                sh_s[tid] = sn;
+                sh_e[tid] = e;
+                sh_p[tid] = p;
+                sh_q[tid] = q;
+            }
+            __syncthreads();
+            // columns: col p <- c col p - s e col q,  col q <- s col p + c e col q
+            for (int idx = tid; idx < dim * half; idx += blockDim.x) {
+                const int row = idx / half, k = idx - row * half;
+                const int p = sh_p[k], q = sh_q[k];
+                if (q >= dim || sh_s[k] == 0.0) continue;
+                const double c = sh_c[k], sn = sh_s[k];
+                const cd2 e = sh_e[k];
+                const cd2 x = H[(size_t)row * ld + p], y = H[(size_t)row * ld + q];
+                const cd2 ey = mk2(e.x * y.x - e.y * y.y, e.x * y.y + e.y * y.x);
+                H[(size_t)row * ld + p] = mk2(c * x.x - sn * ey.x, c * x.y - sn * ey.y);
+                H[(size_t)row * ld + q] = mk2(sn * x.x + c * ey.x, sn * x.y + c * ey.y);
+            }
+            __syncthreads();
+            // rows: row p <- c row p - s conj(e) row q,  row q <- s row p + c conj(e) row q
+            for (int idx = tid; idx < dim * half; idx += blockDim.x) {
+                const int col = idx / half, k = idx - col * half;
+                const int p = sh_p[k], q = sh_q[k];
+                if (q >= dim || sh_s[k] == 0.0) continue;
+                const double c = sh_c[k], sn = sh_s[k];
+                const cd2 ec = mk2(sh_e[k].x, -sh_e[k].y);
+                const cd2 x = H[(size_t)p * ld + col], y = H[(size_t)q * ld + col];
+                const cd2 ey = mk2(ec.x * y.x - ec.y * y.y, ec.x * y.y + ec.y * y.x);
+                H[(size_t)p * ld + col] = mk2(c * x.x - sn * ey.x, c * x.y - sn * ey.y);
+                H[(size_t)q * ld + col] = mk2(sn * x.x + c * ey.x, sn * x.y + c * ey.y);
+            }
+            __syncthreads();
+        }
+        const int any = *sh_any;
+        __syncthreads();
+        if (!any) break;
+    }
+    for (int k = tid; k < dim; k += blockDim.x) evals[k] = H[(size_t)k * ld + k].x;
+    __syncthreads();
+}
+
+// Entanglement entropy of bond b = bonds[blockIdx.x] by the reference's definition
+// (mps_core.py:465-483): the singular values S of the sites 0..b contracted into one
+// (2^(b+1) x chi) matrix M, S > 1e-15 kept, p = S^2 / sum S^2, -sum p log p.  S^2 are
+// the eigenvalues of the Gram matrix M^H M = tr(G) L_{b+1}, with L_{b+1} the
+// trace-normalised left environment and log tr(G) = sum_{j <= b+1} log_traces[j] (the
+// traces the environment recursion divided out).  The environment is copied into the
+// workspace slot of the workgroup and diagonalised there.
+__global__ void __launch_bounds__(kThreads)
+mps_entropy_kernel(const int32_t* __restrict__ bond, const cd2* __restrict__ lenvs, int ld,
+                   const double* __restrict__ log_traces, const int32_t* __restrict__ bonds,
+                   cd2* __restrict__ work, double* __restrict__ evals_out,
+                   double* __restrict__ entropy) {
+    __shared__ double sh_c[kMaxChi / 2 + 1], sh_s[kMaxChi / 2 + 1];
+    __shared__ cd2 sh_e[kMaxChi / 2 + 1];
+    __shared__ int sh_p[kMaxChi / 2 + 1], sh_q[kMaxChi / 2 + 1], sh_any;
+    __shared__ double red[kThreads];
+    const int b = bonds[blockIdx.x];
+    const int dim = bond[b + 1];
+    const cd2* L = lenvs + (size_t)(b + 1) * ld * ld;
+    cd2* H = work + (size_t)blockIdx.x * ld * ld;
+    double* ev = evals_out + (size_t)blockIdx.x * ld;
+    for (int idx = threadIdx.x; idx < dim * dim; idx += kThreads) {
+        const int r = idx / dim, c = idx - r * dim;
+        H[(size_t)r * ld + c] = L[(size_t)r * ld + c];
+    }
+    __syncthreads();
+    wg_hermitian_eigenvalues(H, dim, ld, ev, sh_c, sh_s, sh_e, sh_p, sh_q, &sh_any);
+    double lt = 0.0;
+    for (int j = 0; j <= b + 1; ++j) lt += log_traces[j];
+    // S_k^2 = tr(G) lambda_k; keep S_k > 1e-15, i.e. log lambda_k + lt > log 1e-30
+    const double floor_log = -69.07755278982137;      // log(1e-30)
+    double part = 0.0;
+    for (int k = threadIdx.x; k < dim; k += kThreads) {
+        const double l = ev[k];
+        if (l > 0.0 && log(l) + lt > floor_log) part += l;
+    }
+    const double tot = block_sum(part, red);
+    double ent = 0.0;
+    for (int k = threadIdx.x; k < dim; k += kThreads) {
+        const double l = ev[k];
+        if (l > 0.0 && log(l) + lt > floor_log) {
+            const double p = l / tot;
+            ent -= p * log(fmax(p, 1e-300));
+        }
+    }
+    ent = block_sum(ent, red);
+    if (threadIdx.x == 0) entropy[blockIdx.x] = ent;
+}
+
 int check_chain_args(const void* sites, const int32_t* bond, int64_t stride, int n,
                      int chi_max) {
     QS_REQUIRE(sites != nullptr && bond != nullptr, QS_E_ARG, "null sites or bond array");
@@ -241,7 +370,8 @@ using namespace qs;
 extern "C" {
 
 int qs_mps_left_envs(const double* sites, const int32_t* bond, int64_t site_stride, int n,
-                     int chi_max, double* envs, size_t env_bytes, void* stream) {
+                     int chi_max, double* envs, size_t env_bytes, double* log_traces,
+                     void* stream) {
     clear_error();
     int st = check_chain_args(sites, bond, site_stride, n, chi_max);
     if (st) return st;
@@ -257,13 +387,52 @@ int qs_mps_left_envs(const double* sites, const int32_t* bond, int64_t site_stri
     QS_HIP(hipGetLastError());
     for (int i = 0; i < n; ++i) {
         st = transfer(S, bond, site_stride, S, bond, site_stride, i, 1, E + i * ld2, 0,
-                      E + (i + 1) * ld2, 0, chi_max, nullptr, 1, nullptr, hs);
+                      E + (i + 1) * ld2, 0, chi_max, nullptr, 1, log_traces, hs);
         if (st) return st;
     }
     hipLaunchKernelGGL(mps_trace_normalise_kernel, dim3(n), dim3(kThreads), 0, hs, bond, 1, n,
                        chi_max, E);
     QS_HIP(hipGetLastError());
     return QS_OK;
+}
+
+size_t qs_mps_entropy_work_bytes(int nbonds, int chi_max) {
+    if (nbonds < 1 || chi_max < 1) return 0;
+    return (size_t)nbonds * chi_max * chi_max * sizeof(cd2) +
+           (size_t)nbonds * chi_max * sizeof(double) + (size_t)nbonds * sizeof(int32_t);
+}
+
+int qs_mps_entanglement_entropy(const int32_t* bond, int n, int chi_max, const double* lenvs,
+                                const double* log_traces, const int32_t* bonds, int nbonds,
+                                double* entropy, void* work, size_t work_bytes,
+                                void* stream) {
+    clear_error();
+    QS_REQUIRE(bond && lenvs && log_traces && entropy && work, QS_E_ARG,
+               "null pointer argument");
+    QS_REQUIRE(n >= 2 && chi_max >= 1 && chi_max <= kMaxChi, QS_E_ARG,
+               "chain of %d sites, chi_max %d", n, chi_max);
+    QS_REQUIRE(nbonds >= 0 && (nbonds == 0 || bonds), QS_E_ARG, "bad bond list");
+    for (int i = 0; i < nbonds; ++i)
+        QS_REQUIRE(bonds[i] >= 0 && bonds[i] < n - 1, QS_E_ARG,
+                   "bond %d out of range [0, %d]", bonds[i], n - 2);
+    if (nbonds == 0) return QS_OK;
+    QS_REQUIRE(work_bytes >= qs_mps_entropy_work_bytes(nbonds, chi_max), QS_E_RANGE,
+               "work buffer %zu bytes < %zu needed", work_bytes,
+               qs_mps_entropy_work_bytes(nbonds, chi_max));
+    hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
+    cd2* H = static_cast<cd2*>(work);
+    double* ev = reinterpret_cast<double*>(H + (size_t)nbonds * chi_max * chi_max);
+    int32_t* dbonds = reinterpret_cast<int32_t*>(ev + (size_t)nbonds * chi_max);
+    hipEvent_t copied = nullptr;
+    int st = h2d_copy(dbonds, bonds, (size_t)nbonds * sizeof(int32_t), hs, &copied);
+    if (st) return st;
+    hipLaunchKernelGGL(mps_entropy_kernel, dim3(nbonds), dim3(kThreads), 0, hs, bond,
+                       reinterpret_cast<const cd2*>(lenvs), chi_max, log_traces, dbonds, H, ev,
+                       entropy);
+    const hipError_t le = hipGetLastError();
+    const int wt = h2d_copy_wait(copied);
+    QS_REQUIRE(le == hipSuccess, QS_E_HIP, "entropy launch failed: %s", hipGetErrorString(le));
+    return wt;
 }
 
 size_t qs_mps_expect_work_bytes(int chi_max) {

@@ -107,7 +107,13 @@ _PROTOS = {
                                          ctypes.c_size_t, ctypes.c_void_p]),
     'qs_mps_left_envs': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
-                                        ctypes.c_size_t, ctypes.c_void_p]),
+                                        ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]),
+    'qs_mps_entropy_work_bytes': (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
+    'qs_mps_entanglement_entropy': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_void_p, ctypes.c_void_p,
+                                                   _c_int32_p, ctypes.c_int, ctypes.c_void_p,
+                                                   ctypes.c_void_p, ctypes.c_size_t,
+                                                   ctypes.c_void_p]),
     'qs_mps_expect_work_bytes': (ctypes.c_size_t, [ctypes.c_int]),
     'qs_mps_expect': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_void_p,

@@ -222,7 +222,11 @@ class MPSChain:
         return out.cpu().numpy()
 
     def to_host(self) -> 'HostMPS':
-        return HostMPS(self.site_tensors(), self.norm_log, self.chi_max, self.cutoff)
+        """Host copy of the chain; it keeps a handle on this device chain, so its
+        read-outs run on the device until the copy is modified."""
+        h = HostMPS(self.site_tensors(), self.norm_log, self.chi_max, self.cutoff)
+        h.device_chain = self
+        return h
 
     @classmethod
     def from_host(cls, host: 'HostMPS', device: int = 0, precision: str = 'fp64') -> 'MPSChain':
@@ -287,22 +291,69 @@ class MPSChain:
         return c
 
     # -- transfer-matrix read-outs on the device ---------------------------------
+    def _left_envs(self):
+        import torch
+        nbytes = _lib.lib().qs_mps_env_bytes(self.n, self.chi_max)
+        left = torch.empty(nbytes // 16, dtype=torch.complex128, device=self.sites.device)
+        logs = torch.zeros(self.n, dtype=torch.float64, device=self.sites.device)
+        _lib.call('qs_mps_left_envs', ctypes.c_void_p(self.sites.data_ptr()),
+                  ctypes.c_void_p(self.bond.data_ptr()), self.stride, self.n, self.chi_max,
+                  ctypes.c_void_p(left.data_ptr()), nbytes, ctypes.c_void_p(logs.data_ptr()),
+                  self._stream())
+        return left, logs
+
     def environments(self):
         """(left, right) environments of <psi|psi> on the device, each trace-normalised
         (qs_mps_left_envs / qs_mps_right_envs)."""
         import torch
-        lib = _lib.lib()
-        nbytes = lib.qs_mps_env_bytes(self.n, self.chi_max)
-        dev = self.sites.device
-        left = torch.empty(nbytes // 16, dtype=torch.complex128, device=dev)
-        right = torch.empty(nbytes // 16, dtype=torch.complex128, device=dev)
-        args = (ctypes.c_void_p(self.sites.data_ptr()), ctypes.c_void_p(self.bond.data_ptr()),
-                self.stride, self.n, self.chi_max)
-        _lib.call('qs_mps_left_envs', *args, ctypes.c_void_p(left.data_ptr()), nbytes,
-                  self._stream())
-        _lib.call('qs_mps_right_envs', *args, ctypes.c_void_p(right.data_ptr()), nbytes,
-                  self._stream())
+        nbytes = _lib.lib().qs_mps_env_bytes(self.n, self.chi_max)
+        left, _ = self._left_envs()
+        right = torch.empty(nbytes // 16, dtype=torch.complex128, device=self.sites.device)
+        _lib.call('qs_mps_right_envs', ctypes.c_void_p(self.sites.data_ptr()),
+                  ctypes.c_void_p(self.bond.data_ptr()), self.stride, self.n, self.chi_max,
+                  ctypes.c_void_p(right.data_ptr()), nbytes, self._stream())
         return left, right
+
+    def entanglement_entropies(self, bonds: Optional[Sequence[int]] = None) -> np.ndarray:
+        """Entanglement entropy of each bond (all by default) by the reference's definition
+        (mps_core.py:465-483), computed on the device from the left Gram environments
+        (qs_mps_entanglement_entropy)."""
+        import torch
+        bonds = list(range(self.n - 1)) if bonds is None else [int(b) for b in bonds]
+        for b in bonds:
+            if not 0 <= b < self.n - 1:
+                raise ValueError(f"bond {b} out of range [0, {self.n - 2}]")
+        if not bonds:
+            return np.zeros(0)
+        left, logs = self._left_envs()
+        out = torch.zeros(len(bonds), dtype=torch.float64, device=self.sites.device)
+        nbytes = _lib.lib().qs_mps_entropy_work_bytes(len(bonds), self.chi_max)
+        work = torch.empty(nbytes, dtype=torch.uint8, device=self.sites.device)
+        arr = (ctypes.c_int32 * len(bonds))(*bonds)
+        _lib.call('qs_mps_entanglement_entropy', ctypes.c_void_p(self.bond.data_ptr()), self.n,
+                  self.chi_max, ctypes.c_void_p(left.data_ptr()),
+                  ctypes.c_void_p(logs.data_ptr()), arr, len(bonds),
+                  ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(work.data_ptr()), nbytes,
+                  self._stream())
+        return out.cpu().numpy()
+
+    def entanglement_entropy(self, bond: int) -> float:
+        return float(self.entanglement_entropies([bond])[0])
+
+    def bond_spectrum(self, bond: int) -> np.ndarray:
+        """The reference's bond_spectrum (mps_core.py:485-494): singular values of site
+        `bond` after sites 0..bond are left-normalised by QR, that site included.  A
+        left-normalised site is an isometry, so these are ones; their number is
+        min(2 chi'_l, chi_r) with chi'_l the bond dimension the earlier QRs leave
+        (chi'_0 = 1, chi'_{i+1} = min(2 chi'_i, chi_r(i))), read from the device bond
+        dimensions."""
+        if not 0 <= bond < self.n:
+            raise ValueError(f'bond {bond} outside the chain')
+        b = self.bond.cpu().numpy()
+        cl = 1
+        for i in range(bond):
+            cl = min(2 * cl, int(b[i + 1]))
+        return np.ones(min(2 * cl, int(b[bond + 1])))
 
     def _expect(self, op_a, op_b, span: int) -> np.ndarray:
         import torch
@@ -378,6 +429,9 @@ class HostMPS:
         self.norm_log = norm_log
         self.chi_max = chi_max if chi_max is not None else max([1] + [t.shape[2] for t in sites])
         self.svd_cutoff = svd_cutoff
+        # the device chain this copy was read from (MPSChain.to_host); cleared by any
+        # change to the copy, after which read-outs run on the host tensors
+        self.device_chain = None
 
     def copy(self) -> 'HostMPS':
         return HostMPS([t.copy() for t in self.sites], self.norm_log, self.chi_max,
@@ -399,6 +453,7 @@ class HostMPS:
     def left_normalize_site(self, i: int) -> None:
         """Site i as a (2 chi_l) x chi_r matrix = Q R (reduced QR): site i becomes Q and R
         is absorbed into site i + 1 (mps_core.py:221-234)."""
+        self.device_chain = None
         a = self.sites[i]
         cl, d, cr = a.shape
         q, r = np.linalg.qr(a.reshape(cl * d, cr))
@@ -409,6 +464,7 @@ class HostMPS:
     def right_normalize_site(self, i: int) -> None:
         """Site i as a chi_l x (2 chi_r) matrix = L Q (QR of its adjoint): site i becomes Q
         and L is absorbed into site i - 1 (mps_core.py:236-251)."""
+        self.device_chain = None
         a = self.sites[i]
         cl, d, cr = a.shape
         q, r = np.linalg.qr(a.reshape(cl, d * cr).conj().T)
@@ -426,6 +482,7 @@ class HostMPS:
 
     def apply_1q(self, site: int, gate: np.ndarray) -> None:
         """A'[l, s', r] = sum_s G[s', s] A[l, s, r] (mps_core.py:267-275)."""
+        self.device_chain = None
         self.sites[site] = np.einsum('ts,lsr->ltr', np.asarray(gate, dtype=np.complex128),
                                      self.sites[site])
 
@@ -442,9 +499,13 @@ class HostMPS:
     # -- bond read-outs (mps_core.py:465-494) -----------------------------------
     def entanglement_entropy(self, bond: int) -> float:
         """-sum p log p over the squared singular values (above 1e-15, normalised) of
-        sites 0..bond contracted into a 2^(bond+1) x chi matrix (mps_core.py:465-483)."""
+        sites 0..bond contracted into a 2^(bond+1) x chi matrix (mps_core.py:465-483).
+        On the device (MPSChain.entanglement_entropies) while this copy is unchanged
+        from its device chain; otherwise from the host tensors."""
         if bond < 0 or bond >= self.n - 1:
             raise ValueError(f"bond {bond} out of range [0, {self.n - 2}]")
+        if self.device_chain is not None:
+            return self.device_chain.entanglement_entropy(bond)
         acc = self.sites[0][0]
         for t in self.sites[1:bond + 1]:
             acc = np.tensordot(acc, t, axes=([1], [0])).reshape(-1, t.shape[2])
@@ -457,7 +518,10 @@ class HostMPS:
     def bond_spectrum(self, bond: int) -> np.ndarray:
         """Singular values of site `bond` as a (2 chi_l) x chi_r matrix after sites
         0..bond of a copy are left-normalised, that site included, as the reference
-        does (mps_core.py:485-494)."""
+        does (mps_core.py:485-494).  From the device bond dimensions while this copy is
+        unchanged from its device chain (MPSChain.bond_spectrum)."""
+        if self.device_chain is not None:
+            return self.device_chain.bond_spectrum(bond)
         m = self.copy()
         for i in range(bond + 1):
             m.left_normalize_site(i)

@@ -319,7 +319,8 @@ int qs_mps_sample(const double* sites, const int32_t* bond, int64_t site_stride,
  * Every transfer is renormalised on the way (trace or entry sum), so chains of thousands
  * of sites neither overflow nor underflow. */
 int qs_mps_left_envs(const double* sites, const int32_t* bond, int64_t site_stride, int n,
-                     int chi_max, double* envs, size_t env_bytes, void* stream);
+                     int chi_max, double* envs, size_t env_bytes, double* log_traces,
+                     void* stream);
 size_t qs_mps_expect_work_bytes(int chi_max);
 int qs_mps_expect(const double* sites, const int32_t* bond, int64_t site_stride, int n,
                   int chi_max, const double* lenvs, const double* renvs, const double op_a[8],
@@ -330,6 +331,20 @@ int qs_mps_overlap(const double* bra_sites, const int32_t* bra_bond, int64_t bra
                    const double* ket_sites, const int32_t* ket_bond, int64_t ket_stride, int n,
                    int chi_max, void* work, size_t work_bytes, double out_host[2],
                    void* stream);
+
+/* Entanglement entropy on the device, by the reference's definition (mps_core.py:465-483:
+ * sites 0..b contracted into one matrix M, its singular values S > 1e-15, p = S^2 / sum
+ * S^2, -sum p log p), from the left environments and log traces of qs_mps_left_envs:
+ * S^2 are the eigenvalues of M^H M, which is L_{b+1} times the product of the traces.
+ * One workgroup per requested bond (bonds: host int32[nbonds], each in [0, n-2]) runs a
+ * cyclic Jacobi eigenvalue iteration on a copy of L_{b+1} in `work` (device memory of
+ * qs_mps_entropy_work_bytes(nbonds, chi_max)); entropy: device float64[nbonds].  The
+ * eigenvalues of a Gram matrix carry an absolute error of about 1e-16 of the largest, so
+ * entropies match the reference's SVD of M to about 1e-13. */
+size_t qs_mps_entropy_work_bytes(int nbonds, int chi_max);
+int qs_mps_entanglement_entropy(const int32_t* bond, int n, int chi_max, const double* lenvs,
+                                const double* log_traces, const int32_t* bonds, int nbonds,
+                                double* entropy, void* work, size_t work_bytes, void* stream);
 
 /* SVD tile shape, the autotuner's knob (no counterpart in the reference, whose SVD is
  * LAPACK gesdd on the host).  The register SVD of size class 8, 16, 32 or 64 runs one

@@ -181,6 +181,12 @@ def test_argument_errors_are_reported_without_device():
         t2[k].n_sites = 8
     assert lib.qs_mps_update_batch(t2, 2, 0, ctypes.c_void_p(1), 1 << 30, None, None) == _lib.QS_E_ARG
     assert b'share a site' in lib.qs_last_error()
+    # entanglement entropy: the bond list is checked on the host
+    bl = (ctypes.c_int32 * 2)(0, 7)
+    one = ctypes.c_void_p(16)
+    assert lib.qs_mps_entanglement_entropy(one, 8, 16, one, one, bl, 2, one, one, 1 << 20,
+                                           None) == _lib.QS_E_ARG
+    assert b'out of range' in lib.qs_last_error()
     # batched one-site gates and site resets: argument checks before any launch
     sl = (ctypes.c_int32 * 3)(1, 4, 1)
     mats = (ctypes.c_double * 24)()

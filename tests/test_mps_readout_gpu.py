@@ -90,3 +90,35 @@ def test_thousand_site_chain_readouts_stay_finite():
     lg, _ = a.overlap_log(a)
     assert math.isfinite(lg)
     assert abs(a.fidelity(a) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize('name', ['exact_10q', 'trunc_12q', 'c1_16q'])
+def test_device_entropy_and_spectrum_match_golden_chains(name):
+    """Entanglement entropy on the device (left Gram environments + Jacobi eigenvalues)
+    and bond_spectrum from the device bond dimensions, on the chains of the mps_core
+    golden vectors, against the oracle's restatement of mps_core (dense contraction +
+    LAPACK) and the recorded spectra."""
+    import os
+    from qsim_amd.mps import MPSChain
+    here = os.path.dirname(os.path.abspath(__file__))
+    case = np.load(os.path.join(here, 'golden', f'mps_core_{name}.npz'))
+    n = int(case['n'])
+    ch = MPSChain(n, int(case['chi_max']), float(case['cutoff']))
+    ref = mps_ref.MPS(n, int(case['chi_max']), float(case['cutoff']))
+    for g, i in zip(case['gates'], case['sites']):
+        ch.apply_2q(int(i), g)
+        ref.apply_2q(int(i), int(i) + 1, g)
+    ch.check_status()
+    got = ch.entanglement_entropies()
+    for b in range(n - 1):
+        assert abs(got[b] - ref.entanglement_entropy(b)) < 1e-10, b
+        s = ch.bond_spectrum(b)
+        assert np.allclose(s, case['bond_spectra'][b][:s.size], rtol=1e-9, atol=1e-12)
+        assert s.size == np.count_nonzero(case['bond_spectra'][b])
+
+
+def test_device_entropy_chi100_class():
+    ch, ref = _chain(14, 100, 14, seed=3)
+    got = ch.entanglement_entropies([0, 3, 6, 7, 12])
+    for b, e in zip([0, 3, 6, 7, 12], got):
+        assert abs(e - ref.entanglement_entropy(b)) < 1e-10, b
