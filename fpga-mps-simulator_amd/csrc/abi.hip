@@ -50,6 +50,46 @@ int h2d_copy_wait(hipEvent_t done) {
 }
 
 namespace {
+// Pinned staging ring of one host thread: kRing buffers, each with the event recorded
+// after the copy that last read it.  A buffer is reused only once that copy has completed,
+// so the caller's host memory is free as soon as it has been copied into the ring.  The
+// buffers are kept until the process ends (no destructor: at exit the HIP runtime may
+// already be torn down, and the driver releases pinned memory with the process).
+constexpr int kRing = 8;
+struct StagingRing {
+    void* buf[kRing] = {};
+    size_t cap[kRing] = {};
+    hipEvent_t ev[kRing] = {};
+    bool used[kRing] = {};
+    int next = 0;
+};
+thread_local StagingRing g_ring;
+}  // namespace
+
+int h2d_copy_staged(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+    if (bytes == 0) return QS_OK;
+    StagingRing& R = g_ring;
+    const int k = R.next;
+    R.next = (k + 1) % kRing;
+    if (!R.ev[k]) QS_HIP(hipEventCreateWithFlags(&R.ev[k], hipEventDisableTiming));
+    if (R.used[k]) QS_HIP(hipEventSynchronize(R.ev[k]));   // the copy that last read it
+    if (R.cap[k] < bytes) {
+        if (R.buf[k]) QS_HIP(hipHostFree(R.buf[k]));
+        R.buf[k] = nullptr;
+        R.cap[k] = 0;
+        size_t cap = 4096;
+        while (cap < bytes) cap *= 2;
+        QS_HIP(hipHostMalloc(&R.buf[k], cap, hipHostMallocDefault));
+        R.cap[k] = cap;
+    }
+    std::memcpy(R.buf[k], src, bytes);
+    QS_HIP(hipMemcpyAsync(dst, R.buf[k], bytes, hipMemcpyHostToDevice, stream));
+    QS_HIP(hipEventRecord(R.ev[k], stream));
+    R.used[k] = true;
+    return QS_OK;
+}
+
+namespace {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
@@ -137,24 +177,20 @@ std::vector<char> plan_blob(const SvPlan& plan, int nq, SvBlobLayout* layout) {
     return host;
 }
 
-// Uploads the plan's tables into a stream-ordered allocation and launches it.  The host
-// blob lives until the upload has completed (h2d_copy_wait), whatever the runtime does
-// with pageable sources; the launches are queued before that wait, so the device does
-// not idle while the host waits.
+// Uploads the plan's tables into a stream-ordered allocation (through the pinned staging
+// ring, so the host blob may go as soon as the call returns) and launches it.
 int run_plan(double2* state, int nq, const SvPlan& plan, hipStream_t stream) {
     if (plan.passes.empty()) return QS_OK;
     SvBlobLayout L;
     const std::vector<char> host = plan_blob(plan, nq, &L);
     void* dev = nullptr;
     QS_HIP(hipMallocAsync(&dev, L.total, stream));
-    hipEvent_t copied = nullptr;
-    int st = h2d_copy(dev, host.data(), host.size(), stream, &copied);
+    int st = h2d_copy_staged(dev, host.data(), host.size(), stream);
     if (st == QS_OK) st = sv_launch_plan(state, nq, plan, dev, stream);
     const hipError_t fe = hipFreeAsync(dev, stream);
-    const int wt = copied ? h2d_copy_wait(copied) : QS_OK;
     if (st) return st;
     QS_REQUIRE(fe == hipSuccess, QS_E_HIP, "hipFreeAsync failed: %s", hipGetErrorString(fe));
-    return wt;
+    return QS_OK;
 }
 
 }  // namespace

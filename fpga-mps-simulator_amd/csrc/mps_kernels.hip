@@ -1473,14 +1473,9 @@ int qs_mps_update_batch(const qs_mps_task* tasks, int ntasks, int precision, voi
     hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
     char* ws = static_cast<char*>(workspace);
     DevTask* dtasks = reinterpret_cast<DevTask*>(ws);
-    // the table is pageable host memory: it stays alive until the copy has read it, and
-    // the launches are queued before that wait
-    hipEvent_t copied = nullptr;
-    st = h2d_copy(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask), hs, &copied);
+    st = h2d_copy_staged(dtasks, dt.data(), (size_t)ntasks * sizeof(DevTask), hs);
     if (st) return st;
-    st = launch_batch(dtasks, ntasks, counts, chi, precision, ws, status_dev, hs);
-    const int wt = h2d_copy_wait(copied);
-    return st ? st : wt;
+    return launch_batch(dtasks, ntasks, counts, chi, precision, ws, status_dev, hs);
 }
 
 int qs_mps_update_batch_static(const qs_mps_task* tasks, int ntasks, int precision,

@@ -423,16 +423,13 @@ int qs_mps_entanglement_entropy(const int32_t* bond, int n, int chi_max, const d
     cd2* H = static_cast<cd2*>(work);
     double* ev = reinterpret_cast<double*>(H + (size_t)nbonds * chi_max * chi_max);
     int32_t* dbonds = reinterpret_cast<int32_t*>(ev + (size_t)nbonds * chi_max);
-    hipEvent_t copied = nullptr;
-    int st = h2d_copy(dbonds, bonds, (size_t)nbonds * sizeof(int32_t), hs, &copied);
+    int st = h2d_copy_staged(dbonds, bonds, (size_t)nbonds * sizeof(int32_t), hs);
     if (st) return st;
     hipLaunchKernelGGL(mps_entropy_kernel, dim3(nbonds), dim3(kThreads), 0, hs, bond,
                        reinterpret_cast<const cd2*>(lenvs), chi_max, log_traces, dbonds, H, ev,
                        entropy);
-    const hipError_t le = hipGetLastError();
-    const int wt = h2d_copy_wait(copied);
-    QS_REQUIRE(le == hipSuccess, QS_E_HIP, "entropy launch failed: %s", hipGetErrorString(le));
-    return wt;
+    QS_HIP(hipGetLastError());
+    return QS_OK;
 }
 
 size_t qs_mps_expect_work_bytes(int chi_max) {

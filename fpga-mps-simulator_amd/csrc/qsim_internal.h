@@ -52,6 +52,11 @@ void clear_error();
 // caller keeps the source alive until then and may queue more work first.
 int h2d_copy(void* dst, const void* src, size_t bytes, hipStream_t stream, hipEvent_t* done);
 int h2d_copy_wait(hipEvent_t done);
+// Host-to-device copy through a pinned staging ring of the calling thread (8 buffers, each
+// reused once the copy that last read it has completed): `src` may be released as soon as
+// the call returns, and the call does not wait for the stream.  Allocates pinned host
+// memory (hipHostMalloc) when a buffer has to grow.
+int h2d_copy_staged(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
 // Resolves the device address of the MPS static scratch array on `device`, which must be
 // the current device (qs_init calls it); later calls for the same device do nothing.

@@ -51,6 +51,14 @@ class QsMpsTask(ctypes.Structure):
                 ('reserved', ctypes.c_int32), ('gate', ctypes.c_double * 32)]
 
 
+class QsMpsBwdTask(ctypes.Structure):
+    _fields_ = [('a', ctypes.c_void_p), ('b', ctypes.c_void_p), ('a2', ctypes.c_void_p),
+                ('b2', ctypes.c_void_p), ('ga2', ctypes.c_void_p), ('gb2', ctypes.c_void_p),
+                ('ga', ctypes.c_void_p), ('gb', ctypes.c_void_p), ('cl', ctypes.c_int32),
+                ('cm', ctypes.c_int32), ('cr', ctypes.c_int32), ('k', ctypes.c_int32),
+                ('eps', ctypes.c_double), ('gate', ctypes.c_double * 32)]
+
+
 # name -> (restype, argtypes); every function in include/qsim.h
 _PROTOS = {
     'qs_abi_version': (ctypes.c_int, []),
@@ -157,6 +165,10 @@ _PROTOS = {
                                              ctypes.c_int, ctypes.c_void_p]),
     'qs_mps_reset_site': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    'qs_mps_backward_work_bytes': (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int]),
+    'qs_mps_backward_batch': (ctypes.c_int, [ctypes.POINTER(QsMpsBwdTask), ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                             ctypes.c_void_p, ctypes.c_void_p]),
     'qs_mps_apply_1q': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                        ctypes.c_int, _c_double_p, ctypes.c_void_p]),
 }

@@ -20,7 +20,7 @@ LIB_NAME = 'libqsim_hip.so'
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
 SOURCES = ['abi.hip', 'sv_kernels.hip', 'sv_plan.cpp', 'sv_grad.hip', 'sv_sample.hip',
-           'mps_kernels.hip', 'mps_sample.hip', 'mps_readout.hip']
+           'mps_kernels.hip', 'mps_sample.hip', 'mps_readout.hip', 'mps_grad.hip']
 
 HIPCC_FLAGS = [
     '--offload-arch=gfx950',

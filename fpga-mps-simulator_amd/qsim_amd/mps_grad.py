@@ -30,7 +30,7 @@ mps_core.py:277-350 through the GPU kernels.
 
 from __future__ import annotations
 
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -192,10 +192,11 @@ class MPSEnergyGrad:
         bonds = np.ones(self.c.nq + 1, dtype=np.int64)
         for blk in self.c.blocks:
             if blk.kind == 'rot':
-                for name, q, k in blk.ops:
-                    if record:
+                # a rotation block acts on distinct qubits: one batched launch
+                if record:
+                    for name, q, k in blk.ops:
                         tape.append(('1q', q, name, k, site(q).clone()))
-                    ch.apply_1q(q, rotation(name, theta[k]))
+                ch.apply_1q_batch([(q, rotation(name, theta[k])) for name, q, k in blk.ops])
             else:
                 batch: List[Tuple[int, np.ndarray]] = []
                 busy = set()
@@ -275,14 +276,63 @@ def torch_energy(module: MPSEnergyGrad):
     return _Energy.apply
 
 
+def backward_batch_device(records, cot, eps: float = 1e-24) -> None:
+    """The reverse steps of independent two-site updates (disjoint bonds) in one launch of
+    the hand-written kernel (qs_mps_backward_batch); updates cot in place.
+
+    records: tape entries ('2q', i, G, A, B, A', B'); cot[i] holds the cotangent of site
+    i after the update and receives the one before it."""
+    import ctypes
+    import torch
+    from . import _lib
+    k = len(records)
+    dev = records[0][3].device
+    chi = 1
+    tasks = (_lib.QsMpsBwdTask * k)()
+    outs = []
+    keep = []
+    gates = torch.stack([r[2] for r in records]).detach().cpu().numpy().reshape(k, 16)
+    for j, (_, i, G, A, B, A2, B2) in enumerate(records):
+        gA2 = cot[i].contiguous()
+        gB2 = cot[i + 1].contiguous()
+        gA = torch.empty_like(A)
+        gB = torch.empty_like(B)
+        keep += [gA2, gB2]
+        outs.append((i, gA, gB))
+        t = tasks[j]
+        t.a, t.b, t.a2, t.b2 = A.data_ptr(), B.data_ptr(), A2.data_ptr(), B2.data_ptr()
+        t.ga2, t.gb2, t.ga, t.gb = gA2.data_ptr(), gB2.data_ptr(), gA.data_ptr(), gB.data_ptr()
+        t.cl, t.cm, t.cr, t.k = A.shape[0], A.shape[2], B.shape[2], A2.shape[2]
+        t.eps = float(eps)
+        t.gate[:] = [float(v) for z in gates[j] for v in (z.real, z.imag)]
+        chi = max(chi, t.cl, t.cm, t.cr, t.k)
+    nbytes = _lib.lib().qs_mps_backward_work_bytes(k, chi)
+    work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    status = torch.full((k,), -99, dtype=torch.int32, device=dev)
+    _lib.call('qs_mps_backward_batch', tasks, k, chi, ctypes.c_void_p(work.data_ptr()), nbytes,
+              ctypes.c_void_p(status.data_ptr()), _lib.current_stream(dev.index))
+    bad = int(status.min().item())
+    if bad != 0:
+        raise RuntimeError(f'two-site backward failed with status {bad}')
+    for i, gA, gB in outs:
+        cot[i], cot[i + 1] = gA, gB
+
+
 def energy_and_grad_from_tape(finals, tape, theta, nparams: int, J: float = 1.0,
-                              h: float = 1.0, eps: float = 1e-24) -> Tuple[float, np.ndarray]:
+                              h: float = 1.0, eps: float = 1e-24,
+                              device_kernels: Optional[bool] = None) -> Tuple[float, np.ndarray]:
     """E and dE/dtheta from the final site tensors and the forward tape (reverse sweep).
 
     Tape records: ('1q', site, rotation name, parameter index, A before) and
-    ('2q', left site, G, A, B before, A', B' after).
+    ('2q', left site, G, A, B before, A', B' after).  With device_kernels (the default
+    for a tape of GPU tensors, which is what MPSEnergyGrad records) the two-site steps run
+    in the hand-written kernel, the records of disjoint bonds that follow each other in the
+    tape (a brick-wall layer) in one launch; otherwise, and for tapes of host tensors,
+    through two_site_backward (torch tensor ops, the same formulas).
     """
     import torch
+    if device_kernels is None:
+        device_kernels = bool(finals) and finals[0].is_cuda
     # Called from autograd.Function.backward too, where grad mode is off.
     with torch.enable_grad():
         leaves = [t.detach().requires_grad_(True) for t in finals]
@@ -290,17 +340,40 @@ def energy_and_grad_from_tape(finals, tape, theta, nparams: int, J: float = 1.0,
         gs = torch.autograd.grad(e, leaves)
     cot = {i: g for i, g in enumerate(gs)}
     grad = np.zeros(nparams)
+    contrib: list = []          # (parameter index, device scalar) of the rotations
+    pending: list = []
+    busy: set = set()
+
+    def flush():
+        if pending:
+            backward_batch_device(pending, cot, eps)
+            pending.clear()
+            busy.clear()
+
     for rec in reversed(tape):
         if rec[0] == '1q':
             _, q, name, k, A = rec
+            if q in busy:
+                flush()
             gA2 = cot[q]
             dM = torch.as_tensor(drotation(name, theta[k]), device=A.device)
             M = torch.as_tensor(rotation(name, theta[k]), device=A.device)
             dA2 = torch.einsum('st,atb->asb', dM, A)
-            grad[k] += float(torch.real(torch.sum(gA2.conj() * dA2)))
+            contrib.append((k, torch.real(torch.sum(gA2.conj() * dA2))))
             cot[q] = torch.einsum('ts,atb->asb', M.conj(), gA2)
+        elif device_kernels:
+            i = rec[1]
+            if i in busy or i + 1 in busy:
+                flush()
+            pending.append(rec)
+            busy.update((i, i + 1))
         else:
             _, i, G, A, B, A2, B2 = rec
             gA, gB = two_site_backward(A, B, G, A2, B2, cot[i], cot[i + 1], eps)
             cot[i], cot[i + 1] = gA, gB
+    flush()
+    if contrib:
+        vals = torch.stack([v for _, v in contrib]).cpu().numpy()
+        for (k, _), v in zip(contrib, vals):
+            grad[k] += float(v)
     return float(e.detach()), grad

@@ -265,6 +265,34 @@ int qs_mps_batch_launch(const void* table_dev, int ntasks, const int32_t counts[
                         int chi_max, int precision, void* workspace, size_t ws_bytes,
                         int32_t* status_dev, void* stream);
 
+/* Reverse-mode step of the two-site update (no counterpart in the reference, which has no
+ * differentiation): for each task, the cotangents of the sites before one update, given
+ * the sites before (a: (cl, 2, cm), b: (cm, 2, cr)) and after (a2: (cl, 2, k), b2: (k, 2,
+ * cr)) it and the cotangents ga2, gb2 of the sites after it (device complex128, C order;
+ * cotangent convention dL = Re sum conj(g) dz).  Writes ga (cl, 2, cm) and gb (cm, 2, cr).
+ * The SVD backward uses F_ij = (s_j^2 - s_i^2) / ((s_j^2 - s_i^2)^2 + eps s_max^4) so that
+ * near-degenerate singular values stay finite; S is read back as Re diag(U^H theta V) with U,
+ * V from a2, b2 (whose column / row norms are sqrt(S_t)).  One workgroup per task; tasks
+ * must not write overlapping outputs.  `work`: qs_mps_backward_work_bytes(ntasks, chi_max)
+ * device bytes, chi_max bounding every dimension; status_dev (device int32[ntasks], may be
+ * null) receives QS_OK or QS_E_ARG per task. */
+typedef struct qs_mps_bwd_task {
+    const double* a;
+    const double* b;
+    const double* a2;
+    const double* b2;
+    const double* ga2;
+    const double* gb2;
+    double*       ga;
+    double*       gb;
+    int32_t       cl, cm, cr, k;
+    double        eps;
+    double        gate[32];   /* the update's 4x4 gate, row index 2 sigma_i + sigma_{i+1} */
+} qs_mps_bwd_task;
+size_t qs_mps_backward_work_bytes(int ntasks, int chi_max);
+int qs_mps_backward_batch(const qs_mps_bwd_task* tasks, int ntasks, int chi_max, void* work,
+                          size_t work_bytes, int32_t* status_dev, void* stream);
+
 /* Kernel timing of the MPS batches (instrumentation, no counterpart in the reference).
  * While enabled, every contraction and SVD launch of qs_mps_update_batch /
  * qs_mps_batch_launch is bracketed by HIP events on its stream.  qs_mps_kernel_times
