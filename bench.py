@@ -5,7 +5,10 @@ Headline (`value`): 28-qubit complex128 statevector sweep, gates/s, on the
 workload "20 layers of U3 on every qubit + CZ ladder, then QFT-28" (1520
 logical gates) with the state resident in HBM.  A step applies the whole
 circuit once: one launch of a plan built before the timed region
-(qs_sv_plan_create; its host time is reported as `config.plan_ms`).  The line also carries:
+(qs_sv_plan_create; its host time is reported as `config.plan_ms`), whose passes run
+kernels compiled for the plan (qs_sv_plan_specialize, hipRTC; the compile time is
+reported as `config.jit_ms`; `--interpreter` runs the op-stream interpreter instead).
+The line also carries:
   roofline     : algorithmic HBM bytes of the dominant kernel (the fused pass:
                  read + write of the 4 GiB state) per launch over its average
                  duration, timed with HIP events around every pass launch of the
@@ -69,10 +72,14 @@ def parse():
     ap.add_argument('--calibration-copy', action='store_true',
                     help='after the SV leg, copy the 4 GiB state once (a copy kernel of '
                          'known size for scaling the HBM byte counters of a profile)')
+    ap.add_argument('--interpreter', action='store_true',
+                    help='run the plan on the op-stream interpreter kernel instead of '
+                         'kernels compiled for the plan')
     return ap.parse_args()
 
 
-def sv_leg(steps: int, warmup: int, calibration_copy: bool = False):
+def sv_leg(steps: int, warmup: int, calibration_copy: bool = False,
+           specialize: bool = True):
     import torch
     from qsim_amd import workloads
     from qsim_amd.sv import StateVector, gates_to_records, pass_times, time_passes
@@ -85,6 +92,11 @@ def sv_leg(steps: int, warmup: int, calibration_copy: bool = False):
     t0 = time.perf_counter()
     plan = sv.plan(recs)
     plan_s = time.perf_counter() - t0
+    jit_s, jit_kernels = 0.0, 0
+    if specialize:
+        t0 = time.perf_counter()
+        jit_kernels = plan.specialize()
+        jit_s = time.perf_counter() - t0
     npass = plan.npasses
     for _ in range(warmup):
         sv.launch(plan)
@@ -123,6 +135,7 @@ def sv_leg(steps: int, warmup: int, calibration_copy: bool = False):
                    for p in range(npass)]
     return {
         'per_pass_ms': per_pass_ms, 'records': recs, 'plan_s': plan_s,
+        'jit_s': jit_s, 'jit_kernels': jit_kernels, 'specialized': specialize,
         'plan_device_bytes': plan.device_bytes,
         'wall_s': wall, 'gpu_ms': gpu_ms, 'npass': npass, 'ngates': len(gates),
         'per_launch_s': per_launch_s, 'bytes_per_pass': bytes_per_pass,
@@ -411,7 +424,8 @@ def main():
         from qsim_amd import mps_bench
         print(json.dumps(mps_bench.run_multi(args.multi)))
         return
-    sv = sv_leg(args.steps, args.warmup, args.calibration_copy)
+    sv = sv_leg(args.steps, args.warmup, args.calibration_copy,
+                specialize=not args.interpreter)
     if args.pass_table:
         write_pass_table(args.pass_table, sv, 'python bench.py ' + ' '.join(sys.argv[1:]))
     value = sv['ngates'] * args.steps / (sv['gpu_ms'] / 1e3)
@@ -433,12 +447,16 @@ def main():
                                '(1520 gates), complex128, state resident in HBM',
                    'qubits': SV_QUBITS, 'gates': sv['ngates'], 'passes_per_step': sv['npass'],
                    'plan_ms': sv['plan_s'] * 1e3, 'plan_device_bytes': sv['plan_device_bytes'],
+                   'pass_kernels': ('per-plan kernels (qs_sv_plan_specialize)'
+                                    if sv['specialized'] else 'op-stream interpreter'),
+                   'jit_ms': sv['jit_s'] * 1e3, 'jit_kernels': sv['jit_kernels'],
                    'hbm_gbps_whole_step': sv['npass'] * sv['bytes_per_pass'] * args.steps /
                    (sv['gpu_ms'] / 1e3) / 1e9},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBPS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBPS,
                      'traffic': traffic_from_profiles(),
-                     'kernel': 'sv_pass_kernel<4>',
+                     'kernel': ('qs_sv_pass_* (per-plan pass kernels)' if sv['specialized']
+                                else 'sv_pass_kernel<4>'),
                      'per_launch_us': sv['per_launch_s'] * 1e6,
                      'inplace_rmw_gbps': sv['inplace_rmw_gbps']},
         'check': {'norm_sq_after': sv['norm_sq']},

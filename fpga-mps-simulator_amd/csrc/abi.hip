@@ -308,6 +308,7 @@ struct qs_sv_plan {
     qs::SvPlan plan;
     void* dev = nullptr;       // uploaded tables + block-factor table
     size_t bytes = 0;
+    std::vector<hipFunction_t> spec;   // per-pass kernels after qs_sv_plan_specialize
 };
 
 extern "C" {
@@ -354,7 +355,56 @@ int qs_sv_plan_launch(const qs_sv_plan* plan, void* state, void* stream) {
     QS_REQUIRE(dev == plan->device, QS_E_ARG, "plan lives on device %d, current device is %d",
                plan->device, dev);
     return sv_launch_plan(static_cast<double2*>(state), plan->nq, plan->plan, plan->dev,
-                          as_stream(stream));
+                          as_stream(stream), plan->spec.empty() ? nullptr : plan->spec.data());
+}
+
+int qs_sv_plan_specialize(qs_sv_plan* plan, int* kernels_compiled) {
+    clear_error();
+    QS_REQUIRE(plan != nullptr, QS_E_ARG, "plan is null");
+    if (kernels_compiled) *kernels_compiled = 0;
+    if (plan->plan.passes.empty()) return QS_OK;
+    int dev = -1;
+    QS_HIP(hipGetDevice(&dev));
+    QS_REQUIRE(dev == plan->device, QS_E_ARG, "plan lives on device %d, current device is %d",
+               plan->device, dev);
+    std::vector<hipFunction_t> fns;
+    const int st = sv_jit_specialize(plan->plan, dev, &fns, kernels_compiled);
+    if (st) return st;
+    plan->spec = std::move(fns);
+    return QS_OK;
+}
+
+int qs_sv_jit_source(int nq, const qs_gate* gates, int ng, int pass, char* buf, size_t cap,
+                     size_t* needed) {
+    clear_error();
+    QS_REQUIRE(needed != nullptr, QS_E_ARG, "needed is null");
+    QS_REQUIRE(nq >= 1 && nq <= SV_MAX_QUBITS, QS_E_RANGE, "nq=%d outside [1, %d]", nq,
+               SV_MAX_QUBITS);
+    QS_REQUIRE(ng >= 0 && (ng == 0 || gates), QS_E_ARG, "bad gate array");
+    SvPlan plan;
+    int st = sv_build_plan(nq, gates, ng, &plan);
+    if (st) return st;
+    QS_REQUIRE(pass >= 0 && pass < (int)plan.passes.size(), QS_E_RANGE,
+               "pass %d outside [0, %zu)", pass, plan.passes.size());
+    const std::string src = sv_jit_source(plan, pass);
+    *needed = src.size() + 1;
+    if (buf && cap > 0) {
+        const size_t n = std::min(cap - 1, src.size());
+        std::memcpy(buf, src.data(), n);
+        buf[n] = '\0';
+    }
+    return QS_OK;
+}
+
+int qs_sv_jit_check(int nq, const qs_gate* gates, int ng, int* nkernels, size_t* code_bytes) {
+    clear_error();
+    QS_REQUIRE(nq >= 1 && nq <= SV_MAX_QUBITS, QS_E_RANGE, "nq=%d outside [1, %d]", nq,
+               SV_MAX_QUBITS);
+    QS_REQUIRE(ng >= 0 && (ng == 0 || gates), QS_E_ARG, "bad gate array");
+    SvPlan plan;
+    int st = sv_build_plan(nq, gates, ng, &plan);
+    if (st) return st;
+    return sv_jit_check(plan, nkernels, code_bytes);
 }
 
 int qs_sv_plan_info(const qs_sv_plan* plan, int* nq, int* npasses, size_t* device_bytes) {

@@ -1,0 +1,208 @@
+// Per-plan pass kernels compiled at run time (hipRTC, gfx950).
+//
+// The op-stream interpreter (sv_pass_kernel) reads every op's kind from the op table
+// and branches to its body, once per op and workgroup.  For a plan that is launched many
+// times, the dispatch can be compiled away: each pass gets a kernel that runs the pass's
+// ops as one straight-line sequence of op_body calls whose kind, register bits, control
+// mask and aux word are constants.  The op data (rotation parameters, phase
+// descriptors, layouts) is still read from the uploaded op table, so two passes with the
+// same op sequence share one kernel, and a plan of the same circuit structure with other
+// angles reuses the compiled kernels of the process-wide cache.  The device code is
+// sv_ops.h itself, embedded into the library at build time, so a specialised pass runs
+// the interpreter's op bodies in the interpreter's order and its result is the same bit
+// for bit.
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "qsim_internal.h"
+
+namespace qs {
+
+namespace {
+
+// kSvDeviceTypesSrc, kSvOpsSrc: the text of sv_device_types.h and sv_ops.h (generated
+// by the build from those files).
+#include "sv_jit_src.inc"
+
+// The fields of an op record that op_body reads as int fields of the head, by kind;
+// the others are emitted as 0 so that equal op sequences give equal kernel text.
+void head_constants(const DevGate& G, int* r0, int* r1, int* qa, int* qb, int* aux,
+                    uint64_t* cmask) {
+    *r0 = *r1 = *qa = *qb = *aux = 0;
+    *cmask = 0;
+    const int k = G.kind;
+    if (k >= DG_ROT) {
+        *aux = G.aux;
+    } else if (k == DG_SIGN_REG || k == DG_U1S || k == DG_SWAP || k == DG_SCALE) {
+        *r0 = G.r0;
+    } else if (k == DG_PHASE) {
+        *r0 = G.r0;
+        *qb = G.qb;
+        *aux = G.aux;
+    } else if (k == DG_U1) {
+        *r0 = G.r0;
+        *cmask = G.cmask;
+    } else if (k == DG_U2) {
+        *r0 = G.r0;
+        *r1 = G.r1;
+        *cmask = G.cmask;
+    } else if (k == DG_DIAG1 || k == DG_DIAG2) {
+        *qa = G.qa;
+        *qb = G.qb;
+        *cmask = G.cmask;
+    }
+}
+
+// Kernel text of pass P without its name (the name is spliced in at "@NAME@").
+std::string pass_body(const SvPlan& plan, const DevPass& P) {
+    std::string s;
+    s.reserve(512 + 200 * (size_t)P.nop);
+    s += "extern \"C\" __global__ void __launch_bounds__(1 << (qs::SV_TILE_BITS - qs::SV_REG_BITS))\n"
+         "@NAME@(qs::cd2* __restrict__ state, const qs::DevPass* __restrict__ P,\n"
+         "       const qs::DevGate* __restrict__ gates, const qs::cd2* __restrict__ tfactors,\n"
+         "       const qs::cd2* __restrict__ factors, uint32_t ntiles) {\n"
+         "    using namespace qs;\n";
+    s += "    constexpr int R = " + std::to_string(P.R) + ";\n";
+    s += "    __shared__ cd2 lds[kLdsSlots];\n"
+         "    const int t = threadIdx.x;\n"
+         "    const uint32_t tile = blockIdx.x;\n"
+         "    const cd2* tile_factors = factors + ((size_t)P->slot0 * ntiles + tile) * PHASE_BLOCK_VALUES;\n"
+         "    const uint64_t base = tile_base(P, tile);\n"
+         "    Layout<R> Lcur;\n"
+         "    cd2 a[1 << R];\n"
+         "    pass_load<R>(a, Lcur, state, P, base, t);\n"
+         "    const DevGate* gt = gates + P->gate0;\n";
+    char line[320];
+    for (int i = 0; i < P.nop; ++i) {
+        const DevGate& G = plan.gates[(size_t)P.gate0 + i];
+        int r0, r1, qa, qb, aux;
+        uint64_t cm;
+        head_constants(G, &r0, &r1, &qa, &qb, &aux, &cm);
+        std::snprintf(line, sizeof(line),
+                      "    { const OpHead H = op_head(gt + %d, %d, %d, %d, %d, %d, %d, %lluull); "
+                      "op_body<R>(a, Lcur, H, gt[%d].m, %d, tfactors, tile_factors, ntiles, base, lds, t); }\n",
+                      i, G.kind, r0, r1, qa, qb, aux, (unsigned long long)cm, i, G.kind);
+        s += line;
+    }
+    s += "    pass_store<R>(a, Lcur, state, base);\n}\n";
+    return s;
+}
+
+std::string kernel_name(size_t i) { return "qs_sv_pass_" + std::to_string(i); }
+
+// One hipRTC program over `bodies` (kernel i named kernel_name(i)); the code object into
+// *code.  The log of a failed compile goes into the error message.
+int compile_program(const std::vector<const std::string*>& bodies, std::vector<char>* code) {
+    std::string src = "#include \"sv_ops.h\"\n";
+    for (size_t i = 0; i < bodies.size(); ++i) {
+        std::string b = *bodies[i];
+        b.replace(b.find("@NAME@"), 6, kernel_name(i));
+        src += b;
+    }
+    const char* headers[] = {kSvDeviceTypesSrc, kSvOpsSrc};
+    const char* names[] = {"sv_device_types.h", "sv_ops.h"};
+    hiprtcProgram prog = nullptr;
+    hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "qs_sv_passes.hip", 2, headers, names);
+    QS_REQUIRE(r == HIPRTC_SUCCESS, QS_E_HIP, "hiprtcCreateProgram failed: %s",
+               hiprtcGetErrorString(r));
+    // The same code generation as the library's own build: gfx950, -O3, HIP's default
+    // contraction, no packed FP32 instructions.
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                          "-ffp-contract=fast-honor-pragmas", "-Xclang", "-target-feature",
+                          "-Xclang", "-packed-fp32-ops"};
+    r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    if (r != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        std::string log;
+        if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 1) {
+            log.resize(n);
+            (void)hiprtcGetProgramLog(prog, &log[0]);
+        }
+        (void)hiprtcDestroyProgram(&prog);
+        set_error("hipRTC compile of %zu pass kernels failed: %s\n%.2000s", bodies.size(),
+                  hiprtcGetErrorString(r), log.c_str());
+        return QS_E_HIP;
+    }
+    size_t n = 0;
+    r = hiprtcGetCodeSize(prog, &n);
+    if (r == HIPRTC_SUCCESS) {
+        code->resize(n);
+        r = hiprtcGetCode(prog, code->data());
+    }
+    (void)hiprtcDestroyProgram(&prog);
+    QS_REQUIRE(r == HIPRTC_SUCCESS, QS_E_HIP, "hipRTC code retrieval failed: %s",
+               hiprtcGetErrorString(r));
+    return QS_OK;
+}
+
+// Process-wide cache of loaded pass kernels: (device, kernel text) -> function.  Modules
+// stay loaded until the process ends.
+std::mutex g_cache_mu;
+std::map<std::pair<int, std::string>, hipFunction_t> g_cache;
+
+}  // namespace
+
+std::string sv_jit_source(const SvPlan& plan, int pass) {
+    std::string b = pass_body(plan, plan.passes[pass]);
+    b.replace(b.find("@NAME@"), 6, kernel_name(0));
+    return "#include \"sv_ops.h\"\n" + b;
+}
+
+int sv_jit_check(const SvPlan& plan, int* nkernels, size_t* code_bytes) {
+    std::vector<std::string> uniq;
+    for (const DevPass& P : plan.passes) {
+        std::string b = pass_body(plan, P);
+        if (std::find(uniq.begin(), uniq.end(), b) == uniq.end()) uniq.push_back(std::move(b));
+    }
+    std::vector<const std::string*> ptrs;
+    for (const auto& b : uniq) ptrs.push_back(&b);
+    size_t bytes = 0;
+    if (!ptrs.empty()) {
+        std::vector<char> code;
+        const int st = compile_program(ptrs, &code);
+        if (st) return st;
+        bytes = code.size();
+    }
+    if (nkernels) *nkernels = (int)uniq.size();
+    if (code_bytes) *code_bytes = bytes;
+    return QS_OK;
+}
+
+int sv_jit_specialize(const SvPlan& plan, int device, std::vector<hipFunction_t>* fns,
+                      int* compiled) {
+    std::vector<std::string> body(plan.passes.size());
+    for (size_t p = 0; p < plan.passes.size(); ++p) body[p] = pass_body(plan, plan.passes[p]);
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    std::vector<const std::string*> todo;
+    for (const auto& b : body) {
+        if (g_cache.count({device, b})) continue;
+        if (std::find_if(todo.begin(), todo.end(), [&](const std::string* x) { return *x == b; }) ==
+            todo.end())
+            todo.push_back(&b);
+    }
+    if (!todo.empty()) {
+        // One program for all of them: the run-time compiler serialises compilations
+        // within a process, and one program parses the headers once.
+        std::vector<char> code;
+        const int st = compile_program(todo, &code);
+        if (st) return st;
+        hipModule_t mod = nullptr;
+        QS_HIP(hipModuleLoadData(&mod, code.data()));
+        for (size_t k = 0; k < todo.size(); ++k) {
+            hipFunction_t f = nullptr;
+            QS_HIP(hipModuleGetFunction(&f, mod, kernel_name(k).c_str()));
+            g_cache[{device, *todo[k]}] = f;
+        }
+    }
+    fns->resize(plan.passes.size());
+    for (size_t p = 0; p < body.size(); ++p) (*fns)[p] = g_cache.at({device, body[p]});
+    if (compiled) *compiled = (int)todo.size();
+    return QS_OK;
+}
+
+}  // namespace qs

@@ -77,6 +77,13 @@ _PROTOS = {
                                        ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_size_t)]),
     'qs_sv_plan_destroy': (ctypes.c_int, [ctypes.c_void_p]),
+    'qs_sv_plan_specialize': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    'qs_sv_jit_source': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(QsGate), ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
+                                        ctypes.POINTER(ctypes.c_size_t)]),
+    'qs_sv_jit_check': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(QsGate), ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_size_t)]),
     'qs_sv_plan_passes': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(QsGate), ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int)]),
     'qs_sv_plan_json': (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(QsGate), ctypes.c_int,

@@ -64,6 +64,14 @@ class SvPlan:
             raise RuntimeError('plan already destroyed')
         return self._h
 
+    def specialize(self) -> int:
+        """Compile this plan's per-pass kernels (qs_sv_plan_specialize, hipRTC); later
+        launches run them.  Returns the number of kernels compiled by this call (0 when
+        the process-wide cache already held them all)."""
+        n = ctypes.c_int(0)
+        _lib.call('qs_sv_plan_specialize', self.handle, ctypes.byref(n))
+        return n.value
+
     def destroy(self) -> None:
         if getattr(self, '_h', None) is not None:
             _lib.call('qs_sv_plan_destroy', self._h)
@@ -204,6 +212,27 @@ def plan_passes(nq: int, records: Sequence[tuple]) -> int:
     n = ctypes.c_int(0)
     _lib.call('qs_sv_plan_passes', nq, arr, len(records), ctypes.byref(n))
     return n.value
+
+
+def jit_check(nq: int, records: Sequence[tuple]) -> Tuple[int, int]:
+    """Generate and compile the per-pass kernels of the plan of `records` without a
+    device (qs_sv_jit_check): (distinct kernels, bytes of code objects)."""
+    arr = records_to_array(records)
+    nk, nb = ctypes.c_int(0), ctypes.c_size_t(0)
+    _lib.call('qs_sv_jit_check', nq, arr, len(records), ctypes.byref(nk), ctypes.byref(nb))
+    return nk.value, nb.value
+
+
+def jit_source(nq: int, records: Sequence[tuple], pass_index: int) -> str:
+    """Source text of one pass's specialised kernel (qs_sv_jit_source, host only)."""
+    arr = records_to_array(records)
+    need = ctypes.c_size_t(0)
+    _lib.call('qs_sv_jit_source', nq, arr, len(records), pass_index, None, 0,
+              ctypes.byref(need))
+    buf = ctypes.create_string_buffer(need.value)
+    _lib.call('qs_sv_jit_source', nq, arr, len(records), pass_index, buf, need.value,
+              ctypes.byref(need))
+    return buf.value.decode()
 
 
 def plan_json(nq: int, records: Sequence[tuple]) -> dict:

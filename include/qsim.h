@@ -118,6 +118,26 @@ int qs_sv_plan_launch(const qs_sv_plan* plan, void* state, void* stream);
 int qs_sv_plan_info(const qs_sv_plan* plan, int* nq, int* npasses, size_t* device_bytes);
 int qs_sv_plan_destroy(qs_sv_plan* plan);
 
+/* Per-plan pass kernels (no counterpart in the reference).  qs_sv_plan_specialize
+ * compiles, with hipRTC for gfx950, one kernel per distinct pass of the plan: the pass's
+ * ops as straight-line code whose op kinds, register bits, controls and aux words are
+ * compile-time constants (the op-stream interpreter branches on them per op).  The op
+ * data stays in the plan's tables, so later launches of the plan run these kernels and
+ * give the same state bit for bit.  Kernels live in a process-wide cache keyed by device
+ * and kernel text: passes with the same op sequence, and plans of the same circuit
+ * structure with other parameters, reuse them.  *kernels_compiled (may be NULL) receives
+ * the number compiled by this call.  On failure (QS_E_HIP, the compiler log in
+ * qs_last_error) the plan keeps launching the interpreter.  The device must be the
+ * plan's.  Not thread-safe with launches of the same plan.
+ * qs_sv_jit_check builds the plan of `gates` and compiles its pass kernels without
+ * loading them (host only, no device needed): *nkernels distinct kernels, *code_bytes
+ * bytes of code objects (either may be NULL). */
+int qs_sv_plan_specialize(qs_sv_plan* plan, int* kernels_compiled);
+int qs_sv_jit_check(int nq, const qs_gate* gates, int ng, int* nkernels, size_t* code_bytes);
+/* Source text of pass `pass`'s kernel (host only; buffer contract as qs_sv_plan_json). */
+int qs_sv_jit_source(int nq, const qs_gate* gates, int ng, int pass, char* buf, size_t cap,
+                     size_t* needed);
+
 /* Number of fused passes qs_sv_apply would launch for this gate list (host only). */
 int qs_sv_plan_passes(int nq, const qs_gate* gates, int ng, int* npasses);
 
