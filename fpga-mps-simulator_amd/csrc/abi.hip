@@ -474,7 +474,7 @@ int qs_sv_plan_json(int nq, const qs_gate* gates, int ng, char* buf, size_t cap,
                    << ",\"r1\":" << G.r1 << ",\"qa\":" << G.qa << ",\"qb\":" << G.qb
                    << ",\"cmask\":" << G.cmask << ",\"aux\":" << G.aux << ",\"m\":[";
                 // a SIGN op's m holds its SignDesc words (printed under "phase")
-                for (int k = 0; k < 32; ++k)
+                for (int k = 0; k < kDevGateWords; ++k)
                     os << (k ? "," : "") << (G.kind == DG_PHASE_SIGN || G.kind == DG_SIGN_REG ? 0.0 : G.m[k]);
                 os << "]";
                 if (G.kind == DG_PHASE || G.kind == DG_PHASE_SIGN || G.kind == DG_SIGN_REG ||

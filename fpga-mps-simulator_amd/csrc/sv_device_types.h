@@ -64,9 +64,13 @@ constexpr int SV_THREAD_BITS = SV_TILE_BITS - SV_REG_BITS;   // thread-index bit
 //             DG_ROTS): a DG_SIGN_REG mask applied after the rotations, with its phase
 //             record in qa.
 //   DG_ROTS + mask - 1 (15 kinds): first-form rotations with output factors on every
-//             register bit of mask = r0 (1..15), one op: bit b has t = m[b] and
-//             k = (m[4 + 2b], m[5 + 2b]).  Rotations on distinct bits commute, so a run
-//             of them shares one dispatch.
+//             register bit of mask = r0 (1..15), one op: bit b has t = m[b] and output
+//             factor k_b.  Rotations on distinct bits commute, and so do their output
+//             factors, so the rotations run first and the factors follow as one
+//             diagonal over the register index: amplitude j is multiplied by
+//             f_j = (m[2 + 2j], m[3 + 2j]) (j = 1..15), the product of k_b over the bits b
+//             of j & mask (one complex product per amplitude instead of one per
+//             rotation and amplitude pair).
 enum : int32_t {
     DG_U1 = 0, DG_U2 = 1, DG_DIAG1 = 2, DG_DIAG2 = 3, DG_U1X = 4, DG_PHASE = 5,
     DG_LAYOUT = 6, DG_PHASE_SIGN = 7, DG_U1S = 8, DG_SWAP = 9, DG_SCALE = 10, DG_SIGN_REG = 11,
@@ -74,13 +78,17 @@ enum : int32_t {
     DG_ROTS = 32
 };
 
+// Matrix words of an op record: 32 for the largest matrices (DG_U2, DG_U1X), 34 for
+// DG_ROTS (four rotation parameters and fifteen merged output factors).
+constexpr int kDevGateWords = 36;
+
 struct alignas(16) DevGate {
     int32_t kind;
     int32_t r0, r1;
     int32_t qa, qb;
     int32_t aux;           // DG_PHASE: the group's block-factor slot (-1: none)
     uint64_t cmask;        // global control mask
-    double m[32];
+    double m[kDevGateWords];
 };
 
 // A register layout of a pass, precomputed on the host: which tile positions are the

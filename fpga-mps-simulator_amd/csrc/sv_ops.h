@@ -249,34 +249,52 @@ __device__ __forceinline__ void apply_rot(cd2 (&a)[1 << R], int kind, double t, 
 }
 #undef QS_ROT_CASES
 
-// First-form rotations with output factors on the register bits of M, in one body.
+// First-form rotations on the register bits of M without their output factors, then
+// the factors as one diagonal: amplitude j times f_j (f[2j - 2], f[2j - 1]) when j has a
+// bit of M.  The rotations leave the j-th amplitude's phase factor to the diagonal,
+// which commutes with them (distinct bits).
 template <int R, int M>
-__device__ __forceinline__ void rots_body(cd2 (&a)[1 << R], const double* t, const cd2* k) {
-    if constexpr ((M & 1) != 0) rot_bit<R, 0, false, true>(a, t[0], k[0]);
-    if constexpr ((M & 2) != 0 && R > 1) rot_bit<R, (R > 1 ? 1 : 0), false, true>(a, t[1], k[1]);
-    if constexpr ((M & 4) != 0 && R > 2) rot_bit<R, (R > 2 ? 2 : 0), false, true>(a, t[2], k[2]);
-    if constexpr ((M & 8) != 0 && R > 3) rot_bit<R, (R > 3 ? 3 : 0), false, true>(a, t[3], k[3]);
+__device__ __forceinline__ void rots_body(cd2 (&a)[1 << R], const double* t,
+                                          const double* __restrict__ f) {
+    const cd2 one = mk2(1.0, 0.0);
+    if constexpr ((M & 1) != 0) rot_bit<R, 0, false, false>(a, t[0], one);
+    if constexpr ((M & 2) != 0 && R > 1) rot_bit<R, (R > 1 ? 1 : 0), false, false>(a, t[1], one);
+    if constexpr ((M & 4) != 0 && R > 2) rot_bit<R, (R > 2 ? 2 : 0), false, false>(a, t[2], one);
+    if constexpr ((M & 8) != 0 && R > 3) rot_bit<R, (R > 3 ? 3 : 0), false, false>(a, t[3], one);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 1; j < (1 << R); ++j) {
+        if ((j & M) == 0) continue;            // compile-time after unrolling
+        a[j] = cmul(mk2(f[2 * j - 2], f[2 * j - 1]), a[j]);
+        if (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    }
 }
 
 #define QS_ROTS_CASE(M) \
-    case DG_ROTS + M - 1: QS_CASE("rots " #M " <"); rots_body<R, M>(a, t, k); QS_CASE("rots " #M " >"); break;
+    case DG_ROTS + M - 1: QS_CASE("rots " #M " <"); rots_body<R, M>(a, t, f); QS_CASE("rots " #M " >"); break;
 
-// DG_ROTS + mask - 1: t = m[0..3] and the output factors m[4..11], all in the op head.
+// DG_ROTS + mask - 1: t = m[0..3] from the op head, the merged output factors f_j from
+// the record (m[2 + 2j], m[3 + 2j]).
 template <int R>
-__device__ __forceinline__ void apply_rots(cd2 (&a)[1 << R], int kind, const double* m) {
-    const double* t = m;
-    const cd2 k[4] = {mk2(m[4], m[5]), mk2(m[6], m[7]), mk2(m[8], m[9]), mk2(m[10], m[11])};
+__device__ __forceinline__ void apply_rots(cd2 (&a)[1 << R], int kind, const double* head,
+                                           const double* __restrict__ m) {
+    const double* t = head;
+    const double* f = m + 4;
     if constexpr (R == 4) {
         switch (kind) {
             QS_ROTS_CASE(1) QS_ROTS_CASE(2) QS_ROTS_CASE(3) QS_ROTS_CASE(4) QS_ROTS_CASE(5)
             QS_ROTS_CASE(6) QS_ROTS_CASE(7) QS_ROTS_CASE(8) QS_ROTS_CASE(9) QS_ROTS_CASE(10)
             QS_ROTS_CASE(11) QS_ROTS_CASE(12) QS_ROTS_CASE(13) QS_ROTS_CASE(14)
-            default: QS_CASE("rots 15 <"); rots_body<R, 15>(a, t, k); QS_CASE("rots 15 >"); break;
+            default: QS_CASE("rots 15 <"); rots_body<R, 15>(a, t, f); QS_CASE("rots 15 >"); break;
         }
     } else {
+        // narrow tiles: one rotation with its own output factor k_b = f_(2^b) per bit
         const int mask = kind - DG_ROTS + 1;
         for (int b = 0; b < R; ++b)
-            if ((mask >> b) & 1) apply_rot<R>(a, DG_ROT + 4 * b + 2, t[b], k[b]);
+            if ((mask >> b) & 1) {
+                const int j = 1 << b;
+                apply_rot<R>(a, DG_ROT + 4 * b + 2, t[b], mk2(f[2 * j - 2], f[2 * j - 1]));
+            }
     }
 }
 #undef QS_ROTS_CASE
@@ -538,7 +556,7 @@ __device__ __forceinline__ void op_body(cd2 (&a)[1 << R], Layout<R>& Lcur, const
     constexpr int NR = 1 << R;
     const int r0 = H.r0, r1 = H.r1;
     if (kind >= DG_ROTS) {
-        apply_rots<R>(a, kind, H.m);
+        apply_rots<R>(a, kind, H.m, m);
         if (H.aux) apply_sign_reg<R>(a, (unsigned)H.aux);
     } else if (kind >= DG_ROT) {
         apply_rot<R>(a, kind, H.m[0], mk2(H.m[2], H.m[3]));

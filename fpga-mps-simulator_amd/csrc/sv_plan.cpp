@@ -1010,6 +1010,22 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
             return QS_E_ARG;
         }
     }
+    // DG_ROTS records were built with k_b at (m[4 + 2b], m[5 + 2b]); the kernels apply
+    // the output factors as one diagonal over the register index, f_j = prod of k_b
+    // over the bits b of j & mask, stored at (m[2 + 2j], m[3 + 2j]) for j = 1..15.
+    for (DevGate& G : plan->gates) {
+        if (G.kind < DG_ROTS || G.kind >= DG_ROTS + 15) continue;
+        const int mask = G.r0;
+        cd k[4];
+        for (int b = 0; b < 4; ++b) k[b] = cd(G.m[4 + 2 * b], G.m[5 + 2 * b]);
+        for (int j = 1; j < 16; ++j) {
+            cd f(1.0, 0.0);
+            for (int b = 0; b < 4; ++b)
+                if (((j & mask) >> b) & 1) f *= k[b];
+            G.m[2 + 2 * j] = f.real();
+            G.m[3 + 2 * j] = f.imag();
+        }
+    }
     return QS_OK;
 }
 

@@ -94,14 +94,25 @@ def replay(plan: dict, nq: int, state: np.ndarray) -> np.ndarray:
                     mask = g['r0']
                     assert cm == 0 and g['kind'] == DG_ROTS + mask - 1 and 0 < mask < (1 << R)
                     assert bin(mask).count('1') >= 2, 'a lone rotation stays a DG_ROT'
+                    # the rotations, then their output factors as one diagonal over the
+                    # register index j: f_j = (m[2 + 2j], m[3 + 2j]), the product of the
+                    # single-bit factors f_(2^b) over the bits of j & mask
+                    f = {j: g['m'][2 + 2 * j] + 1j * g['m'][3 + 2 * j] for j in range(1, 16)}
                     for b in range(R):
                         if (mask >> b) & 1:
                             t = g['m'][b]
                             assert 0.0 <= t <= 1.0 / SECOND_FORM_MIN
                             m = np.array([[1, -t], [t, 1]], dtype=complex)
-                            m[1] *= g['m'][4 + 2 * b] + 1j * g['m'][5 + 2 * b]
                             _apply_1q(st, idx, sel, tq[regs[b]], m)
                             assert (tile >> tq[regs[b]]) & 1
+                    reg = np.zeros_like(idx)
+                    for b in range(R):
+                        reg |= ((idx >> tq[regs[b]]) & 1) << b
+                    for j in range(1, 1 << R):
+                        if j & mask:
+                            want = np.prod([f[1 << b] for b in range(R) if (j & mask) >> b & 1])
+                            assert abs(f[j] - want) <= 1e-15 * (1 + abs(want)), (j, f[j], want)
+                            st[sel & (reg == j)] *= f[j]
                 elif g['kind'] == DG_SCALE:
                     f = g['m'][0] + 1j * g['m'][1]
                     assert (g['r0'] == 1) == (g['m'][1] == 0.0)
