@@ -471,7 +471,14 @@ __device__ __forceinline__ double team_sum(double v) {
 template <int CHI>
 struct SvdShared {
     static constexpr int LD = 2 * CHI;
-    cd2 buf[CHI * LD];
+    // column exchange of the sweeps, the Householder vectors of the preconditioning
+    // QR and the kept left singular vectors; the transpose of R reuses it for R's
+    // upper triangle packed by columns (tbuf)
+    union {
+        cd2 buf[CHI * LD];
+        cd2 tbuf[CHI * (LD + 1)];
+    };
+    double qr_tau[2];
     double sval[2 * CHI];
     int order[2 * CHI];
     int rank[2 * CHI];
@@ -482,6 +489,134 @@ struct SvdShared {
     int keep;
     double scale;
 };
+
+// Householder QR of the nr x nc matrix X held as in mps_svd_kernel (team t: columns 2t,
+// 2t+1 in a, b; lane l: rows l + TL k), in place: afterwards the registers hold R
+// (upper triangular, rows below the diagonal zero).  Step j: the team owning column j
+// forms v = x - beta e_j (rows >= j of column j, beta = -e^{i arg x_j} ||x||, so no
+// cancellation), writes R's column j and v into an LDS buffer (two buffers alternate,
+// so one barrier per step), and every team applies H = I - (2 / v^H v) v v^H to its
+// columns after j.  Column norms and dot products are team sums in fixed order.
+template <int CHI, int TL>
+__device__ __forceinline__ void householder_qr(cd2 (&a)[2 * CHI / TL], cd2 (&b)[2 * CHI / TL],
+                                               SvdShared<CHI>& sh, int nr, int nc, int team,
+                                               int l16) {
+    constexpr int RPL = 2 * CHI / TL;
+    constexpr int LD = 2 * CHI;
+    const int steps = nr < nc ? nr : nc;
+    for (int j = 0; j < steps; ++j) {
+        cd2* vbuf = sh.buf + (j & 1) * LD;
+        const int owner = j >> 1;
+        if (team == owner) {
+            cd2 (&x)[RPL] = (j & 1) ? b : a;
+            double s2 = 0.0, ar = 0.0, ai = 0.0;
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) {
+                const int r = l16 + TL * k;
+                if (r > j) s2 = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, s2));
+                if (r == j) {
+                    ar = x[k].x;
+                    ai = x[k].y;
+                }
+            }
+            s2 = team_sum<TL>(s2);
+            ar = team_sum<TL>(ar);
+            ai = team_sum<TL>(ai);
+            const double aa = fma(ar, ar, ai * ai);
+            const double nrm = sqrt(s2 + aa);
+            double br = 0.0, bi = 0.0, tau2 = 0.0;
+            if (nrm > 0.0) {
+                if (aa > 0.0) {
+                    const double f = -nrm / sqrt(aa);
+                    br = f * ar;
+                    bi = f * ai;
+                } else {
+                    br = -nrm;
+                }
+                const double vr = ar - br, vi = ai - bi;
+                tau2 = 2.0 / (s2 + fma(vr, vr, vi * vi));
+            }
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) {
+                const int r = l16 + TL * k;
+                cd2 v = mk2(0.0, 0.0);
+                if (r > j) v = x[k];
+                if (r == j) v = mk2(ar - br, ai - bi);
+                vbuf[r] = v;
+                if (r == j) x[k] = mk2(br, bi);
+                if (r > j) x[k] = mk2(0.0, 0.0);
+            }
+            if (l16 == 0) sh.qr_tau[j & 1] = tau2;
+        }
+        __syncthreads();
+        const double tau2 = sh.qr_tau[j & 1];
+        if (tau2 != 0.0 && 2 * team + 1 > j && 2 * team < nc) {
+            cd2 v[RPL];
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) v[k] = vbuf[l16 + TL * k];
+            auto apply = [&](cd2 (&x)[RPL]) __attribute__((always_inline)) {
+                double wr = 0.0, wi = 0.0;       // w = v^H x
+#pragma unroll
+                for (int k = 0; k < RPL; ++k) {
+                    wr = fma(v[k].x, x[k].x, fma(v[k].y, x[k].y, wr));
+                    wi = fma(v[k].x, x[k].y, fma(-v[k].y, x[k].x, wi));
+                }
+                wr = team_sum<TL>(wr) * tau2;
+                wi = team_sum<TL>(wi) * tau2;
+#pragma unroll
+                for (int k = 0; k < RPL; ++k) {   // x -= (tau2 w) v
+                    x[k].x -= fma(wr, v[k].x, -wi * v[k].y);
+                    x[k].y -= fma(wr, v[k].y, wi * v[k].x);
+                }
+            };
+            if (2 * team > j) apply(a);
+            if (2 * team + 1 > j && 2 * team + 1 < nc) apply(b);
+        }
+    }
+    __syncthreads();
+}
+
+// Replaces the registers holding R (k0 x nc, upper trapezoidal, columns as in
+// householder_qr) by L = R^H (nc x k0): team t receives columns 2t, 2t+1 of L, the
+// conjugated rows 2t, 2t+1 of R, lane l rows l + TL k.  R's nonzero part (rows <= c of
+// column c, at most (2 CHI)(2 CHI + 1) / 2 entries) goes to LDS packed by columns, so one
+// round with one barrier moves all of it.
+template <int CHI, int TL>
+__device__ __forceinline__ void transpose_conj(cd2 (&a)[2 * CHI / TL], cd2 (&b)[2 * CHI / TL],
+                                               SvdShared<CHI>& sh, int nc, int k0, int team,
+                                               int l16) {
+    constexpr int RPL = 2 * CHI / TL;
+    static_assert(sizeof(sh.tbuf) / sizeof(cd2) >= (2 * CHI) * (2 * CHI + 1) / 2,
+                  "packed triangle must fit");
+    auto offset = [&](int c) {
+        return c < k0 ? c * (c + 1) / 2 : k0 * (k0 + 1) / 2 + (c - k0) * k0;
+    };
+    if (2 * team < nc) {
+        const int c0 = 2 * team, c1 = 2 * team + 1;
+        const int o0 = offset(c0), o1 = offset(c1);
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+            const int r = l16 + TL * k;
+            if (r <= c0 && r < k0) sh.tbuf[o0 + r] = a[k];
+            if (c1 < nc && r <= c1 && r < k0) sh.tbuf[o1 + r] = b[k];
+        }
+    }
+    __syncthreads();
+    const int c0 = 2 * team, c1 = 2 * team + 1;   // columns of L = rows of R
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+        const int r = l16 + TL * k;                 // row of L = column of R
+        cd2 u = mk2(0.0, 0.0), v = mk2(0.0, 0.0);
+        if (r < nc) {
+            const int o = offset(r);
+            if (c0 < k0 && c0 <= r) u = sh.tbuf[o + c0];
+            if (c1 < k0 && c1 <= r) v = sh.tbuf[o + c1];
+        }
+        a[k] = mk2(u.x, -u.y);
+        b[k] = mk2(v.x, -v.y);
+    }
+    __syncthreads();
+}
 
 template <int CHI, int TL>
 __global__ void __launch_bounds__(TL * CHI)
@@ -500,22 +635,40 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
         return;
     }
     const int m = 2 * d.cl, n = 2 * d.cr;
-    const int P = n / 2;                         // column pairs = active teams
+    const int k0 = m < n ? m : n;                // singular values
     const int tid = threadIdx.x;
     const int team = tid / TL, l16 = tid % TL;
-    const bool act = team < P;
     const cd2* th = ws + (size_t)blockIdx.x * theta_slot;
     cd2* slot0 = sh.buf;                     // exchange slot of team t: column t
 
+    // Preconditioning: the one-sided Jacobi runs on L = R^H, where theta^H = Q R
+    // (Householder QR of theta^H, n x m).  theta = L Q^H with Q^H having orthonormal
+    // rows, so theta and L have the same singular values and the same left singular
+    // vectors; the columns of the triangular L are much closer to orthogonal than
+    // theta's, and the sweeps drop from about 14 to about 10 at chi 64.  Vh is formed
+    // from theta itself afterwards, as without preconditioning.
+    //
+    // X = theta^H in registers: team t holds columns 2t, 2t+1 of X (conjugated rows of
+    // theta), lane l rows l + TL k of them.
     cd2 ca[RPL], cb[RPL];
+    {
+        const bool xact = team < m / 2;
 #pragma unroll
-    for (int k = 0; k < RPL; ++k) {
-        const int r = l16 + TL * k;
-        ca[k] = (act && r < m) ? th[(size_t)r * n + 2 * team] : mk2(0, 0);
-        cb[k] = (act && r < m) ? th[(size_t)r * n + 2 * team + 1] : mk2(0, 0);
+        for (int k = 0; k < RPL; ++k) {
+            const int r = l16 + TL * k;
+            const cd2 u = (xact && r < n) ? th[(size_t)(2 * team) * n + r] : mk2(0, 0);
+            const cd2 v = (xact && r < n) ? th[(size_t)(2 * team + 1) * n + r] : mk2(0, 0);
+            ca[k] = mk2(u.x, -u.y);
+            cb[k] = mk2(v.x, -v.y);
+        }
     }
-    // Frobenius norm^2 of theta: columns below 1e-14 * ||theta||_F are numerical zeros
-    // and are not rotated (a wide matrix has n - m of them).
+    householder_qr<CHI, TL>(ca, cb, sh, n, m, team, l16);
+    // L = R^H: team t takes columns 2t, 2t+1 of L, i.e. conjugated rows 2t, 2t+1 of R.
+    transpose_conj<CHI, TL>(ca, cb, sh, m, k0, team, l16);
+    const int P = k0 / 2;                        // column pairs of L = active teams
+    const bool act = team < P;
+    // Frobenius norm^2 of theta (that of L): columns below 1e-14 * ||theta||_F are
+    // numerical zeros and are not rotated.
     {
         double s = 0.0;
 #pragma unroll
@@ -655,10 +808,10 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
         sh.sval[2 * team + 1] = nb;
     }
     __syncthreads();
-    if (tid < n) {
+    if (tid < k0) {
         const double v = sh.sval[tid];
         int r = 0;
-        for (int j = 0; j < n; ++j) {
+        for (int j = 0; j < k0; ++j) {
             const double w = sh.sval[j];
             r += (w > v) || (w == v && j < tid);
         }
