@@ -34,6 +34,13 @@ namespace {
 constexpr int kMaxChi = 128;             // largest chi_max of a task
 constexpr int kRegChi = 64;              // register SVD / MFMA contraction: 2*chi <= 128
 constexpr int kMaxSweeps = 60;
+// Stopping rule of the register SVD kernel: a sweep in which every rotation had
+// |a^H b| <= kFinalSweepRot ||a|| ||b|| is the last one.  Cyclic Jacobi converges
+// quadratically at the end (on the chi-64 thetas the largest relative |a^H b| per sweep
+// runs ... 1e-3, 1e-6, 1e-12, then the 3e-15 rounding floor), so what is left after such a
+// sweep is far below the threshold at which rotations are applied at all, and the sweep
+// that would only confirm it is not run.
+constexpr double kFinalSweepRot = 1e-9;
 
 __device__ __forceinline__ cd2 cmul(cd2 a, cd2 b) {
     return mk2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
@@ -703,6 +710,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
         gr = team_sum<TL>(gr);
         gi = team_sum<TL>(gi);
         double c, sn, er, ei, tg;
+        const bool big = fma(gr, gr, gi * gi) > kFinalSweepRot * kFinalSweepRot * nx * ny;
         if (jacobi_rotation(nx, ny, gr, gi, zero_sq, tol, c, sn, er, ei, tg)) {
             nx = fmax(nx - tg, 0.0);
             ny += tg;
@@ -715,7 +723,7 @@ mps_svd_kernel(const DevTask* __restrict__ tasks, const cd2* __restrict__ ws,
                 x[k] = mk2(c * u.x - sy.x, c * u.y - sy.y);
                 y[k] = mk2(c * v.x + sx.x, c * v.y + sx.y);
             }
-            if (l16 == 0) sh.rotated = 1;
+            if (l16 == 0 && big) sh.rotated = 1;
         }
     };
     // Odd-even ordering.  Columns sit at positions 0..n-1, team t holding 2t and 2t+1.
