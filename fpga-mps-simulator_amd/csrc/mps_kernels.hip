@@ -34,7 +34,7 @@ namespace {
 constexpr int kMaxChi = 128;             // largest chi_max of a task
 constexpr int kRegChi = 64;              // register SVD / MFMA contraction: 2*chi <= 128
 constexpr int kMaxSweeps = 60;
-// Stopping rule of the register SVD kernel: a sweep in which every rotation had
+// Stopping rule of the SVD kernels: a sweep in which every rotation had
 // |a^H b| <= kFinalSweepRot ||a|| ||b|| is the last one.  Cyclic Jacobi converges
 // quadratically at the end (on the chi-64 thetas the largest relative |a^H b| per sweep
 // runs ... 1e-3, 1e-6, 1e-12, then the 3e-15 rounding floor), so what is left after such a
@@ -1112,6 +1112,9 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
             if (col < n && r < m) W[(size_t)col * m + r] = src[idx];
         }
     };
+    // A sweep without rotations above kFinalSweepRot (relative) is the last one, as in
+    // the register kernel.
+    bool any_big = false;
     // Rotation of the pair (x, y) held as kBlkRpl rows per lane of one wave; true if
     // the pair was rotated.  nx, ny are the squared norms, exact when the block pair
     // was loaded and moved by t |g| per rotation since, so only a^H b is summed (a sweep
@@ -1127,7 +1130,9 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
         gr = wave_sum(gr);
         gi = wave_sum(gi);
         double c, sn, er, ei, tg;
+        const bool big = fma(gr, gr, gi * gi) > kFinalSweepRot * kFinalSweepRot * nx * ny;
         if (!jacobi_rotation(nx, ny, gr, gi, zero_sq, tol, c, sn, er, ei, tg)) return false;
+        if (big) any_big = true;
         nx = fmax(nx - tg, 0.0);
         ny += tg;
         const cd2 sem = mk2(sn * er, -sn * ei);
@@ -1146,7 +1151,7 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
     for (; sweeps < kMaxSweeps; ++sweeps) {
         if (tid == 0) sh.rotated = 0;
         __syncthreads();
-        bool any_local = false;
+        any_big = false;
         const int orounds = nbp - 1;
         for (int ro = 0; ro < orounds; ++ro) {
             for (int pi = 0; pi < PB; ++pi) {
@@ -1186,7 +1191,6 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
                             double nx = sh.bn[half * kBlkCols + ca];
                             double ny = sh.bn[half * kBlkCols + cb];
                             if (rotate_pair(x, y, nx, ny)) {
-                                any_local = true;
 #pragma unroll
                                 for (int k = 0; k < kBlkRpl; ++k) {
                                     base[ca * kBlkLd + lane + 64 * k] = x[k];
@@ -1214,7 +1218,6 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
                         for (int k = 0; k < kBlkRpl; ++k) y[k] = second[jc * kBlkLd + lane + 64 * k];
                         double ny = sh.bn[kBlkCols + jc];
                         if (rotate_pair(x, y, nx, ny)) {
-                            any_local = true;
 #pragma unroll
                             for (int k = 0; k < kBlkRpl; ++k)
                                 second[jc * kBlkLd + lane + 64 * k] = y[k];
@@ -1231,7 +1234,7 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
                 __syncthreads();
             }
         }
-        if (any_local && lane == 0) sh.rotated = 1;
+        if (any_big && lane == 0) sh.rotated = 1;
         __syncthreads();
         const int any = sh.rotated;
         __syncthreads();
