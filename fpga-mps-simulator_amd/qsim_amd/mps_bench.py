@@ -41,6 +41,24 @@ def contraction_flops(cl: int, cm: int, cr: int) -> float:
     return 32.0 * cl * cr * (cm + 4)
 
 
+def svd_qr_flops(nrow: int, ncol: int) -> float:
+    """Algorithmic real flops of the preconditioning Householder QR of the nrow x ncol
+    complex matrix theta^H (nrow = 2 chi_r, ncol = 2 chi_l): for each of the
+    k = min(nrow, ncol) reflections, every later column forms v^H x and x - t v over the
+    rows at and below the diagonal (16 real flops per row and column): about
+    8 (2 nrow ncol k - (nrow + ncol) k^2 + 2 k^3 / 3)."""
+    k = min(nrow, ncol)
+    return 8.0 * (2.0 * nrow * ncol * k - (nrow + ncol) * k * k + 2.0 * k ** 3 / 3.0)
+
+
+def svd_update_flops(cl: int, cr: int, sweeps: int) -> float:
+    """Algorithmic real flops of one update's SVD: the QR of theta^H (2 chi_r x 2 chi_l)
+    plus `sweeps` one-sided Jacobi sweeps over L = R^H (2 chi_l rows, min(2 chi_l, 2 chi_r)
+    columns)."""
+    m, n = 2 * cl, 2 * cr
+    return svd_qr_flops(n, m) + sweeps * svd_sweep_flops(m, min(m, n))
+
+
 def svd_sweep_flops(m: int, ncol: int) -> float:
     """Algorithmic real flops of one one-sided Jacobi sweep over an m x ncol complex
     matrix: every column pair forms a^H b (8 m) and is rotated (two columns, 10 m each:
@@ -96,7 +114,7 @@ def run_contraction_timed(layers):
             sw = ch.last_sweeps()
             for (i, _), nsw in zip(batch, sw):
                 flops += contraction_flops(b[i], b[i + 1], b[i + 2])
-                svd_flops += float(nsw) * svd_sweep_flops(int(2 * b[i]), int(2 * b[i + 2]))
+                svd_flops += svd_update_flops(int(b[i]), int(b[i + 2]), int(nsw))
             sweeps.extend(int(v) for v in sw)
         torch.cuda.synchronize()
         cap = 8 * len(layers)
@@ -199,8 +217,9 @@ def run(plain_layers=(30, 31)) -> dict:
         },
         'svd': {
             'kernel': 'mps_svd_kernel<64,*>',
-            'model': 'per update: sweeps x (ncol(ncol-1)/2 x 28 m + 4 m ncol) real flops, '
-                     'm = 2 chi_l rows, ncol = 2 chi_r columns',
+            'model': 'per update: Householder QR of theta^H (2 chi_r x 2 chi_l) + sweeps x '
+                     '(ncol(ncol-1)/2 x 28 m + 4 m ncol) real flops of one-sided Jacobi on '
+                     'L = R^H, m = 2 chi_l rows, ncol = min(2 chi_l, 2 chi_r) columns',
             'mean_sweeps_per_update': float(np.mean(sweeps)),
             'algorithmic_tflop': svd_flops / 1e12,
             'seconds': t_svd,
