@@ -1021,20 +1021,24 @@ constexpr int kBigThreads = kBigTeams * kBigLanes;    // 1024
 
 __device__ __forceinline__ double big_sum(double v) { return team_sum<kBigLanes>(v); }
 
-// Column blocks of the chi-128 class: 16 columns of up to 256 rows; a wave holds a
-// column as 4 rows per lane.
-constexpr int kBlkCols = 16;
-constexpr int kBlkLd = 2 * kMaxChi;
-constexpr int kBlkRpl = kBlkLd / 64;
-
-__device__ __forceinline__ double wave_sum(double v) { return team_sum<64>(v); }
+// Column blocks of the chi-128 class: 32 columns of up to 256 rows.  In a block pair the
+// first block sits in registers, one column per 32-lane half-wave (8 rows per lane), the
+// second in LDS (column stride padded by two entries, so the two half-waves of a wave
+// read different banks); inside a block (the tournament's first round) pairs are rotated
+// by whole waves on the block in LDS (4 rows per lane).
+constexpr int kBlkCols = 32;
+constexpr int kBlkRows = 2 * kMaxChi;              // 256
+constexpr int kBlkLd = kBlkRows + 2;
+constexpr int kHalfLanes = 32;                     // lanes per register column
+constexpr int kHalfRpl = kBlkRows / kHalfLanes;    // 8
+constexpr int kWaveRpl = kBlkRows / 64;            // 4
 
 __device__ __forceinline__ int big_player(int slot, int r, int P) {
     return slot == 0 ? 0 : 1 + (slot - 1 + r) % (2 * P - 1);
 }
 
 struct BigShared {
-    double bn[2 * kBlkCols];   // squared norms of the two LDS blocks' columns
+    double bn[kBlkCols];       // squared norms of the LDS block's columns
     double sval[2 * kMaxChi];
     int order[2 * kMaxChi];
     int rank[2 * kMaxChi];
@@ -1045,11 +1049,49 @@ struct BigShared {
     double scale;
 };
 
+// One-sided Jacobi rotation of the column pair (x, y) held by a team of TL lanes (RPL rows
+// per lane): sums a^H b, rotates when |a^H b| > tol ||a|| ||b|| (jacobi_rotation) and
+// moves t |g| between the tracked squared norms.  True if the pair was rotated; any_big is
+// set when the rotation was above the stopping threshold.
+template <int TL, int RPL>
+__device__ __forceinline__ bool rotate_team(cd2 (&x)[RPL], cd2 (&y)[RPL], double& nx,
+                                            double& ny, double zero_sq, double tol,
+                                            bool& any_big) {
+    double gr = 0, gi = 0, gr2 = 0, gi2 = 0;
+#pragma unroll
+    for (int k = 0; k < RPL; k += 2) {
+        gr = fma(x[k].x, y[k].x, fma(x[k].y, y[k].y, gr));
+        gi = fma(x[k].x, y[k].y, fma(-x[k].y, y[k].x, gi));
+        if (k + 1 < RPL) {
+            gr2 = fma(x[k + 1].x, y[k + 1].x, fma(x[k + 1].y, y[k + 1].y, gr2));
+            gi2 = fma(x[k + 1].x, y[k + 1].y, fma(-x[k + 1].y, y[k + 1].x, gi2));
+        }
+    }
+    gr = team_sum<TL>(gr + gr2);
+    gi = team_sum<TL>(gi + gi2);
+    double c, sn, er, ei, tg;
+    const bool big = fma(gr, gr, gi * gi) > kFinalSweepRot * kFinalSweepRot * nx * ny;
+    if (!jacobi_rotation(nx, ny, gr, gi, zero_sq, tol, c, sn, er, ei, tg)) return false;
+    if (big) any_big = true;
+    nx = fmax(nx - tg, 0.0);
+    ny += tg;
+    const cd2 sem = mk2(sn * er, -sn * ei);
+    const cd2 sep = mk2(sn * er, sn * ei);
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+        const cd2 u = x[k], v = y[k];
+        const cd2 sy = cmul(sem, v), sx = cmul(sep, u);
+        x[k] = mk2(c * u.x - sy.x, c * u.y - sy.y);
+        y[k] = mk2(c * v.x + sx.x, c * v.y + sx.y);
+    }
+    return true;
+}
+
 __global__ void __launch_bounds__(kBigThreads)
 mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size_t slot,
                    int32_t* __restrict__ status) {
     __shared__ BigShared sh;
-    __shared__ cd2 blk[2 * kBlkCols * kBlkLd];   // one pair of column blocks (128 KiB)
+    __shared__ cd2 blk[kBlkCols * kBlkLd];       // the LDS block of a block pair (129 KiB)
     const DevTask& DT = tasks[blockIdx.x];
     const qs_mps_task& T = DT.t;
     Dims d;
@@ -1083,158 +1125,131 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
     const double tol = 2.220446049250313e-16 * sqrt((double)m);
 
     // Blocked one-sided Jacobi.  theta's columns are cut into blocks of kBlkCols; a
-    // sweep is a round-robin tournament over the blocks.  For each pair of blocks the
-    // workgroup loads both into LDS, rotates (in the tournament's first round, where
-    // every block appears once) all pairs inside each block, then every cross pair in
-    // kBlkCols cyclic rounds (wave w keeps column w of the first block in registers
-    // and meets column (w + k) mod kBlkCols of the second in round k), and writes both
-    // back to W.  W is read and written once per block pair instead of once per
-    // column round, so the traffic of a sweep falls from about 2n full passes over
-    // theta to nb - 1.
+    // sweep is a round-robin tournament over the blocks.  In the tournament's first
+    // round every block appears once: its inner pairs are rotated on it in LDS (31
+    // cyclic rounds, wave w takes pair w).  Then, for each block pair of every round,
+    // the first block goes to registers (half-wave h holds its column h) and the second
+    // to LDS, and the 32 x 32 cross pairs are rotated in 32 cyclic rounds (half-wave h
+    // meets LDS column (h + k) mod 32 in round k); both blocks go back to W.  W is read
+    // and written once per block pair: about 2 (nb - 1) passes over theta per sweep.
     const int nb = (n + kBlkCols - 1) / kBlkCols;
     const int nbp = nb + (nb & 1);                // even, a dummy block when nb is odd
     const int PB = nbp / 2;
     const int wave = tid / 64, lane = tid % 64;
-    auto load_block = [&](int blkid, int half) {
-        cd2* dst = blk + half * kBlkCols * kBlkLd;
-        for (int idx = tid; idx < kBlkCols * kBlkLd; idx += kBigThreads) {
-            const int i = idx / kBlkLd, r = idx - i * kBlkLd;
+    const int half = tid / kHalfLanes, hl = tid % kHalfLanes;   // 32 half-waves
+    auto load_lds = [&](int blkid) {
+        for (int idx = tid; idx < kBlkCols * kBlkRows; idx += kBigThreads) {
+            const int i = idx / kBlkRows, r = idx - i * kBlkRows;
             const int col = blkid * kBlkCols + i;
-            dst[idx] = (blkid < nb && col < n && r < m) ? W[(size_t)col * m + r] : mk2(0, 0);
+            blk[i * kBlkLd + r] = (col < n && r < m) ? W[(size_t)col * m + r] : mk2(0, 0);
         }
     };
-    auto store_block = [&](int blkid, int half) {
-        if (blkid >= nb) return;
-        const cd2* src = blk + half * kBlkCols * kBlkLd;
-        for (int idx = tid; idx < kBlkCols * kBlkLd; idx += kBigThreads) {
-            const int i = idx / kBlkLd, r = idx - i * kBlkLd;
+    auto store_lds = [&](int blkid) {
+        for (int idx = tid; idx < kBlkCols * kBlkRows; idx += kBigThreads) {
+            const int i = idx / kBlkRows, r = idx - i * kBlkRows;
             const int col = blkid * kBlkCols + i;
-            if (col < n && r < m) W[(size_t)col * m + r] = src[idx];
+            if (col < n && r < m) W[(size_t)col * m + r] = blk[i * kBlkLd + r];
         }
     };
-    // A sweep without rotations above kFinalSweepRot (relative) is the last one, as in
-    // the register kernel.
+    // exact squared norms of the LDS block's columns (half-wave h: column h)
+    auto lds_norms = [&]() {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < kHalfRpl; ++k) {
+            const cd2 z = blk[half * kBlkLd + hl + kHalfLanes * k];
+            v = fma(z.x, z.x, fma(z.y, z.y, v));
+        }
+        v = team_sum<kHalfLanes>(v);
+        if (hl == 0) sh.bn[half] = v;
+    };
     bool any_big = false;
-    // Rotation of the pair (x, y) held as kBlkRpl rows per lane of one wave; true if
-    // the pair was rotated.  nx, ny are the squared norms, exact when the block pair
-    // was loaded and moved by t |g| per rotation since, so only a^H b is summed (a sweep
-    // without rotations tests every pair against exact norms).
-    auto rotate_pair = [&](cd2 (&x)[kBlkRpl], cd2 (&y)[kBlkRpl], double& nx,
-                           double& ny) -> bool {
-        double gr = 0, gi = 0;
-#pragma unroll
-        for (int k = 0; k < kBlkRpl; ++k) {
-            gr = fma(x[k].x, y[k].x, fma(x[k].y, y[k].y, gr));
-            gi = fma(x[k].x, y[k].y, fma(-x[k].y, y[k].x, gi));
-        }
-        gr = wave_sum(gr);
-        gi = wave_sum(gi);
-        double c, sn, er, ei, tg;
-        const bool big = fma(gr, gr, gi * gi) > kFinalSweepRot * kFinalSweepRot * nx * ny;
-        if (!jacobi_rotation(nx, ny, gr, gi, zero_sq, tol, c, sn, er, ei, tg)) return false;
-        if (big) any_big = true;
-        nx = fmax(nx - tg, 0.0);
-        ny += tg;
-        const cd2 sem = mk2(sn * er, -sn * ei);
-        const cd2 sep = mk2(sn * er, sn * ei);
-#pragma unroll
-        for (int k = 0; k < kBlkRpl; ++k) {
-            const cd2 u = x[k], v = y[k];
-            const cd2 sy = cmul(sem, v), sx = cmul(sep, u);
-            x[k] = mk2(c * u.x - sy.x, c * u.y - sy.y);
-            y[k] = mk2(c * v.x + sx.x, c * v.y + sx.y);
-        }
-        return true;
-    };
 
     int sweeps = 0;
     for (; sweeps < kMaxSweeps; ++sweeps) {
         if (tid == 0) sh.rotated = 0;
         __syncthreads();
         any_big = false;
-        const int orounds = nbp - 1;
-        for (int ro = 0; ro < orounds; ++ro) {
+        for (int ro = 0; ro < nbp - 1; ++ro) {
             for (int pi = 0; pi < PB; ++pi) {
                 const int bi = big_player(pi, ro, PB), bj = big_player(nbp - 1 - pi, ro, PB);
-                load_block(bi, 0);
-                load_block(bj, 1);
-                __syncthreads();
-                // exact squared norms of the 32 columns: wave w takes w and w + 16
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int cidx = wave + kBlkCols * h;
-                    double v = 0.0;
-#pragma unroll
-                    for (int k = 0; k < kBlkRpl; ++k) {
-                        const cd2 z = blk[cidx * kBlkLd + lane + 64 * k];
-                        v = fma(z.x, z.x, fma(z.y, z.y, v));
-                    }
-                    v = wave_sum(v);
-                    if (lane == 0) sh.bn[cidx] = v;
-                }
-                __syncthreads();
                 if (ro == 0) {
-                    // pairs inside each block: waves 0..7 the first, 8..15 the second
-                    const int half = wave / 8, pr = wave % 8;
-                    const bool real_blk = (half == 0 ? bi : bj) < nb;
-                    cd2* base = blk + half * kBlkCols * kBlkLd;
-                    for (int rd = 0; rd < kBlkCols - 1; ++rd) {
-                        if (real_blk) {
-                            const int ca = big_player(pr, rd, kBlkCols / 2);
-                            const int cb = big_player(kBlkCols - 1 - pr, rd, kBlkCols / 2);
-                            cd2 x[kBlkRpl], y[kBlkRpl];
+                    // pairs inside each real block of the pair, on the block in LDS
+                    for (int h2 = 0; h2 < 2; ++h2) {
+                        const int b = h2 == 0 ? bi : bj;
+                        if (b >= nb) continue;
+                        load_lds(b);
+                        __syncthreads();
+                        lds_norms();
+                        __syncthreads();
+                        for (int rd = 0; rd < kBlkCols - 1; ++rd) {
+                            const int ca = big_player(wave, rd, kBlkCols / 2);
+                            const int cb = big_player(kBlkCols - 1 - wave, rd, kBlkCols / 2);
+                            cd2 x[kWaveRpl], y[kWaveRpl];
 #pragma unroll
-                            for (int k = 0; k < kBlkRpl; ++k) {
-                                x[k] = base[ca * kBlkLd + lane + 64 * k];
-                                y[k] = base[cb * kBlkLd + lane + 64 * k];
+                            for (int k = 0; k < kWaveRpl; ++k) {
+                                x[k] = blk[ca * kBlkLd + lane + 64 * k];
+                                y[k] = blk[cb * kBlkLd + lane + 64 * k];
                             }
-                            double nx = sh.bn[half * kBlkCols + ca];
-                            double ny = sh.bn[half * kBlkCols + cb];
-                            if (rotate_pair(x, y, nx, ny)) {
+                            double nx = sh.bn[ca], ny = sh.bn[cb];
+                            if (rotate_team<64, kWaveRpl>(x, y, nx, ny, zero_sq, tol, any_big)) {
 #pragma unroll
-                                for (int k = 0; k < kBlkRpl; ++k) {
-                                    base[ca * kBlkLd + lane + 64 * k] = x[k];
-                                    base[cb * kBlkLd + lane + 64 * k] = y[k];
+                                for (int k = 0; k < kWaveRpl; ++k) {
+                                    blk[ca * kBlkLd + lane + 64 * k] = x[k];
+                                    blk[cb * kBlkLd + lane + 64 * k] = y[k];
                                 }
                                 if (lane == 0) {
-                                    sh.bn[half * kBlkCols + ca] = nx;
-                                    sh.bn[half * kBlkCols + cb] = ny;
+                                    sh.bn[ca] = nx;
+                                    sh.bn[cb] = ny;
                                 }
                             }
+                            __syncthreads();
                         }
+                        store_lds(b);
                         __syncthreads();
                     }
                 }
-                if (bi < nb && bj < nb) {
-                    cd2 x[kBlkRpl];
+                if (bi >= nb || bj >= nb) continue;
+                // first block into registers (half-wave h: column h), second into LDS
+                cd2 x[kHalfRpl];
+                {
+                    const int col = bi * kBlkCols + half;
+                    double v = 0.0;
 #pragma unroll
-                    for (int k = 0; k < kBlkRpl; ++k) x[k] = blk[wave * kBlkLd + lane + 64 * k];
-                    double nx = sh.bn[wave];
-                    cd2* second = blk + kBlkCols * kBlkLd;
-                    for (int kk = 0; kk < kBlkCols; ++kk) {
-                        const int jc = (wave + kk) % kBlkCols;
-                        cd2 y[kBlkRpl];
-#pragma unroll
-                        for (int k = 0; k < kBlkRpl; ++k) y[k] = second[jc * kBlkLd + lane + 64 * k];
-                        double ny = sh.bn[kBlkCols + jc];
-                        if (rotate_pair(x, y, nx, ny)) {
-#pragma unroll
-                            for (int k = 0; k < kBlkRpl; ++k)
-                                second[jc * kBlkLd + lane + 64 * k] = y[k];
-                            if (lane == 0) sh.bn[kBlkCols + jc] = ny;
-                        }
-                        __syncthreads();
+                    for (int k = 0; k < kHalfRpl; ++k) {
+                        const int r = hl + kHalfLanes * k;
+                        x[k] = (col < n && r < m) ? W[(size_t)col * m + r] : mk2(0, 0);
+                        v = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, v));
                     }
-#pragma unroll
-                    for (int k = 0; k < kBlkRpl; ++k) blk[wave * kBlkLd + lane + 64 * k] = x[k];
+                    double nx = team_sum<kHalfLanes>(v);
+                    load_lds(bj);
                     __syncthreads();
+                    lds_norms();
+                    __syncthreads();
+                    for (int kk = 0; kk < kBlkCols; ++kk) {
+                        const int jc = (half + kk) % kBlkCols;
+                        cd2 y[kHalfRpl];
+#pragma unroll
+                        for (int k = 0; k < kHalfRpl; ++k) y[k] = blk[jc * kBlkLd + hl + kHalfLanes * k];
+                        double ny = sh.bn[jc];
+                        if (rotate_team<kHalfLanes, kHalfRpl>(x, y, nx, ny, zero_sq, tol, any_big)) {
+#pragma unroll
+                            for (int k = 0; k < kHalfRpl; ++k)
+                                blk[jc * kBlkLd + hl + kHalfLanes * k] = y[k];
+                            if (hl == 0) sh.bn[jc] = ny;
+                        }
+                        __syncthreads();
+                    }
+#pragma unroll
+                    for (int k = 0; k < kHalfRpl; ++k) {
+                        const int r = hl + kHalfLanes * k;
+                        if (col < n && r < m) W[(size_t)col * m + r] = x[k];
+                    }
                 }
-                store_block(bi, 0);
-                store_block(bj, 1);
+                store_lds(bj);
                 __syncthreads();
             }
         }
-        if (any_big && lane == 0) sh.rotated = 1;
+        if (any_big) sh.rotated = 1;
         __syncthreads();
         const int any = sh.rotated;
         __syncthreads();
