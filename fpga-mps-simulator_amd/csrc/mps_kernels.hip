@@ -1087,6 +1087,142 @@ __device__ __forceinline__ bool rotate_team(cd2 (&x)[RPL], cd2 (&y)[RPL], double
     return true;
 }
 
+// Householder QR of the n x n matrix X in W (column-major, leading dimension n) in panels
+// of kBlkCols columns, then W := L = R^H in place.  Panel p sits in registers (half-wave
+// h: column 32 p + h, 8 rows per lane); its 32 reflections are formed one after the other
+// (half-wave j forms v = x - beta e_jj from its column, beta = -e^{i arg x_jj} ||x||,
+// writes v into row j of V in LDS and R's column into its registers; the later columns
+// of the panel apply H_j = I - tau_j v v^H at once, one barrier per reflection), then
+// every trailing block of 32 columns is loaded, takes the panel's 32 reflections in
+// order and is written back: W is read and written once per panel and trailing block.
+// tau lives in `taus` (kBlkCols doubles of LDS), V in `vbuf` (kBlkCols x kBlkLd).
+__device__ __noinline__ void big_lq_precondition(cd2* __restrict__ W, int n,
+                                                 cd2* __restrict__ vbuf,
+                                                 double* __restrict__ taus, int tid) {
+    const int half = tid / kHalfLanes, hl = tid % kHalfLanes;
+    const int nbk = (n + kBlkCols - 1) / kBlkCols;
+    auto reflect = [&](cd2 (&x)[kHalfRpl], int j) __attribute__((always_inline)) {
+        const double tau2 = taus[j];
+        if (tau2 == 0.0) return;
+        const cd2* v = vbuf + j * kBlkLd;
+        double wr = 0.0, wi = 0.0;
+#pragma unroll
+        for (int k = 0; k < kHalfRpl; ++k) {
+            const cd2 vk = v[hl + kHalfLanes * k];
+            wr = fma(vk.x, x[k].x, fma(vk.y, x[k].y, wr));
+            wi = fma(vk.x, x[k].y, fma(-vk.y, x[k].x, wi));
+        }
+        wr = team_sum<kHalfLanes>(wr) * tau2;
+        wi = team_sum<kHalfLanes>(wi) * tau2;
+#pragma unroll
+        for (int k = 0; k < kHalfRpl; ++k) {
+            const cd2 vk = v[hl + kHalfLanes * k];
+            x[k].x -= fma(wr, vk.x, -wi * vk.y);
+            x[k].y -= fma(wr, vk.y, wi * vk.x);
+        }
+    };
+    for (int p = 0; p < nbk; ++p) {
+        const int col = p * kBlkCols + half;
+        cd2 x[kHalfRpl];
+#pragma unroll
+        for (int k = 0; k < kHalfRpl; ++k) {
+            const int r = hl + kHalfLanes * k;
+            x[k] = (col < n && r < n) ? W[(size_t)col * n + r] : mk2(0, 0);
+        }
+        for (int j = 0; j < kBlkCols; ++j) {
+            const int jj = p * kBlkCols + j;
+            if (jj >= n) break;
+            if (half == j) {
+                double s2 = 0.0, ar = 0.0, ai = 0.0;
+#pragma unroll
+                for (int k = 0; k < kHalfRpl; ++k) {
+                    const int r = hl + kHalfLanes * k;
+                    if (r > jj) s2 = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, s2));
+                    if (r == jj) {
+                        ar = x[k].x;
+                        ai = x[k].y;
+                    }
+                }
+                s2 = team_sum<kHalfLanes>(s2);
+                ar = team_sum<kHalfLanes>(ar);
+                ai = team_sum<kHalfLanes>(ai);
+                const double aa = fma(ar, ar, ai * ai);
+                const double nrm = sqrt(s2 + aa);
+                double br = 0.0, bi = 0.0, tau2 = 0.0;
+                if (nrm > 0.0) {
+                    if (aa > 0.0) {
+                        const double f = -nrm / sqrt(aa);
+                        br = f * ar;
+                        bi = f * ai;
+                    } else {
+                        br = -nrm;
+                    }
+                    const double vr = ar - br, vi = ai - bi;
+                    tau2 = 2.0 / (s2 + fma(vr, vr, vi * vi));
+                }
+#pragma unroll
+                for (int k = 0; k < kHalfRpl; ++k) {
+                    const int r = hl + kHalfLanes * k;
+                    cd2 v = mk2(0.0, 0.0);
+                    if (r > jj) v = x[k];
+                    if (r == jj) v = mk2(ar - br, ai - bi);
+                    vbuf[j * kBlkLd + r] = v;
+                    if (r == jj) x[k] = mk2(br, bi);
+                    if (r > jj) x[k] = mk2(0.0, 0.0);
+                }
+                if (hl == 0) taus[j] = tau2;
+            }
+            __syncthreads();
+            if (half > j && col < n) reflect(x, j);
+        }
+#pragma unroll
+        for (int k = 0; k < kHalfRpl; ++k) {
+            const int r = hl + kHalfLanes * k;
+            if (col < n && r < n) W[(size_t)col * n + r] = x[k];
+        }
+        const int nref = n - p * kBlkCols < kBlkCols ? n - p * kBlkCols : kBlkCols;
+        for (int q = p + 1; q < nbk; ++q) {
+            const int cq = q * kBlkCols + half;
+            if (cq >= n) continue;
+            cd2 y[kHalfRpl];
+#pragma unroll
+            for (int k = 0; k < kHalfRpl; ++k) {
+                const int r = hl + kHalfLanes * k;
+                y[k] = r < n ? W[(size_t)cq * n + r] : mk2(0, 0);
+            }
+            for (int j = 0; j < nref; ++j) reflect(y, j);
+#pragma unroll
+            for (int k = 0; k < kHalfRpl; ++k) {
+                const int r = hl + kHalfLanes * k;
+                if (r < n) W[(size_t)cq * n + r] = y[k];
+            }
+        }
+        __syncthreads();   // V and tau are rewritten by the next panel
+    }
+    // W := R^H in place (R upper triangular, zero below the diagonal), by 32 x 32 tile
+    // pairs through LDS: tiles (I, J) and (J, I) are read, conjugated and written back
+    // transposed.  Thread (i, j) of the 1024 handles entry (i, j) of both tiles.
+    cd2* ta = vbuf;
+    cd2* tb = vbuf + kBlkCols * (kBlkCols + 1);
+    const int ti = tid / kBlkCols, tj = tid % kBlkCols;
+    for (int I = 0; I < nbk; ++I) {
+        for (int J = I; J < nbk; ++J) {
+            // entry (row, col) of W = R[row][col]; tile (I, J) covers rows 32 I.., cols 32 J..
+            const int r1 = I * kBlkCols + tj, c1 = J * kBlkCols + ti;
+            const int r2 = J * kBlkCols + tj, c2 = I * kBlkCols + ti;
+            ta[ti * (kBlkCols + 1) + tj] = (r1 < n && c1 < n) ? W[(size_t)c1 * n + r1] : mk2(0, 0);
+            tb[ti * (kBlkCols + 1) + tj] = (r2 < n && c2 < n) ? W[(size_t)c2 * n + r2] : mk2(0, 0);
+            __syncthreads();
+            // L[r][c] = conj(R[c][r]): tile (I, J) of L comes from tile (J, I) of R
+            const cd2 u = tb[tj * (kBlkCols + 1) + ti];   // R[32 J + ti][32 I + tj]
+            const cd2 v = ta[tj * (kBlkCols + 1) + ti];   // R[32 I + ti][32 J + tj]
+            if (r1 < n && c1 < n) W[(size_t)c1 * n + r1] = mk2(u.x, -u.y);   // L[r1][c1]
+            if (r2 < n && c2 < n) W[(size_t)c2 * n + r2] = mk2(v.x, -v.y);   // L[r2][c2]
+            __syncthreads();
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kBigThreads)
 mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size_t slot,
                    int32_t* __restrict__ status) {
@@ -1105,11 +1241,21 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
     const cd2* th = ws + (size_t)blockIdx.x * slot;
     cd2* W = ws + (size_t)blockIdx.x * slot + slot / 2;
 
+    // Square theta (m == n, every update at full bond) is preconditioned as in the
+    // register kernel: W starts as X = theta^H (column c of X is the conjugated row c of
+    // theta, so W is the conjugated theta entry by entry), is factored X = Q R in place
+    // and replaced by L = R^H, on which the sweeps run (same singular values and left
+    // singular vectors as theta, about half the sweeps at chi 128).  Other shapes run on
+    // theta's columns directly.
+    const bool precond = m == n;
     double fpart = 0.0;
     for (int idx = tid; idx < m * n; idx += kBigThreads) {
         const int r = idx / n, c = idx - r * n;
         const cd2 v = th[idx];
-        W[(size_t)c * m + r] = v;
+        if (precond)
+            W[idx] = mk2(v.x, -v.y);
+        else
+            W[(size_t)c * m + r] = v;
         fpart = fma(v.x, v.x, fma(v.y, v.y, fpart));
     }
     fpart = big_sum(fpart);
@@ -1123,6 +1269,7 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
     __syncthreads();
     const double zero_sq = 1e-28 * sh.fro;
     const double tol = 2.220446049250313e-16 * sqrt((double)m);
+    if (precond) big_lq_precondition(W, n, blk, sh.bn, tid);
 
     // Blocked one-sided Jacobi.  theta's columns are cut into blocks of kBlkCols; a
     // sweep is a round-robin tournament over the blocks.  In the tournament's first
