@@ -1477,16 +1477,42 @@ mps_svd_big_kernel(const DevTask* __restrict__ tasks, cd2* __restrict__ ws, size
         const cd2 u = W[(size_t)sh.order[k] * m + r];
         Ao[idx] = mk2(u.x * f, u.y * f);
     }
-    // B' = sqrt(S_t) / S * U^H theta = sqrt(S_t) / S^2 * W_k^H theta.
+    // B' = sqrt(S_t) / S * U^H theta = sqrt(S_t) / S^2 * W_k^H theta, in passes of
+    // kBlkCols kept columns: the pass's columns of W go to LDS, thread (c, g) (column c of
+    // theta, group g of 1024 / 256) accumulates 8 of them over the rows with theta[r][c]
+    // read once per pass (theta is read about keep / 8 times instead of keep times).
     cd2* Bo = site_ptr(T, T.site + 1);
-    for (int idx = tid; idx < keep * n; idx += kBigThreads) {
-        const int k = idx / n, c = idx - k * n;
-        const cd2* wk = W + (size_t)sh.order[k] * m;
-        cd2 acc = mk2(0, 0);
-        for (int r = 0; r < m; ++r) acc = cjmadd(acc, wk[r], th[(size_t)r * n + c]);
-        const double sv = sh.sval[sh.order[k]];
-        const double f = sv > 0 ? sqrt(sv * scale) / (sv * sv) : 0.0;
-        Bo[idx] = mk2(acc.x * f, acc.y * f);
+    constexpr int kGroups = kBigThreads / kBlkRows;            // 4
+    constexpr int kPerGroup = kBlkCols / kGroups;              // 8
+    for (int k0 = 0; k0 < keep; k0 += kBlkCols) {
+        __syncthreads();
+        for (int idx = tid; idx < kBlkCols * kBlkRows; idx += kBigThreads) {
+            const int i = idx / kBlkRows, r = idx - i * kBlkRows;
+            const int k = k0 + i;
+            blk[i * kBlkLd + r] = (k < keep && r < m) ? W[(size_t)sh.order[k] * m + r] : mk2(0, 0);
+        }
+        __syncthreads();
+        const int c = tid % kBlkRows, g = tid / kBlkRows;
+        if (c < n) {
+            cd2 acc[kPerGroup];
+#pragma unroll
+            for (int q = 0; q < kPerGroup; ++q) acc[q] = mk2(0, 0);
+            for (int r = 0; r < m; ++r) {
+                const cd2 t = th[(size_t)r * n + c];
+#pragma unroll
+                for (int q = 0; q < kPerGroup; ++q)
+                    acc[q] = cjmadd(acc[q], blk[(g * kPerGroup + q) * kBlkLd + r], t);
+            }
+#pragma unroll
+            for (int q = 0; q < kPerGroup; ++q) {
+                const int k = k0 + g * kPerGroup + q;
+                if (k < keep) {
+                    const double sv = sh.sval[sh.order[k]];
+                    const double f = sv > 0 ? sqrt(sv * scale) / (sv * sv) : 0.0;
+                    Bo[(size_t)k * n + c] = mk2(acc[q].x * f, acc[q].y * f);
+                }
+            }
+        }
     }
 }
 
