@@ -14,6 +14,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -58,6 +59,48 @@ void head_constants(const DevGate& G, int* r0, int* r1, int* qa, int* qb, int* a
     }
 }
 
+// A +-1 phase group (DG_PHASE_SIGN) with its sign descriptor as constants: the per-thread
+// negation mask of sign_mask() reduces to the terms the group has (the interpreter
+// rebuilds it from all eight thread-bit words and their pair rows), and its block factors
+// are read from the group's slot.  The same mask as sign_mask(), so the same result.
+std::string sign_group_code(const DevGate& G) {
+    SignDesc Q;
+    std::memcpy(&Q, G.m, sizeof(Q));
+    char buf[160];
+    std::string c = "    {\n";
+    std::snprintf(buf, sizeof(buf), "        unsigned M = %uu;\n", Q.m0);
+    c += buf;
+    for (int b = 0; b < SV_THREAD_BITS; ++b) {
+        if (!Q.mb[b]) continue;
+        std::snprintf(buf, sizeof(buf),
+                      "        M ^= (0u - ((unsigned)(t >> %d) & 1u)) & %uu;\n", b, Q.mb[b]);
+        c += buf;
+    }
+    bool any_pp = false;
+    for (int b = 0; b < SV_THREAD_BITS; ++b) any_pp = any_pp || Q.pp[b];
+    if (any_pp) {
+        c += "        unsigned par = 0;\n";
+        for (int a = 0; a < SV_THREAD_BITS - 1; ++a) {
+            if (!Q.pp[a]) continue;
+            std::snprintf(buf, sizeof(buf),
+                          "        par ^= (unsigned)(t >> %d) & (unsigned)__builtin_popcount(%uu & (unsigned)t);\n",
+                          a, Q.pp[a]);
+            c += buf;
+        }
+        c += "        M ^= 0u - (par & 1u);\n";
+    }
+    if (Q.slot >= 0) {
+        std::snprintf(buf, sizeof(buf),
+                      "        const const_cd2_ptr bf = (const_cd2_ptr)(tile_factors + (size_t)%d * ntiles * PHASE_BLOCK_VALUES);\n",
+                      Q.slot);
+        c += buf;
+    } else {
+        c += "        const const_cd2_ptr bf = nullptr;\n";
+    }
+    c += "        apply_phase_sign<R>(a, M & 0xFFFFu, bf, t);\n    }\n";
+    return c;
+}
+
 // Kernel text of pass P without its name (the name is spliced in at "@NAME@").
 std::string pass_body(const SvPlan& plan, const DevPass& P) {
     std::string s;
@@ -80,6 +123,10 @@ std::string pass_body(const SvPlan& plan, const DevPass& P) {
     char line[320];
     for (int i = 0; i < P.nop; ++i) {
         const DevGate& G = plan.gates[(size_t)P.gate0 + i];
+        if (G.kind == DG_PHASE_SIGN) {
+            s += sign_group_code(G);
+            continue;
+        }
         int r0, r1, qa, qb, aux;
         uint64_t cm;
         head_constants(G, &r0, &r1, &qa, &qb, &aux, &cm);
