@@ -104,7 +104,7 @@ std::string sign_group_code(const DevGate& G) {
 // Kernel text of pass P without its name (the name is spliced in at "@NAME@").
 std::string pass_body(const SvPlan& plan, const DevPass& P) {
     std::string s;
-    s.reserve(512 + 200 * (size_t)P.nop);
+    s.reserve(1024 + 240 * (size_t)P.nop);
     s += "extern \"C\" __global__ void __launch_bounds__(1 << (qs::SV_TILE_BITS - qs::SV_REG_BITS))\n"
          "@NAME@(qs::cd2* __restrict__ state, const qs::DevPass* __restrict__ P,\n"
          "       const qs::DevGate* __restrict__ gates, const qs::cd2* __restrict__ tfactors,\n"
@@ -120,11 +120,54 @@ std::string pass_body(const SvPlan& plan, const DevPass& P) {
          "    cd2 a[1 << R];\n"
          "    pass_load<R>(a, Lcur, state, P, base, t);\n"
          "    const DevGate* gt = gates + P->gate0;\n";
-    char line[320];
+    char line[512];
+    // Complex phase groups read 5 per-thread factors from a table (L2-resident): each
+    // group's reads are issued after the previous group (the first group's after the tile
+    // loads), so that they are in flight during the ops in between instead of costing
+    // one round trip at the group.
+    std::vector<int> phase_ops;
+    for (int i = 0; i < P.nop; ++i)
+        if (plan.gates[(size_t)P.gate0 + i].kind == DG_PHASE) phase_ops.push_back(i);
+    auto table_loads = [&](int i) {
+        const DevGate& G = plan.gates[(size_t)P.gate0 + i];
+        std::snprintf(line, sizeof(line),
+                      "    cd2 ph%d[PHASE_THREAD_VALUES];\n"
+                      "#pragma unroll\n"
+                      "    for (int k = 0; k < PHASE_THREAD_VALUES; ++k) ph%d[k] = tfactors[((size_t)%d + t) * PHASE_THREAD_VALUES + k];\n",
+                      i, i, G.qb);
+        s += line;
+    };
+    // load point of each group's table: after op load_after[g], the previous group (-1:
+    // before the first op).  Measured against loading at the group (0.3 ms per 28-qubit
+    // step faster) and against also waiting for a transpose in between (as fast overall,
+    // individual passes either way).
+    std::vector<int> load_after(phase_ops.size());
+    for (size_t g = 0; g < phase_ops.size(); ++g) load_after[g] = g ? phase_ops[g - 1] : -1;
+    auto loads_after = [&](int i) {
+        for (size_t g = 0; g < phase_ops.size(); ++g)
+            if (load_after[g] == i) table_loads(phase_ops[g]);
+    };
+    loads_after(-1);
     for (int i = 0; i < P.nop; ++i) {
         const DevGate& G = plan.gates[(size_t)P.gate0 + i];
+        if (i > 0) loads_after(i - 1);
+        std::snprintf(line, sizeof(line), "    // op %d kind %d\n", i, G.kind);
+        s += line;
         if (G.kind == DG_PHASE_SIGN) {
             s += sign_group_code(G);
+            continue;
+        }
+        if (G.kind == DG_PHASE) {
+            // op_body's DG_PHASE branch with the table entry loaded ahead
+            std::snprintf(line, sizeof(line),
+                          "    { const OpHead H = op_head(gt + %d, %d, %d, 0, 0, %d, %d, 0ull); "
+                          "const const_cd2_ptr bf = %s; "
+                          "apply_phase_cplx<R>(a, H.m, ph%d, bf, t, %d); }\n",
+                          i, G.kind, G.r0, G.qb, G.aux,
+                          G.aux >= 0 ? "(const_cd2_ptr)(tile_factors + (size_t)H.aux * ntiles * PHASE_BLOCK_VALUES)"
+                                     : "nullptr",
+                          i, G.r0);
+            s += line;
             continue;
         }
         int r0, r1, qa, qb, aux;

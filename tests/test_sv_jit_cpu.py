@@ -18,7 +18,8 @@ def _ops(pj):
 
 
 def test_source_runs_the_plan_ops_in_order():
-    """One op_body call per op of the pass, with the plan's kinds in the plan's order."""
+    """One op per op of the pass (marked in the source), with the plan's kinds in the plan's
+    order."""
     rng = np.random.default_rng(21)
     nq = 14
     recs, _ = gates_to_records(random_circuit(rng, nq, 250), nq)
@@ -26,8 +27,10 @@ def test_source_runs_the_plan_ops_in_order():
     assert len(kinds) >= 2
     for p, want in enumerate(kinds):
         src = jit_source(nq, recs, p)
-        got = [int(k) for k in re.findall(r'op_body<R>\(a, Lcur, H, gt\[\d+\]\.m, (\d+),', src)]
+        got = [int(k) for k in re.findall(r'// op \d+ kind (\d+)', src)]
         assert got == want
+        idx = [int(k) for k in re.findall(r'// op (\d+) kind', src)]
+        assert idx == list(range(len(want)))
         assert 'qs_sv_pass_0' in src and 'pass_load<R>' in src and 'pass_store<R>' in src
 
 
