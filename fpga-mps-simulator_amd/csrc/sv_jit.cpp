@@ -101,6 +101,23 @@ std::string sign_group_code(const DevGate& G) {
     return c;
 }
 
+// A DevLayout as a constant initializer (regs, rq, tpos, tq).
+std::string layout_literal(const DevLayout& D) {
+    std::string c = "{{";
+    auto arr = [&](const int32_t* v, int n) {
+        for (int i = 0; i < n; ++i) c += (i ? "," : "") + std::to_string(v[i]);
+    };
+    arr(D.regs, 4);
+    c += "},{";
+    arr(D.rq, 4);
+    c += "},{";
+    arr(D.tpos, SV_THREAD_BITS);
+    c += "},{";
+    arr(D.tq, SV_THREAD_BITS);
+    c += "}}";
+    return c;
+}
+
 // Kernel text of pass P without its name (the name is spliced in at "@NAME@").
 std::string pass_body(const SvPlan& plan, const DevPass& P) {
     std::string s;
@@ -148,11 +165,23 @@ std::string pass_body(const SvPlan& plan, const DevPass& P) {
             if (load_after[g] == i) table_loads(phase_ops[g]);
     };
     loads_after(-1);
+    // The register layout in force, known here: the pass's load layout, then each
+    // DG_LAYOUT's target.  Transposes between two known layouts use constant LDS offsets.
+    DevLayout cur_layout = P.load;
     for (int i = 0; i < P.nop; ++i) {
         const DevGate& G = plan.gates[(size_t)P.gate0 + i];
         if (i > 0) loads_after(i - 1);
         std::snprintf(line, sizeof(line), "    // op %d kind %d\n", i, G.kind);
         s += line;
+        if (G.kind == DG_LAYOUT) {
+            DevLayout to;
+            std::memcpy(&to, G.m, sizeof(to));
+            s += "    { constexpr DevLayout from = " + layout_literal(cur_layout) +
+                 "; constexpr DevLayout to = " + layout_literal(to) +
+                 "; layout_const<R>(a, Lcur, from, to, lds, t); }\n";
+            cur_layout = to;
+            continue;
+        }
         if (G.kind == DG_PHASE_SIGN) {
             s += sign_group_code(G);
             continue;

@@ -533,13 +533,53 @@ __device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const double*
 }
 
 // LDS barrier for the tile transposes: waits for this wave's LDS accesses, then syncs
-// the workgroup.  Unlike __syncthreads() it carries no memory fence, so it never waits
-// for global memory operations in flight; the memory clobber keeps the compiler from
-// moving LDS accesses across it.
+// the workgroup.  The fences are workgroup-scope and restricted to the local (LDS)
+// address space, so the barrier never waits for global memory operations in flight
+// (unlike __syncthreads()), and the compiler keeps every LDS access on its side.  An
+// inline-asm "s_waitcnt; s_barrier" with a memory clobber is not enough for the latter:
+// with constant transpose offsets the compiler has moved a tile's LDS stores past such a
+// barrier.
 __device__ __forceinline__ void lds_barrier() {
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+
+// DG_LAYOUT with both layouts known when the kernel is compiled (per-plan kernels): the
+// register part of every local index is a constant and the thread part does not depend
+// on the register index, and since the two parts occupy disjoint bits,
+// lds_slot(thread | reg) = lds_slot(thread) + lds_slot(reg) (the low four bits cannot
+// carry).  Each LDS access is then the thread's base address plus a constant offset (an
+// immediate of the instruction), the same slots as op_body's DG_LAYOUT.
+template <int R>
+__device__ __forceinline__ void layout_const(cd2 (&a)[1 << R], Layout<R>& Lcur,
+                                             const DevLayout& from, const DevLayout& to,
+                                             cd2* __restrict__ lds, int t) {
+    Layout<R> LA;
+    make_layout<R>(LA, from, t);
+    cd2* pa = lds + lds_slot(LA.loc_t);
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        int rl = 0;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (j & (1 << i)) rl |= 1 << from.regs[i];
+        pa[lds_slot(rl)] = a[j];
+    }
+    lds_barrier();
+    make_layout<R>(Lcur, to, t);
+    const cd2* pb = lds + lds_slot(Lcur.loc_t);
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        int rl = 0;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (j & (1 << i)) rl |= 1 << to.regs[i];
+        a[j] = pb[lds_slot(rl)];
+    }
+    lds_barrier();
+}
 
 // Runs one op of a pass on the register block.  H holds the record's head (kind through
 // m[11]) and m the record's full matrix area (DevGate::m); `kind` equals H.kind.  The
