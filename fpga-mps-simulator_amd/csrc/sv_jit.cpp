@@ -118,6 +118,24 @@ std::string layout_literal(const DevLayout& D) {
     return c;
 }
 
+// Local position along which layout_split halves the transpose from -> to (R register
+// bits each): a position that is a register bit under both layouts keeps every thread
+// busy in both rounds; otherwise a register bit of one layout that is a thread bit of the
+// other, preferring the highest thread bit (bits 6 and 7 select the wave, so the threads
+// that sit a round out are whole waves).
+int split_position(const DevLayout& from, const DevLayout& to, int R) {
+    for (int i = 0; i < R; ++i)
+        for (int k = 0; k < R; ++k)
+            if (from.regs[i] == to.regs[k]) return from.regs[i];
+    int best = -1, best_bit = -1;
+    for (int b = 0; b < SV_THREAD_BITS; ++b)
+        for (int i = 0; i < R; ++i) {
+            if (to.tpos[b] == from.regs[i] && b > best_bit) best = from.regs[i], best_bit = b;
+            if (from.tpos[b] == to.regs[i] && b > best_bit) best = to.regs[i], best_bit = b;
+        }
+    return best;
+}
+
 // Kernel text of pass P without its name (the name is spliced in at "@NAME@").
 std::string pass_body(const SvPlan& plan, const DevPass& P) {
     std::string s;
@@ -128,8 +146,10 @@ std::string pass_body(const SvPlan& plan, const DevPass& P) {
          "       const qs::cd2* __restrict__ factors, uint32_t ntiles) {\n"
          "    using namespace qs;\n";
     s += "    constexpr int R = " + std::to_string(P.R) + ";\n";
-    s += "    __shared__ cd2 lds[kLdsSlots];\n"
-         "    const int t = threadIdx.x;\n"
+    // Full tiles transpose half a tile at a time through LDS (layout_split).
+    const bool split = P.K == SV_TILE_BITS && P.R == SV_REG_BITS;
+    s += split ? "    __shared__ cd2 lds[kLdsHalfSlots];\n" : "    __shared__ cd2 lds[kLdsSlots];\n";
+    s += "    const int t = threadIdx.x;\n"
          "    const uint32_t tile = blockIdx.x;\n"
          "    const cd2* tile_factors = factors + ((size_t)P->slot0 * ntiles + tile) * PHASE_BLOCK_VALUES;\n"
          "    const uint64_t base = tile_base(P, tile);\n"
@@ -177,8 +197,15 @@ std::string pass_body(const SvPlan& plan, const DevPass& P) {
             DevLayout to;
             std::memcpy(&to, G.m, sizeof(to));
             s += "    { constexpr DevLayout from = " + layout_literal(cur_layout) +
-                 "; constexpr DevLayout to = " + layout_literal(to) +
-                 "; layout_const<R>(a, Lcur, from, to, lds, t); }\n";
+                 "; constexpr DevLayout to = " + layout_literal(to) + "; ";
+            if (split) {
+                std::snprintf(line, sizeof(line),
+                              "layout_split<R>(a, Lcur, from, to, %d, lds, t); }\n",
+                              split_position(cur_layout, to, P.R));
+                s += line;
+            } else {
+                s += "layout_const<R>(a, Lcur, from, to, lds, t); }\n";
+            }
             cur_layout = to;
             continue;
         }

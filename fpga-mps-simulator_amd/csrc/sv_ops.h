@@ -581,6 +581,81 @@ __device__ __forceinline__ void layout_const(cd2 (&a)[1 << R], Layout<R>& Lcur,
     lds_barrier();
 }
 
+// LDS of a per-plan kernel that transposes half a tile at a time (layout_split): 2048
+// amplitudes plus one padding slot per 16.
+constexpr int kLdsHalfSlots = kLdsSlots / 2;
+
+// Local index l without bit p (the bits above p move down by one).
+__device__ __forceinline__ constexpr int drop_bit(int l, int p) {
+    return (l & ((1 << p) - 1)) | ((l >> (p + 1)) << p);
+}
+
+// Local index bits of register j under layout D.
+template <int R>
+__device__ __forceinline__ constexpr int reg_local(const DevLayout& D, int j) {
+    int rl = 0;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+        if (j & (1 << i)) rl |= 1 << D.regs[i];
+    return rl;
+}
+
+// DG_LAYOUT between two compile-time layouts through half a tile of LDS: round h moves the
+// amplitudes whose local bit p equals h.  Under a layout where p is register bit i, a
+// thread writes (or reads) its 8 registers with bit i = h; where p is thread bit b, the
+// threads with bit b of t = h write (or read) all 16.  Slots are indexed by the local
+// index without bit p, and as in layout_const each access is the thread's base plus a
+// constant (dropping bit p and the padding both keep disjoint bit sets additive).  With
+// half the LDS of layout_const, up to four workgroups fit on a CU instead of two.
+template <int R>
+__device__ __forceinline__ void layout_split(cd2 (&a)[1 << R], Layout<R>& Lcur,
+                                             const DevLayout& from, const DevLayout& to, int p,
+                                             cd2* __restrict__ lds, int t) {
+    constexpr int NR = 1 << R;
+    int fi = -1, ti = -1, fb = -1, tb = -1;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        if (from.regs[i] == p) fi = i;
+        if (to.regs[i] == p) ti = i;
+    }
+#pragma unroll
+    for (int b = 0; b < SV_THREAD_BITS; ++b) {
+        if (from.tpos[b] == p) fb = b;
+        if (to.tpos[b] == p) tb = b;
+    }
+    Layout<R> LA;
+    make_layout<R>(LA, from, t);
+    make_layout<R>(Lcur, to, t);
+    cd2* pa = lds + lds_slot(drop_bit(LA.loc_t & ~(1 << p), p));
+    const cd2* pb = lds + lds_slot(drop_bit(Lcur.loc_t & ~(1 << p), p));
+    cd2 b[NR];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (fi >= 0) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j)
+                if (((j >> fi) & 1) == h)
+                    pa[lds_slot(drop_bit(reg_local<R>(from, j) & ~(1 << p), p))] = a[j];
+        } else if (((t >> fb) & 1) == h) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j) pa[lds_slot(drop_bit(reg_local<R>(from, j), p))] = a[j];
+        }
+        lds_barrier();
+        if (ti >= 0) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j)
+                if (((j >> ti) & 1) == h)
+                    b[j] = pb[lds_slot(drop_bit(reg_local<R>(to, j) & ~(1 << p), p))];
+        } else if (((t >> tb) & 1) == h) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j) b[j] = pb[lds_slot(drop_bit(reg_local<R>(to, j), p))];
+        }
+        lds_barrier();
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) a[j] = b[j];
+}
+
 // Runs one op of a pass on the register block.  H holds the record's head (kind through
 // m[11]) and m the record's full matrix area (DevGate::m); `kind` equals H.kind.  The
 // op-stream interpreter passes the head it loaded; a per-plan kernel passes the kind,
