@@ -101,6 +101,14 @@ std::string sign_group_code(const DevGate& G) {
     return c;
 }
 
+// The launch bounds every pass kernel text carries.
+constexpr const char* kLaunchBounds = "__launch_bounds__(1 << (qs::SV_TILE_BITS - qs::SV_REG_BITS))";
+
+// A half-tile-LDS kernel (34 KiB, four workgroups per CU by LDS) that needs more than 168
+// VGPRs runs two waves per SIMD; asking the compiler for three often fits it without
+// spilling (the first light-cone pass: 172 -> 164 VGPRs, 3.9 -> 3.3 ms).
+constexpr int kThreeWaveVgprs = 168;
+
 // A DevLayout as a constant initializer (regs, rq, tpos, tq).
 std::string layout_literal(const DevLayout& D) {
     std::string c = "{{";
@@ -140,8 +148,8 @@ int split_position(const DevLayout& from, const DevLayout& to, int R) {
 std::string pass_body(const SvPlan& plan, const DevPass& P) {
     std::string s;
     s.reserve(1024 + 240 * (size_t)P.nop);
-    s += "extern \"C\" __global__ void __launch_bounds__(1 << (qs::SV_TILE_BITS - qs::SV_REG_BITS))\n"
-         "@NAME@(qs::cd2* __restrict__ state, const qs::DevPass* __restrict__ P,\n"
+    s += std::string("extern \"C\" __global__ void ") + kLaunchBounds + "\n";
+    s += "@NAME@(qs::cd2* __restrict__ state, const qs::DevPass* __restrict__ P,\n"
          "       const qs::DevGate* __restrict__ gates, const qs::cd2* __restrict__ tfactors,\n"
          "       const qs::cd2* __restrict__ factors, uint32_t ntiles) {\n"
          "    using namespace qs;\n";
@@ -241,13 +249,20 @@ std::string pass_body(const SvPlan& plan, const DevPass& P) {
 
 std::string kernel_name(size_t i) { return "qs_sv_pass_" + std::to_string(i); }
 
+
 // One hipRTC program over `bodies` (kernel i named kernel_name(i)); the code object into
 // *code.  The log of a failed compile goes into the error message.
-int compile_program(const std::vector<const std::string*>& bodies, std::vector<char>* code) {
+int compile_program(const std::vector<const std::string*>& bodies, std::vector<char>* code,
+                    bool three_waves = false) {
     std::string src = "#include \"sv_ops.h\"\n";
     for (size_t i = 0; i < bodies.size(); ++i) {
         std::string b = *bodies[i];
         b.replace(b.find("@NAME@"), 6, kernel_name(i));
+        if (three_waves) {
+            const size_t at = b.find(kLaunchBounds);
+            if (at != std::string::npos)
+                b.insert(at + std::strlen(kLaunchBounds), " __attribute__((amdgpu_waves_per_eu(3)))");
+        }
         src += b;
     }
     const char* headers[] = {kSvDeviceTypesSrc, kSvOpsSrc};
@@ -339,10 +354,32 @@ int sv_jit_specialize(const SvPlan& plan, int device, std::vector<hipFunction_t>
         if (st) return st;
         hipModule_t mod = nullptr;
         QS_HIP(hipModuleLoadData(&mod, code.data()));
+        std::vector<const std::string*> wide;
         for (size_t k = 0; k < todo.size(); ++k) {
             hipFunction_t f = nullptr;
             QS_HIP(hipModuleGetFunction(&f, mod, kernel_name(k).c_str()));
             g_cache[{device, *todo[k]}] = f;
+            int vgprs = 0;
+            QS_HIP(hipFuncGetAttribute(&vgprs, HIP_FUNC_ATTRIBUTE_NUM_REGS, f));
+            if (vgprs > kThreeWaveVgprs && todo[k]->find("kLdsHalfSlots") != std::string::npos)
+                wide.push_back(todo[k]);
+        }
+        if (!wide.empty()) {
+            // Second compile of those kernels for three waves per SIMD; a kernel that then
+            // spills keeps its first build.
+            std::vector<char> code3;
+            const int st3 = compile_program(wide, &code3, true);
+            if (st3) return st3;
+            hipModule_t mod3 = nullptr;
+            QS_HIP(hipModuleLoadData(&mod3, code3.data()));
+            for (size_t k = 0; k < wide.size(); ++k) {
+                hipFunction_t f = nullptr;
+                QS_HIP(hipModuleGetFunction(&f, mod3, kernel_name(k).c_str()));
+                int scratch = 0, vgprs = 0;
+                QS_HIP(hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f));
+                QS_HIP(hipFuncGetAttribute(&vgprs, HIP_FUNC_ATTRIBUTE_NUM_REGS, f));
+                if (scratch == 0 && vgprs <= kThreeWaveVgprs) g_cache[{device, *wide[k]}] = f;
+            }
         }
     }
     fns->resize(plan.passes.size());
