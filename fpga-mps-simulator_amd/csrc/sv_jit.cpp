@@ -144,6 +144,26 @@ int split_position(const DevLayout& from, const DevLayout& to, int R) {
     return best;
 }
 
+// How entry e of thread 0's row of a complex phase group's table (column e mod
+// PHASE_THREAD_VALUES) varies over the nthr threads: -1 when every thread has the same
+// value (bit for bit), b when the value depends on thread bit b alone, SV_THREAD_BITS
+// otherwise.
+int thread_dependence(const SvPlan& plan, size_t e, int nthr) {
+    const double* v = plan.thread_factors.data();
+    auto at = [&](int t) { return v + 2 * (e + (size_t)t * PHASE_THREAD_VALUES); };
+    auto same = [&](int t, int u) { return std::memcmp(at(t), at(u), 2 * sizeof(double)) == 0; };
+    int bits = 0;
+    for (int t = 1; t < nthr; ++t)
+        if (!same(t, 0)) bits |= t;
+    if (!bits) return -1;
+    for (int b = 0; (1 << b) < nthr; ++b) {
+        bool ok = true;
+        for (int t = 0; t < nthr && ok; ++t) ok = same(t, t & (1 << b));
+        if (ok) return b;
+    }
+    return SV_THREAD_BITS;
+}
+
 // Kernel text of pass P without its name (the name is spliced in at "@NAME@").
 std::string pass_body(const SvPlan& plan, const DevPass& P) {
     std::string s;
@@ -173,14 +193,37 @@ std::string pass_body(const SvPlan& plan, const DevPass& P) {
     std::vector<int> phase_ops;
     for (int i = 0; i < P.nop; ++i)
         if (plan.gates[(size_t)P.gate0 + i].kind == DG_PHASE) phase_ops.push_back(i);
+    // A table column (A_t or kappa_r(t)) that is the same for every thread, or depends on
+    // one thread bit only, is read as one or two uniform values (scalar loads of thread 0's
+    // and thread 2^b's entries) instead of one vector load per thread: the QFT's groups
+    // have one column that depends on several thread bits and four that do not.  The
+    // values are the table's own entries, so the group computes what the interpreter does.
+    const int nthr = 1 << (P.K - P.R);
     auto table_loads = [&](int i) {
         const DevGate& G = plan.gates[(size_t)P.gate0 + i];
-        std::snprintf(line, sizeof(line),
-                      "    cd2 ph%d[PHASE_THREAD_VALUES];\n"
-                      "#pragma unroll\n"
-                      "    for (int k = 0; k < PHASE_THREAD_VALUES; ++k) ph%d[k] = tfactors[((size_t)%d + t) * PHASE_THREAD_VALUES + k];\n",
-                      i, i, G.qb);
+        std::snprintf(line, sizeof(line), "    cd2 ph%d[PHASE_THREAD_VALUES];\n", i);
         s += line;
+        for (int k = 0; k < PHASE_THREAD_VALUES; ++k) {
+            const size_t e = (size_t)G.qb * PHASE_THREAD_VALUES + k;
+            switch (thread_dependence(plan, e, nthr)) {
+                case -1:
+                    std::snprintf(line, sizeof(line),
+                                  "    ph%d[%d] = ((const_cd2_ptr)tfactors)[%zu];\n", i, k, e);
+                    break;
+                case SV_THREAD_BITS:
+                    std::snprintf(line, sizeof(line),
+                                  "    ph%d[%d] = tfactors[%zu + (size_t)t * PHASE_THREAD_VALUES];\n",
+                                  i, k, e);
+                    break;
+                default: {
+                    const int b = thread_dependence(plan, e, nthr);
+                    std::snprintf(line, sizeof(line),
+                                  "    ph%d[%d] = ((t >> %d) & 1) ? ((const_cd2_ptr)tfactors)[%zu] : ((const_cd2_ptr)tfactors)[%zu];\n",
+                                  i, k, b, e + ((size_t)1 << b) * PHASE_THREAD_VALUES, e);
+                }
+            }
+            s += line;
+        }
     };
     // load point of each group's table: after op load_after[g], the previous group (-1:
     // before the first op).  Measured against loading at the group (0.3 ms per 28-qubit
