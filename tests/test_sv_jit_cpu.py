@@ -67,3 +67,71 @@ def test_narrow_state_compiles():
                                    ([('cx', [0, nq - 1], None)] if nq > 1 else []), nq)
         nk, nbytes = jit_check(nq, recs)
         assert nk >= 1 and nbytes > 0
+
+
+def _layout(lit):
+    g = [list(map(int, x.split(','))) for x in re.findall(r'\{([-\d,]*)\}', lit)]
+    return {'regs': g[0][:4], 'tpos': g[2]}
+
+
+def _split_moves(fr, to, p, nthr=256, nreg=16, half=2176):
+    """Model of layout_split's addressing (sv_ops.h): per round h, the LDS slot every
+    (thread, register) pair writes under `fr` and reads under `to`.  Returns the
+    (thread, register) -> local index map the reads produce."""
+    def slot(l):
+        return l + (l >> 4)
+
+    def drop(l):
+        return (l & ((1 << p) - 1)) | ((l >> (p + 1)) << p)
+
+    def loc_t(D, t):
+        return sum(((t >> b) & 1) << D['tpos'][b] for b in range(len(D['tpos'])))
+
+    def rl(D, j):
+        return sum(1 << D['regs'][i] for i in range(4) if (j >> i) & 1)
+
+    fi = fr['regs'].index(p) if p in fr['regs'] else -1
+    ti = to['regs'].index(p) if p in to['regs'] else -1
+    fb = -1 if fi >= 0 else fr['tpos'].index(p)
+    tb = -1 if ti >= 0 else to['tpos'].index(p)
+    got = {}
+    for h in range(2):
+        lds = {}
+        for t in range(nthr):
+            base = slot(drop(loc_t(fr, t) & ~(1 << p)))
+            for j in range(nreg):
+                if (fi >= 0 and ((j >> fi) & 1) != h) or (fi < 0 and ((t >> fb) & 1) != h):
+                    continue
+                s = base + slot(drop(rl(fr, j) & ~(1 << p)))
+                assert 0 <= s < half and s not in lds
+                lds[s] = loc_t(fr, t) | rl(fr, j)
+        for t in range(nthr):
+            base = slot(drop(loc_t(to, t) & ~(1 << p)))
+            for j in range(nreg):
+                if (ti >= 0 and ((j >> ti) & 1) != h) or (ti < 0 and ((t >> tb) & 1) != h):
+                    continue
+                got[(t, j)] = lds[base + slot(drop(rl(to, j) & ~(1 << p)))]
+    return got
+
+
+def test_split_transposes_move_every_amplitude():
+    """Each half-tile transpose the generator emits (split position included) writes every
+    amplitude once into 2176 LDS slots per round and reads back, for each (thread,
+    register) of the new layout, the amplitude with that local index (a model of
+    layout_split's addressing over the benchmark plan's first passes)."""
+    nq = 28
+    recs, _ = gates_to_records(workloads.sv_benchmark_circuit(nq, 20, 1234), nq)
+    pat = re.compile(r'constexpr DevLayout from = (\{\{.*?\}\}); constexpr DevLayout to = '
+                     r'(\{\{.*?\}\}); layout_split<R>\(a, Lcur, from, to, (\d+), lds, t\)')
+    n = 0
+    for p in (0, 2, 27):
+        for fr, to, pos in pat.findall(jit_source(nq, recs, p)):
+            fr, to = _layout(fr), _layout(to)
+            got = _split_moves(fr, to, int(pos))
+            for (t, j), v in got.items():
+                want = sum(((t >> b) & 1) << to['tpos'][b] for b in range(8)) | \
+                    sum(1 << to['regs'][i] for i in range(4) if (j >> i) & 1)
+                assert v == want
+            assert len(got) == 4096
+            n += 1
+    assert n >= 10
