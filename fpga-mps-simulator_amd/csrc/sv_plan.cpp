@@ -958,24 +958,26 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
 
     std::vector<char> done(items.size(), 0);
     size_t remaining = items.size();
-    while (remaining > 0) {
-        DevPass P{};
-        P.K = K;
-        P.R = R;
-        P.tile_is_all = (K == nq) ? 1 : 0;
-        uint64_t tile = low;
-        if (P.tile_is_all) tile = (nq >= 64) ? ~0ull : ((1ull << nq) - 1);
+    // Indices of the items not yet placed, in gate order.
+    std::vector<size_t> left(items.size());
+    for (size_t i = 0; i < items.size(); ++i) left[i] = i;
+    // One pass over the pending items in gate order: an item joins the pass unless it
+    // must wait for a deferred item it does not commute with, or (non-diagonal) one of
+    // its targets is not a tile qubit; with `extend` such a target joins the tile while
+    // it has room.  Returns the number of non-diagonal items that join (the pass's work;
+    // diagonal items cost little wherever they run), scanning at most `limit` items;
+    // `inc` (when not null) receives the joining items and `tile` the final tile.
+    auto scan = [&](uint64_t& tile, bool extend, size_t limit, std::vector<size_t>* inc) {
         uint64_t blocked_any = 0;   // qubits of deferred non-diagonal gates
         uint64_t blocked_nd = 0;    // qubits of deferred diagonal gates (block non-diagonal)
-        std::vector<const Item*> inc;
-        std::vector<size_t> inc_idx;
-        for (size_t i = 0; i < items.size(); ++i) {
-            if (done[i]) continue;
-            const Item& it = items[i];
+        int work = 0;
+        const size_t n = std::min(limit, left.size());
+        for (size_t k = 0; k < n; ++k) {
+            const Item& it = items[left[k]];
             bool defer = it.diag ? (it.touch & blocked_any) != 0
                                  : (it.touch & (blocked_any | blocked_nd)) != 0;
             if (!defer && (it.targets & ~tile)) {
-                if (popcount64(tile | it.targets) <= K)
+                if (extend && popcount64(tile | it.targets) <= K)
                     tile |= it.targets;
                 else
                     defer = true;
@@ -987,9 +989,62 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
                     blocked_any |= it.touch;
                 continue;
             }
-            inc.push_back(&it);
-            inc_idx.push_back(i);
+            if (!it.diag) ++work;
+            if (inc) inc->push_back(left[k]);
         }
+        return work;
+    };
+    while (remaining > 0) {
+        DevPass P{};
+        P.K = K;
+        P.R = R;
+        P.tile_is_all = (K == nq) ? 1 : 0;
+        uint64_t tile = low;
+        if (P.tile_is_all) tile = (nq >= 64) ? ~0ull : ((1ull << nq) - 1);
+        bool extend = true;
+        if (!P.tile_is_all) {
+            // The tile: of the tile the gate order fills first-come, a tile grown one
+            // qubit at a time by the most work, and tiles of one or two runs of
+            // consecutive qubits above the low ones, the one under which the most
+            // non-diagonal gates can run.  A layered nearest-neighbour circuit then
+            // advances a light cone over a whole window per pass instead of a chain of
+            // first-come qubits: the 28-qubit benchmark circuit takes 18 passes instead
+            // of 31.  Candidates are scored over the first kScoreItems pending items.
+            constexpr size_t kScoreItems = 8192;
+            uint64_t t0 = low;
+            int best = scan(t0, true, kScoreItems, nullptr);
+            uint64_t best_tile = t0;
+            auto consider = [&](uint64_t cand) {
+                uint64_t t = cand;
+                const int w = scan(t, false, kScoreItems, nullptr);
+                if (w > best) best = w, best_tile = cand, extend = false;
+            };
+            uint64_t grow = low;
+            while (popcount64(grow) < K) {
+                int gbest = -1, gq = -1;
+                for (int q = 0; q < nq; ++q) {
+                    if ((grow >> q) & 1) continue;
+                    uint64_t t = grow | (1ull << q);
+                    const int w = scan(t, false, kScoreItems, nullptr);
+                    if (w > gbest) gbest = w, gq = q;
+                }
+                grow |= 1ull << gq;
+            }
+            consider(grow);
+            const int F = K - L;   // tile qubits above the low ones
+            auto run = [](int a, int w) { return ((w >= 64 ? ~0ull : ((1ull << w) - 1)) << a); };
+            for (int a = L; a + F <= nq; ++a) consider(low | run(a, F));
+            for (int w1 = 1; w1 < F; ++w1)
+                for (int a = L; a + w1 < nq; ++a)
+                    for (int b = a + w1 + 1; b + (F - w1) <= nq; ++b)
+                        consider(low | run(a, w1) | run(b, F - w1));
+            tile = best_tile;
+        }
+        std::vector<size_t> inc_idx;
+        scan(tile, extend, left.size(), &inc_idx);
+        std::vector<const Item*> inc;
+        inc.reserve(inc_idx.size());
+        for (size_t i : inc_idx) inc.push_back(&items[i]);
         // Fill the tile up to K qubits so every pass covers 2^K amplitudes per block.
         for (int q = 0; q < nq && popcount64(tile) < K; ++q) tile |= 1ull << q;
         P.tile_mask = tile;
@@ -1005,6 +1060,8 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
         plan->passes.push_back(P);
         for (size_t i : inc_idx) done[i] = 1;
         remaining -= inc_idx.size();
+        left.erase(std::remove_if(left.begin(), left.end(), [&](size_t i) { return done[i] != 0; }),
+                   left.end());
         if (inc_idx.empty()) {
             set_error("planner made no progress (internal error)");
             return QS_E_ARG;

@@ -118,13 +118,14 @@ def test_split_transposes_move_every_amplitude():
     """Each half-tile transpose the generator emits (split position included) writes every
     amplitude once into 2176 LDS slots per round and reads back, for each (thread,
     register) of the new layout, the amplitude with that local index (a model of
-    layout_split's addressing over the benchmark plan's first passes)."""
+    layout_split's addressing over three passes of the benchmark plan)."""
     nq = 28
     recs, _ = gates_to_records(workloads.sv_benchmark_circuit(nq, 20, 1234), nq)
     pat = re.compile(r'constexpr DevLayout from = (\{\{.*?\}\}); constexpr DevLayout to = '
                      r'(\{\{.*?\}\}); layout_split<R>\(a, Lcur, from, to, (\d+), lds, t\)')
     n = 0
-    for p in (0, 2, 27):
+    npass = len(plan_json(nq, recs)['passes'])
+    for p in (0, 1, npass - 1):
         for fr, to, pos in pat.findall(jit_source(nq, recs, p)):
             fr, to = _layout(fr), _layout(to)
             got = _split_moves(fr, to, int(pos))
