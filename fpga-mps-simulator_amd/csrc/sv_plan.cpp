@@ -961,19 +961,20 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
     // Indices of the items not yet placed, in gate order.
     std::vector<size_t> left(items.size());
     for (size_t i = 0; i < items.size(); ++i) left[i] = i;
-    // One pass over the pending items in gate order: an item joins the pass unless it
-    // must wait for a deferred item it does not commute with, or (non-diagonal) one of
+    // One pass over the pending items `lst` in gate order: an item joins the pass unless
+    // it must wait for a deferred item it does not commute with, or (non-diagonal) one of
     // its targets is not a tile qubit; with `extend` such a target joins the tile while
     // it has room.  Returns the number of non-diagonal items that join (the pass's work;
     // diagonal items cost little wherever they run), scanning at most `limit` items;
     // `inc` (when not null) receives the joining items and `tile` the final tile.
-    auto scan = [&](uint64_t& tile, bool extend, size_t limit, std::vector<size_t>* inc) {
+    auto scan = [&](const std::vector<size_t>& lst, uint64_t& tile, bool extend, size_t limit,
+                    std::vector<size_t>* inc) {
         uint64_t blocked_any = 0;   // qubits of deferred non-diagonal gates
         uint64_t blocked_nd = 0;    // qubits of deferred diagonal gates (block non-diagonal)
         int work = 0;
-        const size_t n = std::min(limit, left.size());
+        const size_t n = std::min(limit, lst.size());
         for (size_t k = 0; k < n; ++k) {
-            const Item& it = items[left[k]];
+            const Item& it = items[lst[k]];
             bool defer = it.diag ? (it.touch & blocked_any) != 0
                                  : (it.touch & (blocked_any | blocked_nd)) != 0;
             if (!defer && (it.targets & ~tile)) {
@@ -990,10 +991,51 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
                 continue;
             }
             if (!it.diag) ++work;
-            if (inc) inc->push_back(left[k]);
+            if (inc) inc->push_back(lst[k]);
         }
         return work;
     };
+    // Candidate tiles of a pass over `lst`, with the work each admits: the tile the scan
+    // fills first-come (extend), a tile grown one qubit at a time by the most work, and
+    // every tile of one or two runs of consecutive qubits above the low ones.  Candidates
+    // are scored over the first kScoreItems pending items.
+    struct Cand {
+        int work;
+        uint64_t tile;
+        bool extend;
+    };
+    constexpr size_t kScoreItems = 8192;
+    const int F = K - L;   // tile qubits above the low ones
+    auto run = [](int a, int w) { return ((w >= 64 ? ~0ull : ((1ull << w) - 1)) << a); };
+    auto candidates = [&](const std::vector<size_t>& lst, std::vector<Cand>* out) {
+        out->clear();
+        uint64_t t0 = low;
+        const int w0 = scan(lst, t0, true, kScoreItems, nullptr);
+        out->push_back({w0, t0, true});
+        auto consider = [&](uint64_t cand) {
+            uint64_t t = cand;
+            out->push_back({scan(lst, t, false, kScoreItems, nullptr), cand, false});
+        };
+        uint64_t grow = low;
+        while (popcount64(grow) < K) {
+            int gbest = -1, gq = -1;
+            for (int q = 0; q < nq; ++q) {
+                if ((grow >> q) & 1) continue;
+                uint64_t t = grow | (1ull << q);
+                const int w = scan(lst, t, false, kScoreItems, nullptr);
+                if (w > gbest) gbest = w, gq = q;
+            }
+            grow |= 1ull << gq;
+        }
+        consider(grow);
+        for (int a = L; a + F <= nq; ++a) consider(low | run(a, F));
+        for (int w1 = 1; w1 < F; ++w1)
+            for (int a = L; a + w1 < nq; ++a)
+                for (int b = a + w1 + 1; b + (F - w1) <= nq; ++b)
+                    consider(low | run(a, w1) | run(b, F - w1));
+    };
+    std::vector<Cand> cands, next;
+    std::vector<size_t> trial, rest;
     while (remaining > 0) {
         DevPass P{};
         P.K = K;
@@ -1003,45 +1045,42 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
         if (P.tile_is_all) tile = (nq >= 64) ? ~0ull : ((1ull << nq) - 1);
         bool extend = true;
         if (!P.tile_is_all) {
-            // The tile: of the tile the gate order fills first-come, a tile grown one
-            // qubit at a time by the most work, and tiles of one or two runs of
-            // consecutive qubits above the low ones, the one under which the most
-            // non-diagonal gates can run.  A layered nearest-neighbour circuit then
-            // advances a light cone over a whole window per pass instead of a chain of
-            // first-come qubits: the 28-qubit benchmark circuit takes 18 passes instead
-            // of 31.  Candidates are scored over the first kScoreItems pending items.
-            constexpr size_t kScoreItems = 8192;
-            uint64_t t0 = low;
-            int best = scan(t0, true, kScoreItems, nullptr);
-            uint64_t best_tile = t0;
-            auto consider = [&](uint64_t cand) {
-                uint64_t t = cand;
-                const int w = scan(t, false, kScoreItems, nullptr);
-                if (w > best) best = w, best_tile = cand, extend = false;
-            };
-            uint64_t grow = low;
-            while (popcount64(grow) < K) {
-                int gbest = -1, gq = -1;
-                for (int q = 0; q < nq; ++q) {
-                    if ((grow >> q) & 1) continue;
-                    uint64_t t = grow | (1ull << q);
-                    const int w = scan(t, false, kScoreItems, nullptr);
-                    if (w > gbest) gbest = w, gq = q;
+            // The tile whose pass admits the most non-diagonal gates together with the best
+            // pass after it, over the kLookahead candidates of most work (ties: the earlier
+            // candidate, the first-come tile first).  Diagonal gates run under any tile, so
+            // in a layered circuit of one-qubit gates and nearest-neighbour diagonals a
+            // window of consecutive qubits advances a light cone: the 28-qubit benchmark
+            // takes 17 passes instead of the 31 of first-come tiles, whose windows chased
+            // single chains.
+            constexpr int kLookahead = 4;
+            candidates(left, &cands);
+            std::stable_sort(cands.begin(), cands.end(),
+                             [](const Cand& x, const Cand& y) { return x.work > y.work; });
+            int best = -1;
+            for (int c = 0; c < (int)cands.size() && c < kLookahead; ++c) {
+                uint64_t t = cands[c].tile;
+                trial.clear();
+                scan(left, t, cands[c].extend, left.size(), &trial);
+                rest.clear();
+                size_t k = 0;
+                for (size_t i : left) {
+                    if (k < trial.size() && trial[k] == i) ++k;
+                    else rest.push_back(i);
                 }
-                grow |= 1ull << gq;
+                int after = 0;
+                if (!rest.empty()) {
+                    candidates(rest, &next);
+                    for (const Cand& x : next) after = std::max(after, x.work);
+                }
+                if (cands[c].work + after > best) {
+                    best = cands[c].work + after;
+                    tile = cands[c].tile;
+                    extend = cands[c].extend;
+                }
             }
-            consider(grow);
-            const int F = K - L;   // tile qubits above the low ones
-            auto run = [](int a, int w) { return ((w >= 64 ? ~0ull : ((1ull << w) - 1)) << a); };
-            for (int a = L; a + F <= nq; ++a) consider(low | run(a, F));
-            for (int w1 = 1; w1 < F; ++w1)
-                for (int a = L; a + w1 < nq; ++a)
-                    for (int b = a + w1 + 1; b + (F - w1) <= nq; ++b)
-                        consider(low | run(a, w1) | run(b, F - w1));
-            tile = best_tile;
         }
         std::vector<size_t> inc_idx;
-        scan(tile, extend, left.size(), &inc_idx);
+        scan(left, tile, extend, left.size(), &inc_idx);
         std::vector<const Item*> inc;
         inc.reserve(inc_idx.size());
         for (size_t i : inc_idx) inc.push_back(&items[i]);
