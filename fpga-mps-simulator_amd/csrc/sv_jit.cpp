@@ -280,6 +280,15 @@ std::string pass_body(const SvPlan& plan, const DevPass& P) {
         int r0, r1, qa, qb, aux;
         uint64_t cm;
         head_constants(G, &r0, &r1, &qa, &qb, &aux, &cm);
+        if (G.kind >= DG_ROTS && aux != 0 && P.R == SV_REG_BITS) {
+            // rotations, merged output factors and the register-only sign mask in one body
+            std::snprintf(line, sizeof(line),
+                          "    { const OpHead H = op_head(gt + %d, %d, 0, 0, 0, 0, %d, 0ull); "
+                          "rots_body_signed<R, %d, %uu>(a, H.m, gt[%d].m + 4); }\n",
+                          i, G.kind, aux, G.kind - DG_ROTS + 1, (unsigned)aux, i);
+            s += line;
+            continue;
+        }
         std::snprintf(line, sizeof(line),
                       "    { const OpHead H = op_head(gt + %d, %d, %d, %d, %d, %d, %d, %lluull); "
                       "op_body<R>(a, Lcur, H, gt[%d].m, %d, tfactors, tile_factors, ntiles, base, lds, t); }\n",

@@ -270,6 +270,38 @@ __device__ __forceinline__ void rots_body(cd2 (&a)[1 << R], const double* t,
     }
 }
 
+// A DG_ROTS op followed by its register-only sign mask S (the op's aux word), both known
+// when a per-plan kernel is compiled.  Amplitude j with a merged output factor takes the
+// sign on the factor, cmul(-f_j, a_j) = -cmul(f_j, a_j) exactly (up to the sign of a zero
+// result), so the negation is a free modifier of the factor's scalar operands.  Left to
+// itself the compiler keeps -a_j pending and, where the next op's three-address FMAs or a
+// transpose cannot take a modifier, materialises it with two instructions per double.
+// Amplitudes without a factor are multiplied by -1.0 from an SGPR the compiler cannot
+// fold, one instruction per double, as the interpreter's apply_sign_reg does.
+template <int R, int M, unsigned S>
+__device__ __forceinline__ void rots_body_signed(cd2 (&a)[1 << R], const double* t,
+                                                 const double* __restrict__ f) {
+    const cd2 one = mk2(1.0, 0.0);
+    if constexpr ((M & 1) != 0) rot_bit<R, 0, false, false>(a, t[0], one);
+    if constexpr ((M & 2) != 0 && R > 1) rot_bit<R, (R > 1 ? 1 : 0), false, false>(a, t[1], one);
+    if constexpr ((M & 4) != 0 && R > 2) rot_bit<R, (R > 2 ? 2 : 0), false, false>(a, t[2], one);
+    if constexpr ((M & 8) != 0 && R > 3) rot_bit<R, (R > 3 ? 3 : 0), false, false>(a, t[3], one);
+    __builtin_amdgcn_sched_barrier(0);
+    double m1;
+    asm("s_mov_b64 %0, -1.0" : "=s"(m1));
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+        const bool neg = (S >> j) & 1u;
+        if ((j & M) != 0) {
+            const cd2 fj = mk2(f[2 * j - 2], f[2 * j - 1]);
+            a[j] = cmul(neg ? mk2(-fj.x, -fj.y) : fj, a[j]);
+        } else if (neg) {
+            a[j] = a[j] * m1;
+        }
+        if (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 #define QS_ROTS_CASE(M) \
     case DG_ROTS + M - 1: QS_CASE("rots " #M " <"); rots_body<R, M>(a, t, f); QS_CASE("rots " #M " >"); break;
 
@@ -450,18 +482,16 @@ __device__ __forceinline__ void apply_phase_sign(cd2 (&a)[1 << R], unsigned M,
         if (R > 2 && (bits & (4u << SV_THREAD_BITS))) M ^= reg_col_mask(2);
         if (R > 3 && (bits & (8u << SV_THREAD_BITS))) M ^= reg_col_mask(3);
     }
-    // Factor +-1 of amplitude j: bit j of M moved to the sign bit of 1.0's high word (an
-    // AND and one shift-OR; left to itself the compiler splits the shift-OR in two),
-    // then two FP64 products.  The products write fresh registers whole; flipping only
-    // the high words would copy the low ones.
-    const unsigned one_hi = 0x3FF00000u;
+    // Factor +-1 of amplitude j: bit j of M shifted to bit 31 and XOR-ed into the sign
+    // bits of the real and imaginary high words (one shift and two three-input bit ops,
+    // v_bitop3, per amplitude; exact negation, as a product with -1.0 is).  Products
+    // with +-1.0 took five instructions per amplitude: the factor's high word, a move of
+    // its zero low word into a register pair, and two FP64 products.
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
-        unsigned hi;
-        asm("v_lshl_or_b32 %0, %1, %2, %3"
-            : "=v"(hi)
-            : "v"(M & (1u << j)), "i"(31 - j), "s"(one_hi));
-        a[j] = a[j] * __hiloint2double((int)hi, 0);
+        const int s = (int)((M << (31 - j)) & 0x80000000u);
+        a[j].x = __hiloint2double(__double2hiint(a[j].x) ^ s, __double2loint(a[j].x));
+        a[j].y = __hiloint2double(__double2hiint(a[j].y) ^ s, __double2loint(a[j].y));
     }
 }
 
