@@ -265,15 +265,31 @@ std::string pass_body(const SvPlan& plan, const DevPass& P) {
             continue;
         }
         if (G.kind == DG_PHASE) {
-            // op_body's DG_PHASE branch with the table entry loaded ahead
+            // op_body's DG_PHASE branch with the table entry loaded ahead and the factors
+            // that are exactly 1 for every thread and tile left out of the products: a
+            // table column that is the same 1 for every thread (unless block factors
+            // multiply it), a register-pair factor of the record that is 1.
+            unsigned ones = 0;
+            for (int k = 0; k < PHASE_THREAD_VALUES; ++k) {
+                const size_t e = (size_t)G.qb * PHASE_THREAD_VALUES + k;
+                const double* v = plan.thread_factors.data() + 2 * e;
+                if (thread_dependence(plan, e, nthr) == -1 && v[0] == 1.0 && v[1] == 0.0)
+                    ones |= 1u << k;
+            }
+            if (G.aux >= 0) {
+                ones &= ~1u;                              // A times the block factor A_B
+                if (G.r0 & 2) ones &= 1u;                 // kappa_r times their block factors
+            }
+            for (int k = 0; k < 6; ++k)
+                if (G.m[2 * k] == 1.0 && G.m[2 * k + 1] == 0.0) ones |= 1u << (5 + k);
             std::snprintf(line, sizeof(line),
                           "    { const OpHead H = op_head(gt + %d, %d, %d, 0, 0, %d, %d, 0ull); "
                           "const const_cd2_ptr bf = %s; "
-                          "apply_phase_cplx<R>(a, H.m, ph%d, bf, t, %d); }\n",
+                          "apply_phase_cplx_known<R, %uu>(a, H.m, ph%d, bf, t, %d); }\n",
                           i, G.kind, G.r0, G.qb, G.aux,
                           G.aux >= 0 ? "(const_cd2_ptr)(tile_factors + (size_t)H.aux * ntiles * PHASE_BLOCK_VALUES)"
                                      : "nullptr",
-                          i, G.r0);
+                          ones, i, G.r0);
             s += line;
             continue;
         }

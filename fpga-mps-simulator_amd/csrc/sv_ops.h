@@ -562,6 +562,92 @@ __device__ __forceinline__ void apply_phase_cplx(cd2 (&a)[1 << R], const double*
     }
 }
 
+// Product a * b where a (OA) or b (OB) is known, when the kernel is compiled, to be
+// exactly 1: the other operand, which equals cmul(a, b) up to the sign of a zero part.
+template <bool OA, bool OB>
+__device__ __forceinline__ cd2 mul1(cd2 a, cd2 b) {
+    if constexpr (OA) return b;
+    else if constexpr (OB) return a;
+    else return cmul(a, b);
+}
+
+// apply_phase_cplx for a per-plan kernel that knows which factors are exactly 1 (bit 0:
+// A after its block factors, bits 1..4: kappa_0..3 after theirs, bits 5..10: the register
+// pair factors c01 c02 c03 c12 c13 c23).  The same products in the same order with the
+// products by 1 left out: a QFT group (controlled phases after one H) has one thread
+// column and one register qubit's pair factors that are not 1.
+template <int R, unsigned O>
+__device__ __forceinline__ void apply_phase_cplx_known(cd2 (&a)[1 << R], const double* crr,
+                                                       const cd2 (&e)[PHASE_THREAD_VALUES],
+                                                       const_cd2_ptr bf, int t, int parts) {
+    cd2 A = e[0], k0 = e[1], k1 = e[2], k2 = e[3], k3 = e[4];
+    if (bf) {
+        A = cmul(A, bf[0]);
+        if (parts & 1) {
+#pragma unroll
+            for (int q = 0; q < SV_THREAD_BITS; ++q) {
+                const cd2 Af = cmul(A, bf[1 + q]);
+                A = ((t >> q) & 1) ? Af : A;
+            }
+        }
+        if (parts & 2) {
+            k0 = cmul(k0, bf[PHASE_KAPPA0]);
+            if (R > 1) k1 = cmul(k1, bf[PHASE_KAPPA0 + 1]);
+            if (R > 2) k2 = cmul(k2, bf[PHASE_KAPPA0 + 2]);
+            if (R > 3) k3 = cmul(k3, bf[PHASE_KAPPA0 + 3]);
+        }
+    }
+    constexpr bool oA = O & 1, o0 = (O >> 1) & 1, o1 = (O >> 2) & 1, o2 = (O >> 3) & 1,
+                   o3 = (O >> 4) & 1;
+    constexpr bool p01 = (O >> 5) & 1, p02 = (O >> 6) & 1, p03 = (O >> 7) & 1,
+                   p12 = (O >> 8) & 1, p13 = (O >> 9) & 1, p23 = (O >> 10) & 1;
+    const cd2 c01 = mk2(crr[0], crr[1]), c02 = mk2(crr[2], crr[3]);
+    const cd2 c03 = mk2(crr[4], crr[5]), c12 = mk2(crr[6], crr[7]);
+    const cd2 c13 = mk2(crr[8], crr[9]), c23 = mk2(crr[10], crr[11]);
+    constexpr bool og1 = oA && o0;
+    const cd2 g0 = A, g1 = mul1<oA, o0>(A, k0);
+    if constexpr (!oA) a[0] = cmul(g0, a[0]);
+    if constexpr (!og1) a[1] = cmul(g1, a[1]);
+    if constexpr (R > 1) {
+        constexpr bool ok1c = o1 && p01, og2 = oA && o1, og3 = og1 && ok1c;
+        const cd2 g2 = mul1<oA, o1>(A, k1), g3 = mul1<og1, ok1c>(g1, mul1<o1, p01>(k1, c01));
+        if constexpr (!og2) a[2] = cmul(g2, a[2]);
+        if constexpr (!og3) a[3] = cmul(g3, a[3]);
+        if constexpr (R > 2) {
+            __builtin_amdgcn_sched_barrier(0);
+            constexpr bool ow1 = o2 && p02, ow2 = o2 && p12, ow3 = ow1 && p12;
+            const cd2 w1 = mul1<o2, p02>(k2, c02), w2 = mul1<o2, p12>(k2, c12);
+            const cd2 w3 = mul1<ow1, p12>(w1, c12);
+            constexpr bool og4 = oA && o2, og5 = og1 && ow1, og6 = og2 && ow2, og7 = og3 && ow3;
+            const cd2 g4 = mul1<oA, o2>(A, k2), g5 = mul1<og1, ow1>(g1, w1);
+            const cd2 g6 = mul1<og2, ow2>(g2, w2), g7 = mul1<og3, ow3>(g3, w3);
+            if constexpr (!og4) a[4] = cmul(g4, a[4]);
+            if constexpr (!og5) a[5] = cmul(g5, a[5]);
+            if constexpr (!og6) a[6] = cmul(g6, a[6]);
+            if constexpr (!og7) a[7] = cmul(g7, a[7]);
+            if constexpr (R > 3) {
+                __builtin_amdgcn_sched_barrier(0);
+                constexpr bool ov1 = o3 && p03, ov2 = o3 && p13, ov4 = o3 && p23;
+                constexpr bool ov3 = ov1 && p13, ov5 = ov1 && p23, ov6 = ov2 && p23,
+                               ov7 = ov3 && p23;
+                const cd2 v1 = mul1<o3, p03>(k3, c03), v2 = mul1<o3, p13>(k3, c13);
+                const cd2 v4 = mul1<o3, p23>(k3, c23);
+                const cd2 v3 = mul1<ov1, p13>(v1, c13), v5 = mul1<ov1, p23>(v1, c23);
+                const cd2 v6 = mul1<ov2, p23>(v2, c23), v7 = mul1<ov3, p23>(v3, c23);
+                if constexpr (!(oA && o3)) a[8] = cmul(mul1<oA, o3>(A, k3), a[8]);
+                if constexpr (!(og1 && ov1)) a[9] = cmul(mul1<og1, ov1>(g1, v1), a[9]);
+                if constexpr (!(og2 && ov2)) a[10] = cmul(mul1<og2, ov2>(g2, v2), a[10]);
+                if constexpr (!(og3 && ov3)) a[11] = cmul(mul1<og3, ov3>(g3, v3), a[11]);
+                __builtin_amdgcn_sched_barrier(0);
+                if constexpr (!(og4 && ov4)) a[12] = cmul(mul1<og4, ov4>(g4, v4), a[12]);
+                if constexpr (!(og5 && ov5)) a[13] = cmul(mul1<og5, ov5>(g5, v5), a[13]);
+                if constexpr (!(og6 && ov6)) a[14] = cmul(mul1<og6, ov6>(g6, v6), a[14]);
+                if constexpr (!(og7 && ov7)) a[15] = cmul(mul1<og7, ov7>(g7, v7), a[15]);
+            }
+        }
+    }
+}
+
 // LDS barrier for the tile transposes: waits for this wave's LDS accesses, then syncs
 // the workgroup.  The fences are workgroup-scope and restricted to the local (LDS)
 // address space, so the barrier never waits for global memory operations in flight
