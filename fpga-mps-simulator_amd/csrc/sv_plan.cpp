@@ -961,6 +961,12 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
     // Indices of the items not yet placed, in gate order.
     std::vector<size_t> left(items.size());
     for (size_t i = 0; i < items.size(); ++i) left[i] = i;
+    const uint64_t all_q = (nq >= 64) ? ~0ull : ((1ull << nq) - 1);
+    // Items examined by the scans so far.  Once kSearchBudget is spent (long circuits of
+    // many small passes, such as random gates on many qubits), the remaining passes take
+    // the first-come tile, which costs one scan.
+    constexpr size_t kSearchBudget = size_t(64) << 20;
+    size_t scanned = 0;
     // One pass over the pending items `lst` in gate order: an item joins the pass unless
     // it must wait for a deferred item it does not commute with, or (non-diagonal) one of
     // its targets is not a tile qubit; with `extend` such a target joins the tile while
@@ -973,7 +979,8 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
         uint64_t blocked_nd = 0;    // qubits of deferred diagonal gates (block non-diagonal)
         int work = 0;
         const size_t n = std::min(limit, lst.size());
-        for (size_t k = 0; k < n; ++k) {
+        size_t k = 0;
+        for (; k < n; ++k) {
             const Item& it = items[lst[k]];
             bool defer = it.diag ? (it.touch & blocked_any) != 0
                                  : (it.touch & (blocked_any | blocked_nd)) != 0;
@@ -988,11 +995,17 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
                     blocked_nd |= it.touch;
                 else
                     blocked_any |= it.touch;
+                // Nothing later joins once every qubit is blocked for every item (or, when
+                // only the work is counted, for every non-diagonal item).
+                if ((blocked_any & all_q) == all_q ||
+                    (!inc && ((blocked_any | blocked_nd) & all_q) == all_q))
+                    break;
                 continue;
             }
             if (!it.diag) ++work;
             if (inc) inc->push_back(lst[k]);
         }
+        scanned += k;
         return work;
     };
     // Candidate tiles of a pass over `lst`, with the work each admits: the tile the scan
@@ -1044,7 +1057,7 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan) {
         uint64_t tile = low;
         if (P.tile_is_all) tile = (nq >= 64) ? ~0ull : ((1ull << nq) - 1);
         bool extend = true;
-        if (!P.tile_is_all) {
+        if (!P.tile_is_all && scanned < kSearchBudget) {
             // The tile whose pass admits the most non-diagonal gates together with the best
             // pass after it, over the kLookahead candidates of most work (ties: the earlier
             // candidate, the first-come tile first).  Diagonal gates run under any tile, so
