@@ -334,6 +334,16 @@ int qs_sv_plan_create(int nq, const qs_gate* gates, int ng, qs_sv_plan** plan_ou
             st = QS_E_HIP;
         }
         p->bytes = L.total;
+        // The block-factor tables depend on the plan alone: written once here, read by
+        // every launch.
+        if (st == QS_OK) st = sv_plan_block_factors(nq, p->plan, p->dev, nullptr);
+        if (st == QS_OK) {
+            e = hipStreamSynchronize(nullptr);
+            if (e != hipSuccess) {
+                set_error("block-factor tables failed: %s", hipGetErrorString(e));
+                st = QS_E_HIP;
+            }
+        }
     }
     if (st != QS_OK) {
         if (p->dev) (void)hipFree(p->dev);
@@ -355,7 +365,8 @@ int qs_sv_plan_launch(const qs_sv_plan* plan, void* state, void* stream) {
     QS_REQUIRE(dev == plan->device, QS_E_ARG, "plan lives on device %d, current device is %d",
                plan->device, dev);
     return sv_launch_plan(static_cast<double2*>(state), plan->nq, plan->plan, plan->dev,
-                          as_stream(stream), plan->spec.empty() ? nullptr : plan->spec.data());
+                          as_stream(stream), plan->spec.empty() ? nullptr : plan->spec.data(),
+                          false);
 }
 
 int qs_sv_plan_specialize(qs_sv_plan* plan, int* kernels_compiled) {

@@ -98,8 +98,15 @@ int sv_build_plan(int nq, const qs_gate* gates, int ng, SvPlan* plan);
 // Launches every pass of `plan` (already uploaded to `dev_blob`) on `stream`.  spec, when
 // not null, holds per pass a kernel compiled for that pass (sv_jit_specialize) or null
 // for the op-stream interpreter.
+// With block_factors, each pass's block-factor table is written by its prologue kernel
+// first; a plan whose table was written once (sv_plan_block_factors) passes false.
 int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_blob,
-                   hipStream_t stream, const hipFunction_t* spec = nullptr);
+                   hipStream_t stream, const hipFunction_t* spec = nullptr,
+                   bool block_factors = true);
+
+// Writes the block-factor tables of every pass of an uploaded plan on `stream` (one
+// launch per pass with block-class phase terms).  They depend on the plan only.
+int sv_plan_block_factors(int nq, const SvPlan& plan, const void* dev_blob, hipStream_t stream);
 
 // Per-plan pass kernels (sv_jit.cpp).  sv_jit_specialize compiles, with hipRTC for
 // gfx950, the kernels of the plan's passes that the process-wide cache lacks, loads them

@@ -198,8 +198,27 @@ struct PassTimer {
 };
 thread_local PassTimer g_timer;
 
+int sv_plan_block_factors(int nq, const SvPlan& plan, const void* dev_blob, hipStream_t stream) {
+    const char* blob = reinterpret_cast<const char*>(dev_blob);
+    const SvBlobLayout L = sv_blob_layout(plan, nq);
+    const DevPass* dpasses = reinterpret_cast<const DevPass*>(blob + L.passes);
+    const DevPhase* dphases = reinterpret_cast<const DevPhase*>(blob + L.phases);
+    const DevTerm* dterms = reinterpret_cast<const DevTerm*>(blob + L.terms);
+    cd2* dfactors = reinterpret_cast<cd2*>(const_cast<char*>(blob) + L.factors);
+    for (size_t p = 0; p < plan.passes.size(); ++p) {
+        const DevPass& P = plan.passes[p];
+        if (P.nslot <= 0) continue;
+        const unsigned tiles = P.tile_is_all ? 1u : (1u << (nq - P.K));
+        const uint64_t work = (uint64_t)P.nslot * tiles;
+        hipLaunchKernelGGL(sv_phase_block_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256),
+                           0, stream, nq, dpasses + p, dphases, dterms, dfactors, tiles);
+        QS_HIP(hipGetLastError());
+    }
+    return QS_OK;
+}
+
 int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_blob,
-                   hipStream_t stream, const hipFunction_t* spec) {
+                   hipStream_t stream, const hipFunction_t* spec, bool block_factors) {
     const char* blob = reinterpret_cast<const char*>(dev_blob);
     const SvBlobLayout L = sv_blob_layout(plan, nq);
     const DevPass* dpasses = reinterpret_cast<const DevPass*>(blob + L.passes);
@@ -215,7 +234,7 @@ int sv_launch_plan(double2* state, int nq, const SvPlan& plan, const void* dev_b
         const unsigned tiles = P.tile_is_all ? 1u : (1u << (nq - P.K));
         const unsigned threads = 1u << (P.K - P.R);
         const unsigned grid = tiles;
-        if (P.nslot > 0) {
+        if (block_factors && P.nslot > 0) {
             const uint64_t work = (uint64_t)P.nslot * tiles;
             hipLaunchKernelGGL(sv_phase_block_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256),
                                0, stream, nq, dP, dphases, dterms, dfactors, tiles);

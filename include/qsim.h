@@ -108,8 +108,9 @@ int qs_sv_apply_words(void* state, int nq, const int32_t* gate_sequence, int num
  * current device owned by the plan.  qs_sv_plan_launch applies the planned gates to
  * `state` on `stream`: it plans nothing, copies nothing from the host and allocates
  * nothing, so it can be recorded into a HIP graph; its result equals qs_sv_apply's bit for
- * bit.  A plan keeps a block-factor table that its launches rewrite, so one plan must not
- * be launched on two streams at once.  qs_sv_plan_info reports the qubit count, the
+ * bit.  The plan's block-factor table is written once by qs_sv_plan_create and only read by
+ * launches, so one plan may be launched on several streams at once (on different
+ * states).  qs_sv_plan_info reports the qubit count, the
  * passes and the device bytes the plan holds.  qs_sv_plan_destroy frees it (hipFree: it
  * waits for the device, so launches in flight finish first); NULL is accepted. */
 typedef struct qs_sv_plan qs_sv_plan;
