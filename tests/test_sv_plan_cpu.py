@@ -124,6 +124,27 @@ def test_plan_u3_cz_layers_pass_count():
     assert len(plan['passes']) <= 2 * 4 + 2
 
 
+def test_benchmark_plan_takes_light_cone_windows():
+    """The 28-qubit benchmark circuit (20 x [U3 on every qubit + CZ ladder] + QFT-28): the
+    tile choice advances light cones over windows of consecutive qubits, 17 passes instead
+    of the 31 that first-come tiles take, and every one-qubit rotation lands in exactly one
+    pass, on a tile qubit."""
+    _lib_or_skip()
+    from qsim_amd import workloads
+    nq = 28
+    recs, _ = gates_to_records(workloads.sv_benchmark_circuit(nq, 20, 1234), nq)
+    plan = plan_json(nq, recs)
+    assert len(plan['passes']) <= 17
+    nrot = 0
+    for p in plan['passes']:
+        assert p['K'] == 12 and p['tq'][:4] == [0, 1, 2, 3] and len(set(p['tq'])) == 12
+        for s in p['stages']:
+            for g in s['gates']:
+                k = g['kind']
+                nrot += 1 if 16 <= k < 32 else (bin(k - 31).count('1') if k >= 32 else 0)
+    assert nrot == 20 * nq + nq           # the U3 layers and the QFT's Hadamards
+
+
 def test_plan_qft_roundtrip_and_phase():
     _lib_or_skip()
     nq = 14
