@@ -72,10 +72,54 @@ def parse():
     ap.add_argument('--calibration-copy', action='store_true',
                     help='after the SV leg, copy the 4 GiB state once (a copy kernel of '
                          'known size for scaling the HBM byte counters of a profile)')
+    ap.add_argument('--sv-sizes', default='', metavar='N,N,..',
+                    help='instead of the headline line, print one JSON line per state size: '
+                         'the statevector workload structure (20 x [U3 all + CZ ladder] + '
+                         'QFT) on N qubits, per-plan kernels, --steps timed launches')
     ap.add_argument('--interpreter', action='store_true',
                     help='run the plan on the op-stream interpreter kernel instead of '
                          'kernels compiled for the plan')
     return ap.parse_args()
+
+
+def sv_size(nq: int, steps: int, warmup: int) -> dict:
+    """The statevector workload's structure at nq qubits: plan, per-plan kernels, `steps`
+    timed launches of the prebuilt plan (HIP events on the launch stream)."""
+    import torch
+    from qsim_amd import workloads
+    from qsim_amd.sv import StateVector, gates_to_records
+
+    gates = workloads.sv_benchmark_circuit(nq, SV_DEPTH, SV_SEED)
+    recs, _ = gates_to_records(gates, nq)
+    sv = StateVector(nq, 0)
+    t0 = time.perf_counter()
+    plan = sv.plan(recs)
+    plan_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan.specialize()
+    jit_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        sv.launch(plan)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        sv.launch(plan)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    per_pass_s = ms / 1e3 / plan.npasses
+    out = {'workload': f'{nq}q statevector: {SV_DEPTH}x[U3 all qubits + CZ ladder] + '
+                       f'QFT-{nq} ({len(gates)} gates), complex128, state resident in HBM',
+           'qubits': nq, 'gates': len(gates), 'passes_per_step': plan.npasses,
+           'steps': steps, 'ms_per_step': ms, 'gates_per_s': len(gates) / (ms / 1e3),
+           'per_pass_ms': per_pass_s * 1e3,
+           'per_pass_gbps': 2 * 16 * (1 << nq) / per_pass_s / 1e9,
+           'plan_ms': plan_s * 1e3, 'jit_ms': jit_s * 1e3, 'norm_sq_after': sv.norm_sq()}
+    del sv, plan
+    torch.cuda.empty_cache()
+    return out
 
 
 def sv_leg(steps: int, warmup: int, calibration_copy: bool = False,
@@ -423,6 +467,10 @@ def main():
     if args.multi > 0:
         from qsim_amd import mps_bench
         print(json.dumps(mps_bench.run_multi(args.multi)))
+        return
+    if args.sv_sizes:
+        for nq in (int(x) for x in args.sv_sizes.split(',')):
+            print(json.dumps(sv_size(nq, args.steps, args.warmup)), flush=True)
         return
     sv = sv_leg(args.steps, args.warmup, args.calibration_copy,
                 specialize=not args.interpreter)
