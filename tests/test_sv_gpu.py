@@ -401,7 +401,9 @@ def test_plan_launch_equals_apply_bitwise_and_replays_in_a_graph():
 
 def test_28q_depth20_benchmark_circuit_inverse_roundtrip():
     """The full 28-qubit benchmark circuit (20 x [U3 + CZ ladder] + QFT-28, 1520 gates)
-    through a prebuilt plan, then its inverse: back to |0..0> and unit norm."""
+    through a prebuilt plan run by its per-plan kernels (the benchmark's path): equal to the
+    op-stream interpreter's run of the same plan, then its inverse: back to |0..0> and
+    unit norm."""
     import torch
     from qsim_amd.sv import StateVector, gates_to_records
     nq = 28
@@ -410,7 +412,15 @@ def test_28q_depth20_benchmark_circuit_inverse_roundtrip():
     inv, _ = gates_to_records(workloads.inverse_circuit(g), nq)
     sv = StateVector(nq, 0)
     fwd = sv.plan(recs)
+    fwd.specialize()
     sv.launch(fwd)
+    ref = StateVector(nq, 0)
+    ref_plan = ref.plan(recs)
+    ref.launch(ref_plan)
+    torch.cuda.synchronize()
+    assert torch.equal(sv.tensor, ref.tensor)
+    del ref
+    ref_plan.destroy()
     assert abs(sv.norm_sq() - 1.0) < 1e-10
     sv.apply_records(inv)
     torch.cuda.synchronize()
