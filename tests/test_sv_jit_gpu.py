@@ -25,7 +25,7 @@ def _pair(nq, recs):
 
 
 @pytest.mark.parametrize('nq,ng,seed', [(1, 20, 0), (4, 120, 1), (9, 250, 2), (12, 300, 3),
-                                        (14, 300, 4), (16, 300, 5)])
+                                        (14, 300, 4), (16, 300, 5), (24, 300, 9)])
 def test_random_circuit_specialized_equals_interpreter(nq, ng, seed):
     import torch
     from qsim_amd.sv import gates_to_records
