@@ -261,10 +261,11 @@ class MPS:
         self.trace.append((i, k, dlog))
 
     # -- environments and sampling (mps_core.py:354-429) ---------------------
-    def right_environments(self) -> List[np.ndarray]:
+    def right_environments(self, stop: int = 0) -> List[np.ndarray]:
+        """R_i for i >= stop (entries below `stop` stay None; 0 gives all of them)."""
         envs: List[Optional[np.ndarray]] = [None] * (self.n + 1)
         envs[self.n] = np.ones((1, 1), dtype=np.complex128)
-        for i in range(self.n - 1, -1, -1):
+        for i in range(self.n - 1, stop - 1, -1):
             a = self.sites[i]
             r = np.einsum('asb,bc,dsc->ad', a, envs[i + 1], a.conj())
             tr = float(np.real(np.trace(r)))
@@ -274,16 +275,18 @@ class MPS:
         return envs  # type: ignore[return-value]
 
     def sample(self, shots: int, measured: Optional[List[int]] = None,
-               rng: Optional[np.random.Generator] = None) -> List[str]:
+               rng: Optional[np.random.Generator] = None,
+               envs: Optional[List[np.ndarray]] = None) -> List[str]:
         """Sequential conditional sampling (mps_core.py:371-420); bitstring[0] is qubit n-1.
 
         With a measured subset the key is built as mps_core.py:415-417 builds it: for q
         in the measured sites in descending order, the outcome of site n-1-q (the
-        mirrored site, not site q itself).
+        mirrored site, not site q itself).  `envs` replaces the right environments
+        (right_environments() when None).
         """
         rng = rng if rng is not None else np.random.default_rng()
         measured = list(range(self.n)) if measured is None else list(measured)
-        envs = self.right_environments()
+        envs = self.right_environments() if envs is None else envs
         out = []
         full = set(measured) == set(range(self.n))
         for _ in range(shots):
