@@ -202,6 +202,29 @@ def test_sampling_marginals():
     assert chi2 < 2 * (1 << n) + 30
 
 
+def test_partial_environments_and_given_envs():
+    """right_environments(stop=k) holds R_i for i >= k exactly as the full recursion does
+    (None below k), and sample(envs=...) with the chain's own environments draws the same
+    shots as sample() from the same generator (the bench's CPU sampling leg times both)."""
+    n = 8
+    rng = np.random.default_rng(4)
+    m = mps_ref.MPS(n, 16, 1e-14)
+    for _ in range(3):
+        for i in range(n - 1):
+            m.apply_2q(i, i + 1, mps_ref.haar_4x4(rng))
+    full = m.right_environments()
+    part = m.right_environments(stop=5)
+    assert all(e is None for e in part[:5])
+    for i in range(5, n + 1):
+        assert np.array_equal(part[i], full[i])
+    assert m.sample(50, None, np.random.default_rng(9), envs=full) == \
+        m.sample(50, None, np.random.default_rng(9))
+    # identity environments are a valid (different) conditional model of the same shapes
+    eye = [np.eye(a.shape[0], dtype=np.complex128) for a in m.sites] + [full[n]]
+    shots = m.sample(20, None, np.random.default_rng(9), envs=eye)
+    assert len(shots) == 20 and all(len(x) == n and set(x) <= {'0', '1'} for x in shots)
+
+
 def test_pauli_expectation_closed_form():
     sv = np.zeros(8, dtype=complex)
     sv[0b011] = 1                     # qubits 0,1 = 1
